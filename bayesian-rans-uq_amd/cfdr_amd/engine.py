@@ -1,0 +1,374 @@
+"""ctypes binding of the gfx950 engine (include/rans_amd.h) and the
+ForwardBatch front end.
+
+``ForwardBatch`` mirrors the reference's batch forward model and its Python
+binding (rans-uq/include/ForwardBatch.hpp:23-125,
+rans-uq/python/bindings.cpp:830-916): the constructor deep-copies the
+problem, ``evaluate(theta, workers)`` returns one row per coefficient vector
+with the reference's status, log-likelihood and NaN conventions. All rows run
+on the GPU in one persistent kernel launch. The engine library must be built
+(``__graft_entry__.build()``); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .config import (PATCH_TYPES_ORDER, SCHEMES, SOLVERS, STATUS_NAMES, FlowBoundaryConditions,
+                     InferenceParameterSet, ObservationOperator, SolverSettings, Vec3,
+                     validate_coefficients)
+from .mesh import Mesh
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librans_amd.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
+
+
+class RqMesh(C.Structure):
+    _fields_ = [("n_cells", C.c_int32), ("n_internal", C.c_int32), ("n_faces", C.c_int32),
+                ("n_patches", C.c_int32), ("owner", _ip), ("neighbor", _ip),
+                ("face_center", _dp), ("face_normal", _dp), ("face_area", _dp),
+                ("face_delta", _dp), ("face_d", _dp), ("face_weight", _dp), ("face_shift", _dp),
+                ("cell_center", _dp), ("cell_volume", _dp), ("wall_distance", _dp),
+                ("cell_face_ptr", _ip), ("cell_face_idx", _ip), ("patch_type", _ip),
+                ("patch_face_ptr", _ip), ("patch_face_idx", _ip)]
+
+
+class RqBc(C.Structure):
+    _fields_ = [("u_type", _ip), ("p_type", _ip), ("k_type", _ip), ("w_type", _ip),
+                ("u_val", _dp), ("p_val", _dp), ("k_val", _dp), ("w_val", _dp)]
+
+
+class RqSettings(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("convergence_tol", C.c_double),
+                ("divergence_limit", C.c_double), ("convergence_window", C.c_int32),
+                ("monitor_interval", C.c_int32), ("alpha_u", C.c_double),
+                ("alpha_p", C.c_double), ("alpha_k", C.c_double), ("alpha_omega", C.c_double),
+                ("inner_iterations", C.c_int32), ("inner_tolerance", C.c_double),
+                ("turb_start_iter", C.c_int32), ("turb_update_interval", C.c_int32),
+                ("nut_floor_iters", C.c_int32), ("momentum_convection", C.c_int32),
+                ("turbulence_convection", C.c_int32), ("body_force_x", C.c_double),
+                ("body_force_y", C.c_double), ("k_min", C.c_double), ("k_max", C.c_double),
+                ("omega_min", C.c_double), ("pressure_solver", C.c_int32),
+                ("momentum_solver", C.c_int32), ("turbulence_solver", C.c_int32),
+                ("sst_variant", C.c_int32), ("mixed_precision", C.c_int32),
+                ("block_size", C.c_int32)]
+
+
+class RqParams(C.Structure):
+    _fields_ = [("n_active", C.c_int32), ("active_indices", _ip), ("base", _dp),
+                ("lower", _dp), ("upper", _dp)]
+
+
+class RqObs(C.Structure):
+    _fields_ = [("n_obs", C.c_int32), ("type", _ip), ("component", _ip), ("observed", _dp),
+                ("variance", _dp), ("ref_velocity", _dp), ("ref_pressure", _dp),
+                ("ref_density", _dp), ("cell", _ip), ("offx", _dp), ("offy", _dp),
+                ("axx", _dp), ("axy", _dp), ("ayy", _dp), ("det", _dp), ("nb_ptr", _ip),
+                ("nb_cell", _ip), ("nb_face", _ip), ("nb_dx", _dp), ("nb_dy", _dp),
+                ("nb_w", _dp), ("face_a", _ip), ("face_b", _ip), ("t", _dp)]
+
+
+class RqBatchOut(C.Structure):
+    _fields_ = [("status", _ip), ("loglik", _dp), ("predictions", _dp), ("iterations", _ip),
+                ("final_worst_measure", _dp), ("contraction_rate_u", _dp), ("stalled", _u8p),
+                ("converged", _u8p), ("row_errors", C.c_char_p), ("row_error_stride", C.c_size_t),
+                ("fields", _dp), ("krylov_iterations", _i64p)]
+
+
+EXPORTS = ["rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
+           "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
+           "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info"]
+
+_lib = None
+
+
+class EngineMissing(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the engine library; raises EngineMissing when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineMissing(f"engine library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    lib.rq_engine_create.argtypes = [C.POINTER(RqMesh), C.POINTER(RqParams), C.POINTER(RqObs),
+                                     C.POINTER(RqBc), C.c_double, C.POINTER(RqSettings), _dp,
+                                     C.c_char_p, C.c_size_t]
+    lib.rq_engine_create.restype = C.c_void_p
+    lib.rq_engine_evaluate.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int32, C.c_int32,
+                                       C.POINTER(RqBatchOut), C.c_char_p, C.c_size_t]
+    lib.rq_engine_evaluate.restype = C.c_int
+    for name in ("rq_engine_n_observations", "rq_engine_n_parameters", "rq_engine_n_cells",
+                 "rq_engine_n_faces"):
+        getattr(lib, name).argtypes = [C.c_void_p]
+        getattr(lib, name).restype = C.c_int32
+    lib.rq_engine_last_timing.argtypes = [C.c_void_p, _dp, _ip, _ip, _i64p]
+    lib.rq_engine_last_timing.restype = C.c_int
+    lib.rq_engine_destroy.argtypes = [C.c_void_p]
+    lib.rq_engine_destroy.restype = None
+    lib.rq_build_info.argtypes = []
+    lib.rq_build_info.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+class _Keep:
+    def __init__(self):
+        self.items = []
+
+    def d(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.items.append(a)
+        return a.ctypes.data_as(_dp)
+
+    def i(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        self.items.append(a)
+        return a.ctypes.data_as(_ip)
+
+
+def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size: int = 0):
+    r = RqSettings()
+    r.max_iterations, r.convergence_tol, r.divergence_limit = s.max_iterations, \
+        s.convergence_tol, s.divergence_limit
+    r.convergence_window, r.monitor_interval = s.convergence_window, s.monitor_interval
+    r.alpha_u, r.alpha_p, r.alpha_k, r.alpha_omega = s.alpha_u, s.alpha_p, s.alpha_k, s.alpha_omega
+    r.inner_iterations, r.inner_tolerance = s.inner_iterations, s.inner_tolerance
+    r.turb_start_iter, r.turb_update_interval = s.turb_start_iter, s.turb_update_interval
+    r.nut_floor_iters = s.nut_floor_iters
+    r.momentum_convection = SCHEMES[s.momentum_convection]
+    r.turbulence_convection = SCHEMES[s.turbulence_convection]
+    r.body_force_x, r.body_force_y = s.body_force.x, s.body_force.y
+    r.k_min, r.k_max, r.omega_min = s.k_min, s.k_max, s.omega_min
+    r.pressure_solver = SOLVERS[s.pressure_solver]
+    r.momentum_solver = SOLVERS[s.momentum_solver]
+    r.turbulence_solver = SOLVERS[s.turbulence_solver]
+    r.sst_variant = s.sst_variant
+    r.mixed_precision = 1 if mixed_precision else 0
+    r.block_size = block_size
+    return r
+
+
+def mesh_struct(m: Mesh, keep: _Keep):
+    r = RqMesh()
+    r.n_cells, r.n_internal, r.n_faces, r.n_patches = m.n_cells, m.n_internal, m.n_faces, \
+        m.n_patches()
+    r.owner, r.neighbor = keep.i(m.owner), keep.i(m.neighbor)
+    r.face_center, r.face_normal = keep.d(m.face_center), keep.d(m.face_normal)
+    r.face_area, r.face_delta = keep.d(m.face_area), keep.d(m.face_delta)
+    r.face_d, r.face_weight, r.face_shift = keep.d(m.face_d), keep.d(m.face_weight), \
+        keep.d(m.face_shift)
+    r.cell_center, r.cell_volume = keep.d(m.cell_center), keep.d(m.cell_volume)
+    r.wall_distance = keep.d(m.wall_distance())
+    r.cell_face_ptr, r.cell_face_idx = keep.i(m.cell_face_ptr), keep.i(m.cell_face_idx)
+    r.patch_type = keep.i([PATCH_TYPES_ORDER.get(p.type, 4) for p in m.patches] or [0])
+    ptr = np.zeros(m.n_patches() + 1, np.int32)
+    ptr[1:] = np.cumsum([len(p.faces) for p in m.patches])
+    r.patch_face_ptr = keep.i(ptr)
+    r.patch_face_idx = keep.i(np.concatenate([p.faces for p in m.patches])
+                              if m.patches else np.zeros(1, np.int32))
+    return r
+
+
+def obs_struct(P, keep: _Keep):
+    r = RqObs()
+    r.n_obs = P.n_obs
+    r.type, r.component = keep.i(P.type), keep.i(P.component)
+    r.observed, r.variance = keep.d(P.observed), keep.d(P.variance)
+    r.ref_velocity, r.ref_pressure, r.ref_density = keep.d(P.ref_velocity), \
+        keep.d(P.ref_pressure), keep.d(P.ref_density)
+    r.cell, r.offx, r.offy = keep.i(P.cell), keep.d(P.offx), keep.d(P.offy)
+    r.axx, r.axy, r.ayy, r.det = keep.d(P.axx), keep.d(P.axy), keep.d(P.ayy), keep.d(P.det)
+    r.nb_ptr, r.nb_cell, r.nb_face = keep.i(P.nb_ptr), keep.i(P.nb_cell), keep.i(P.nb_face)
+    r.nb_dx, r.nb_dy, r.nb_w = keep.d(P.nb_dx), keep.d(P.nb_dy), keep.d(P.nb_w)
+    r.face_a, r.face_b, r.t = keep.i(P.face_a), keep.i(P.face_b), keep.d(P.t)
+    return r
+
+
+@dataclass
+class EvaluationResult:
+    status: int
+    log_lik: float
+    predictions: np.ndarray
+    simple_iters: int
+
+    @property
+    def status_name(self) -> str:
+        return STATUS_NAMES[self.status]
+
+
+@dataclass
+class ForwardBatchRow:
+    """ForwardBatchRow (rans-uq/include/ForwardBatch.hpp:8-14)."""
+
+    result: EvaluationResult
+    final_worst_measure: float
+    contraction_rate_u: float
+    stalled: bool
+    error: str
+
+
+@dataclass
+class ForwardBatchResult:
+    """ForwardBatchResult with the array views of the reference's binding
+    (rans-uq/python/bindings.cpp:830-887); every array is owned."""
+
+    n_observations: int
+    rows: list
+    predictions: np.ndarray
+    log_likelihoods: np.ndarray
+    statuses: np.ndarray
+    iteration_counts: np.ndarray
+    errors: list
+    convergence_mask: np.ndarray
+    final_worst_measures: np.ndarray
+    contraction_rates: np.ndarray
+    stalled: np.ndarray
+    krylov_iterations: np.ndarray
+    fields: np.ndarray = None
+    kernel_ms: float = 0.0
+
+    @property
+    def converged(self) -> np.ndarray:
+        return self.convergence_mask
+
+
+class ForwardBatch:
+    """Synchronous coefficient batch on one GPU (ForwardBatch.hpp:23-125)."""
+
+    def __init__(self, mesh: Mesh, param_set: InferenceParameterSet,
+                 obs_op: ObservationOperator, bcs: FlowBoundaryConditions, nu: float,
+                 settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
+                 p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
+                 mixed_precision: bool = False, block_size: int = 0):
+        settings = settings if settings is not None else SolverSettings()
+        settings.validate()
+        param_set.validate()
+        bcs.validate(mesh)
+        mesh.validate_2d()
+        if not mesh.wall_distance_valid():
+            raise ValueError("ForwardBatch needs compute_wall_distance before construction")
+        if not math.isfinite(nu) or nu <= 0:
+            raise ValueError("SIMPLE kinematic viscosity must be finite and positive")
+        if isinstance(u_init, (tuple, list)):
+            u_init = Vec3(*u_init)
+        if u_init.z != 0.0:
+            raise ValueError("planar solver: initial velocity must have zero z component")
+        self.lib = load_library()
+        self.param_set = param_set
+        self.n_obs = obs_op.n_obs()
+        self.n_cells = mesh.n_cells
+        self.n_faces = mesh.n_faces
+        prepared = obs_op.prepare(mesh)
+        keep = _Keep()
+        m = mesh_struct(mesh, keep)
+        b = RqBc()
+        ut, uv, pt, pv, kt, kv, wt, wv = bcs.arrays()
+        b.u_type, b.p_type, b.k_type, b.w_type = keep.i(ut), keep.i(pt), keep.i(kt), keep.i(wt)
+        b.u_val, b.p_val, b.k_val, b.w_val = keep.d(uv), keep.d(pv), keep.d(kv), keep.d(wv)
+        p = RqParams()
+        p.n_active = param_set.n_active()
+        p.active_indices = keep.i(param_set.active_indices)
+        p.base = keep.d(param_set.base)
+        p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
+        o = obs_struct(prepared, keep)
+        s = settings_struct(settings, mixed_precision, block_size)
+        init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
+        err = C.create_string_buffer(512)
+        h = self.lib.rq_engine_create(C.byref(m), C.byref(p), C.byref(o), C.byref(b), float(nu),
+                                      C.byref(s), init.ctypes.data_as(_dp), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and self.lib is not None:
+            self.lib.rq_engine_destroy(h)
+            self._h = None
+
+    def n_parameters(self) -> int:
+        return self.param_set.n_active()
+
+    def n_observations(self) -> int:
+        return self.n_obs
+
+    def last_timing(self):
+        ms, bs, lds, nbytes = C.c_double(), C.c_int32(), C.c_int32(), C.c_int64()
+        self.lib.rq_engine_last_timing(self._h, C.byref(ms), C.byref(bs), C.byref(lds),
+                                       C.byref(nbytes))
+        return {"kernel_ms": ms.value, "block_size": bs.value, "arena_in_lds": bool(lds.value),
+                "arena_bytes": nbytes.value}
+
+    def evaluate(self, theta, workers: int = 1, fields: bool = False) -> ForwardBatchResult:
+        theta = np.asarray(theta, dtype=np.float64)
+        if theta.ndim == 1 and theta.size == 0:
+            theta = theta.reshape(0, self.n_parameters())
+        if theta.ndim != 2:
+            raise ValueError("theta must be a two-dimensional array")
+        if workers < 1:
+            raise ValueError("workers must be positive")
+        if theta.shape[1] != self.n_parameters():
+            raise ValueError("batch rows must all have n_parameters entries")
+        theta = np.ascontiguousarray(theta)
+        n, nobs = theta.shape[0], self.n_obs
+        status = np.zeros(n, np.int32)
+        loglik = np.zeros(n)
+        pred = np.full((n, max(nobs, 0)), np.nan)
+        iters = np.zeros(n, np.int32)
+        worst = np.zeros(n)
+        rate = np.zeros(n)
+        stalled = np.zeros(n, np.uint8)
+        conv = np.zeros(n, np.uint8)
+        kry = np.zeros((n, 5), np.int64)
+        stride = 256
+        errs = C.create_string_buffer(max(1, n * stride))
+        fld = np.zeros((n, 10 * self.n_cells + self.n_faces)) if fields else None
+        out = RqBatchOut()
+        out.status = status.ctypes.data_as(_ip)
+        out.loglik = loglik.ctypes.data_as(_dp)
+        out.predictions = pred.ctypes.data_as(_dp)
+        out.iterations = iters.ctypes.data_as(_ip)
+        out.final_worst_measure = worst.ctypes.data_as(_dp)
+        out.contraction_rate_u = rate.ctypes.data_as(_dp)
+        out.stalled = stalled.ctypes.data_as(_u8p)
+        out.converged = conv.ctypes.data_as(_u8p)
+        out.row_errors = C.cast(errs, C.c_char_p)
+        out.row_error_stride = stride
+        out.fields = fld.ctypes.data_as(_dp) if fields else None
+        out.krylov_iterations = kry.ctypes.data_as(_i64p)
+        err = C.create_string_buffer(512)
+        rc = self.lib.rq_engine_evaluate(self._h, theta.ctypes.data_as(_dp), n,
+                                         self.n_parameters(), int(workers), C.byref(out), err, 512)
+        if rc == 1:
+            raise ValueError(err.value.decode())
+        if rc == 3:
+            raise MemoryError(err.value.decode())
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        raw = errs.raw
+        errors = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(n)]
+        rows = []
+        for i in range(n):
+            res = EvaluationResult(int(status[i]), float(loglik[i]), pred[i].copy(), int(iters[i]))
+            rows.append(ForwardBatchRow(res, float(worst[i]), float(rate[i]), bool(stalled[i]),
+                                        errors[i]))
+        timing = self.last_timing()
+        return ForwardBatchResult(nobs, rows, pred, loglik, status, iters, errors,
+                                  conv.astype(bool), worst, rate, stalled.astype(bool), kry,
+                                  fld, timing["kernel_ms"])
+
+
+def build_info() -> str:
+    return load_library().rq_build_info().decode()

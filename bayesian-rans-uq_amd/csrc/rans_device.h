@@ -1,0 +1,109 @@
+// Plain-data structures shared by the host engine and the gfx950 kernels.
+// Everything here is POD so it can be copied to the device as a kernel
+// argument or into device memory.
+#pragma once
+#include <stdint.h>
+
+namespace rq {
+
+constexpr int kSlots = 4;          // faces per cell (ELL width): quadrilaterals and triangles
+constexpr int kMaxWindow = 256;    // largest contraction window supported
+
+enum BcType { BC_DIRICHLET = 0, BC_NEUMANN, BC_INLET, BC_OUTLET, BC_WALL_NOSLIP,
+              BC_WALL_KOMEGA, BC_WALL_MOVING, BC_SYMMETRY, BC_CYCLIC };
+enum PatchType { PATCH_WALL = 0, PATCH_INLET, PATCH_OUTLET, PATCH_SYMMETRY, PATCH_OTHER };
+enum SolverKind { SOLVER_PCG = 0, SOLVER_BICGSTAB = 1, SOLVER_AMG = 2 };
+enum Scheme { CONV_UPWIND = 0, CONV_LINEAR_UPWIND = 1, CONV_LIMITED = 2 };
+enum Status { ST_CONVERGED = 0, ST_UNCONVERGED, ST_DIVERGED, ST_INVALID, ST_UNKNOWN };
+enum ObsKind { OBS_VELOCITY = 0, OBS_PRESSURE = 1, OBS_SKIN_FRICTION = 2 };
+// device error codes (mapped to the reference's exception texts on the host)
+enum DevError { ERR_NONE = 0, ERR_NONFINITE_SYSTEM = 1, ERR_NONFINITE_MONITOR = 2 };
+
+// Per-draw arena, array-of-structures: every cell, face and boundary face is
+// one record of doubles with an odd stride (so consecutive records start in
+// different LDS banks), and a variable is a fixed offset inside its record.
+// Geometry lives in the same records, pre-filled by the host.
+enum CellVar {
+    V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw, V_S, V_aPs, V_UxP, V_UyP, V_pP,
+    V_gUxx, V_gUxy, V_gUyx, V_gUyy, V_gPx, V_gPy, V_gKx, V_gKy, V_gWx, V_gWy,
+    V_diag, V_src, V_off0, V_off1, V_off2, V_off3, V_wallv,
+    V_kx, V_kr, V_krh, V_kp, V_kv, V_ks, V_kt, V_kph, V_ksh,
+    V_vol, V_oms, V_oml, V_wd, V_COUNT
+};
+// face record: transported state, then geometry Sx=nx*A, Sy=ny*A, A,
+// dlt=max(delta,1e-20), w, wm=1-w, nx, ny, ex/ey=d/max(delta,1e-30), the
+// upwind offset vectors for owner-upwind (dO) and neighbour-upwind (dN)
+// faces, and the planar-balance wall-normal gradient omg.
+enum FaceVar {
+    W_phi, W_phiBar, W_phiPred, W_pCoeff, W_Sx, W_Sy, W_A, W_dlt, W_w, W_wm, W_nx, W_ny, W_ex,
+    W_ey, W_dOx, W_dOy, W_dNx, W_dNy, W_omg, W_COUNT
+};
+enum BndVar { B_Ux, B_Uy, B_p, B_k, B_w, B_pP, B_COUNT };
+constexpr int kNA = V_COUNT | 1;
+constexpr int kNF = W_COUNT | 1;
+constexpr int kNB = B_COUNT | 1;
+
+// Offsets in doubles inside one draw's arena; the int tables start at
+// ibase (in doubles) and their offsets are in ints from there.
+struct ArenaLayout {
+    int64_t stride;                  // doubles per draw
+    int64_t face, bnd, ring, q, qprev;
+    int64_t ibase;                   // start of the int tables (doubles)
+    int64_t islot, iface, ibflag;    // in ints: [nc][8] slot codes then neighbours,
+                                     // [nf][2] owner/neighbour, [nb] boundary flags
+    int64_t image_bytes;             // bytes of the pre-filled image copied at launch
+};
+
+// Mesh and problem tables in device memory, shared by every draw on the mesh.
+struct DevMesh {
+    const double *img;  // pre-filled arena image (geometry, zero state, int tables)
+    int nc, nif, nf, nb, npatch;
+    int has_outlet, planar;
+    const double *wall_y;        // [nc] wall-face distance of the constrained cell, NaN otherwise
+    const int *bpatch;           // [nb] patch of each boundary face
+    const int *u_type, *p_type, *k_type, *w_type;                  // [npatch]
+    const double *u_valx, *u_valy, *p_val, *k_val, *w_val;         // [npatch]
+    // prepared observations
+    int nobs;
+    const int *ob_type, *ob_comp, *ob_cell, *ob_nb_ptr, *ob_nb_cell, *ob_nb_face;
+    const int *ob_fa, *ob_fb;
+    const double *ob_offx, *ob_offy, *ob_axx, *ob_axy, *ob_ayy, *ob_det;
+    const double *ob_nb_dx, *ob_nb_dy, *ob_nb_w, *ob_t, *ob_refv, *ob_refp, *ob_refrho;
+    const double *ob_floor;
+    ArenaLayout lay;  // per-draw arena layout for this mesh
+};
+
+struct DevSettings {
+    int max_iterations;
+    double convergence_tol, divergence_limit;
+    int convergence_window, monitor_interval;
+    double alpha_u, alpha_p, alpha_k, alpha_omega;
+    int inner_iterations;
+    double inner_tolerance;
+    int turb_start_iter, turb_update_interval, nut_floor_iters;
+    int momentum_convection, turbulence_convection;
+    double body_force_x, body_force_y;
+    double k_min, k_max, omega_min;
+    int pressure_solver, momentum_solver, turbulence_solver;
+    int sst_variant;
+    double nu;
+    double init_ux, init_uy, init_p, init_k, init_w;
+    int mixed_precision;   // 1: FP32 Krylov with FP64 defect correction
+};
+
+// One draw to solve: its mesh, its arena and its coefficients.
+struct DevSlot {
+    const DevMesh *mesh;
+    double *arena;          // used when the arena lives in global memory
+    double coeffs[11];
+    int row;                // output row index
+};
+
+// Per-draw results written by the kernel.
+struct DevResult {
+    int status, iters, stalled, converged, diverged, error;
+    double worst, rate;
+    long long krylov[5];
+};
+
+}  // namespace rq

@@ -1,0 +1,803 @@
+// Host side of the MI355X forward-solve engine: admission of the problem,
+// mesh tables for the cell-centric kernels, device memory, launch and the
+// per-row result conventions of the reference's ForwardBatch
+// (rans-uq/include/ForwardBatch.hpp:52-124) and ForwardModel
+// (rans-uq/src/ForwardModel.cpp:35-137).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/rans_amd.h"
+#include "rans_device.h"
+
+namespace rq {
+hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int nslots,
+                               const DevSettings &s, DevResult *results, double *pred,
+                               int pred_stride, double *fields, int64_t field_stride,
+                               size_t lds_bytes, hipStream_t stream);
+}
+
+using namespace rq;
+
+namespace {
+
+void set_err(char *err, size_t cap, const std::string &msg) {
+    if (err && cap) {
+        std::snprintf(err, cap, "%s", msg.c_str());
+    }
+}
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHECK(expr)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw Fail{e_ == hipErrorOutOfMemory ? RQ_ENOMEM : RQ_EDEVICE,                  \
+                       std::string(#expr) + ": " + hipGetErrorString(e_)};                  \
+    } while (0)
+
+// Device buffer holding many host arrays back to back.
+struct Blob {
+    std::vector<char> host;
+    template <class T>
+    size_t add(const std::vector<T> &v) {
+        size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + v.size() * sizeof(T) + 1);
+        if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+bool finite3(const double *v) {
+    return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+}  // namespace
+
+struct rq_engine {
+    // owned problem copy
+    int nc = 0, nif = 0, nf = 0, nb = 0, npatch = 0, K = 0;
+    int n_active = 0, nobs = 0;
+    std::vector<int> active;
+    std::vector<double> base, lower, upper, observed, variance;
+    DevSettings settings{};
+    int block = 64;
+    bool lds = true;
+    size_t lds_bytes = 0;
+    ArenaLayout lay{};
+    // device
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    char *d_blob = nullptr;
+    DevMesh *d_mesh = nullptr;
+    DevSlot *d_slots = nullptr;
+    DevResult *d_results = nullptr;
+    double *d_pred = nullptr, *d_fields = nullptr, *d_arena = nullptr;
+    size_t cap_slots = 0, cap_results = 0, cap_pred = 0, cap_fields = 0, cap_arena = 0;
+    double last_ms = 0.0;
+
+    ~rq_engine() {
+        if (d_blob) (void)hipFree(d_blob);
+        if (d_mesh) (void)hipFree(d_mesh);
+        if (d_slots) (void)hipFree(d_slots);
+        if (d_results) (void)hipFree(d_results);
+        if (d_pred) (void)hipFree(d_pred);
+        if (d_fields) (void)hipFree(d_fields);
+        if (d_arena) (void)hipFree(d_arena);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void validate_settings(const rq_settings &s) {
+    // SolverSettings::validate (core/include/FlowFields.hpp:203-240)
+    if (s.sst_variant < 0 || s.sst_variant > 2)
+        throw Fail{RQ_EINVAL, "unknown SST variant: expected Full (0), NoLimiter (1), or KOmega (2)"};
+    if (s.max_iterations < 0 || s.inner_iterations <= 0 || s.convergence_window < 2 ||
+        s.monitor_interval <= 0 || s.turb_update_interval <= 0 || s.turb_start_iter < 0 ||
+        s.nut_floor_iters < 0 ||
+        s.turb_start_iter > std::numeric_limits<int>::max() - s.nut_floor_iters)
+        throw Fail{RQ_EINVAL, "invalid SIMPLE iteration count or cadence"};
+    if (!std::isfinite(s.divergence_limit) || s.divergence_limit < 0.0)
+        throw Fail{RQ_EINVAL, "SIMPLE divergence limit must be finite and nonnegative"};
+    if (!std::isfinite(s.convergence_tol) || s.convergence_tol < 0.0)
+        throw Fail{RQ_EINVAL, "SIMPLE convergence tolerance must be finite and nonnegative"};
+    for (double v : {s.inner_tolerance, s.alpha_u, s.alpha_k, s.alpha_omega})
+        if (!std::isfinite(v) || v <= 0.0)
+            throw Fail{RQ_EINVAL,
+                       "SIMPLE tolerances and relaxation denominators must be finite and positive"};
+    if (!std::isfinite(s.alpha_p) || s.alpha_p < 0.)
+        throw Fail{RQ_EINVAL, "SIMPLE pressure/source relaxation must be finite and nonnegative"};
+    if (!std::isfinite(s.body_force_x) || !std::isfinite(s.body_force_y))
+        throw Fail{RQ_EINVAL, "SIMPLE body force must be finite"};
+    for (double v : {s.k_min, s.k_max, s.omega_min})
+        if (!std::isfinite(v) || v < 0.)
+            throw Fail{RQ_EINVAL, "SIMPLE turbulence limits must be finite and nonnegative"};
+    if (s.k_max < s.k_min) throw Fail{RQ_EINVAL, "SIMPLE kMax must be at least kMin"};
+    for (int v : {s.momentum_convection, s.turbulence_convection})
+        if (v < 0 || v > 2) throw Fail{RQ_EINVAL, "unknown SIMPLE convection scheme"};
+    for (int v : {s.pressure_solver, s.momentum_solver, s.turbulence_solver})
+        if (v < 0 || v > 2) throw Fail{RQ_EINVAL, "unknown linear solver"};
+    if (s.momentum_solver == SOLVER_AMG || s.turbulence_solver == SOLVER_AMG)
+        throw Fail{RQ_EINVAL, "AMG is supported for the pressure correction only"};
+    if (s.pressure_solver == SOLVER_AMG)
+        throw Fail{RQ_EINVAL, "the AMG pressure solver is not available in this engine build"};
+    if (s.convergence_window > kMaxWindow)
+        throw Fail{RQ_EINVAL, "convergence window exceeds the engine limit of 256"};
+    if (s.mixed_precision != 0 && s.mixed_precision != 1)
+        throw Fail{RQ_EINVAL, "mixed_precision must be 0 or 1"};
+    if (s.mixed_precision) throw Fail{RQ_EINVAL, "mixed precision is not available in this engine build"};
+    if (s.block_size != 0 && s.block_size != 64 && s.block_size != 128 && s.block_size != 256 &&
+        s.block_size != 512)
+        throw Fail{RQ_EINVAL, "block_size must be 0, 64, 128, 256 or 512"};
+}
+
+// preparePlanarOmegaBalance (rans-uq/src/SIMPLESolver.cpp:92-153)
+bool planar_balance(const rq_mesh_desc &m, const std::vector<int> &bpatch, const rq_bc_desc &bc,
+                    std::vector<double> &sing, std::vector<double> &lap, std::vector<double> &grad) {
+    const int nif = m.n_internal, nf = m.n_faces, nc = m.n_cells;
+    if (nf == nif) return false;
+    const int f0 = nif;
+    const double inx = m.face_normal[3 * f0] * -1.0, iny = m.face_normal[3 * f0 + 1] * -1.0;
+    auto dotc = [&](const double *v) { return v[0] * inx + v[1] * iny; };
+    const double origin = dotc(&m.face_center[3 * f0]);
+    double height = 0;
+    bool periodic = false;
+    for (int f = nif; f < nf; ++f) {
+        const int pi = bpatch[f - nif];
+        if (m.patch_type[pi] != PATCH_WALL || bc.w_type[pi] != BC_WALL_KOMEGA ||
+            std::abs(std::abs(dotc(&m.face_normal[3 * f])) - 1.0) > 1e-12)
+            return false;
+        height = std::max(height, dotc(&m.face_center[3 * f]) - origin);
+    }
+    if (!(height > 0)) return false;
+    const double tolerance = 1e-12 * height;
+    for (int f = 0; f < nf; ++f) {
+        const double *d = &m.face_d[3 * f], *n = &m.face_normal[3 * f], *s = &m.face_shift[3 * f];
+        const double de = m.face_delta[f];
+        const double ex = d[0] - n[0] * de, ey = d[1] - n[1] * de, ez = d[2] - n[2] * de;
+        if (std::sqrt(ex * ex + ey * ey + ez * ez) > 1e-9 * de || std::abs(dotc(s)) > tolerance)
+            return false;
+        periodic = periodic || (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) > 0;
+        if (f >= nif) {
+            const double y = dotc(&m.face_center[3 * f]) - origin;
+            if (std::min(std::abs(y), std::abs(y - height)) > tolerance) return false;
+            if ((y < height / 2 ? -1.0 : 1.0) * dotc(n) < 1 - 1e-12) return false;
+        }
+    }
+    if (!periodic) return false;
+    sing.assign(nc, 0.0);
+    lap.assign(nc, 0.0);
+    grad.assign(std::max(nif, 1), 0.0);
+    for (int c = 0; c < nc; ++c) {
+        const double y = dotc(&m.cell_center[3 * c]) - origin;
+        if (!(y > 0 && y < height)) return false;
+        sing[c] = 1 / (y * y) + 1 / ((height - y) * (height - y));
+        lap[c] = 6 * (std::pow(y, -4) + std::pow(height - y, -4));
+    }
+    for (int f = 0; f < nif; ++f) {
+        const double y = dotc(&m.face_center[3 * f]) - origin;
+        if (!(y > 0 && y < height)) return false;
+        grad[f] = 2 * (std::pow(height - y, -3) - std::pow(y, -3)) * dotc(&m.face_normal[3 * f]);
+    }
+    return true;
+}
+
+int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, int window) {
+    (void)nif;
+    int64_t o = (int64_t)nc * kNA;
+    L.face = o;
+    o += (int64_t)nf * kNF;
+    L.bnd = o;
+    o += (int64_t)std::max(nb, 1) * kNB;
+    L.ring = o;
+    o += 8 * (int64_t)window;
+    L.q = o;
+    o += std::max(nobs, 1);
+    L.qprev = o;
+    o += std::max(nobs, 1);
+    o = (o + 1) & ~int64_t(1);
+    L.ibase = o;
+    int64_t ni = 0;
+    L.islot = ni;
+    ni += 8 * (int64_t)nc;
+    L.iface = ni;
+    ni += 2 * (int64_t)nf;
+    L.ibflag = ni;
+    ni += std::max(nb, 1);
+    o += (ni + 1) / 2;
+    o = (o + 1) & ~int64_t(1);  // 16-byte multiple
+    L.stride = o;
+    L.image_bytes = o * 8;
+    return o;
+}
+
+rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq_obs_desc *op,
+                       const rq_bc_desc *bp, double nu, const rq_settings *sp, const double *init) {
+    if (!mp || !pp || !op || !bp || !sp || !init) throw Fail{RQ_EINVAL, "null argument"};
+    const rq_mesh_desc &m = *mp;
+    const rq_settings &s = *sp;
+    validate_settings(s);
+    // simple_admission::problem / initial (rans-uq/include/SIMPLEValidation.hpp:32-60)
+    if (!std::isfinite(nu) || nu <= 0)
+        throw Fail{RQ_EINVAL, "SIMPLE kinematic viscosity must be finite and positive"};
+    if (!std::isfinite(init[0]) || !std::isfinite(init[1]) || !std::isfinite(init[2]))
+        throw Fail{RQ_EINVAL, "SIMPLE initial velocity and pressure must be finite"};
+    if (!std::isfinite(init[3]) || init[3] < 0)
+        throw Fail{RQ_EINVAL, "SIMPLE initial k must be finite and nonnegative"};
+    if (!std::isfinite(init[4]) || init[4] <= 0)
+        throw Fail{RQ_EINVAL, "SIMPLE initial omega must be finite and positive"};
+    const int nc = m.n_cells, nif = m.n_internal, nf = m.n_faces, np = m.n_patches;
+    if (nc <= 0 || nif < 0 || nf < nif || np < 0) throw Fail{RQ_EINVAL, "invalid mesh counts"};
+    const int nb = nf - nif;
+    for (int f = 0; f < nf; ++f) {
+        const bool internal = f < nif;
+        if (m.owner[f] < 0 || m.owner[f] >= nc ||
+            (internal ? (m.neighbor[f] < 0 || m.neighbor[f] >= nc) : m.neighbor[f] != -1))
+            throw Fail{RQ_EINVAL, "invalid mesh face connectivity"};
+        if (m.face_normal[3 * f + 2] != 0.0)
+            throw Fail{RQ_EINVAL, "mesh must be 2D: every face normal needs a zero z component"};
+        if (!finite3(&m.face_normal[3 * f]) || !finite3(&m.face_center[3 * f]) ||
+            !finite3(&m.face_d[3 * f]) || !finite3(&m.face_shift[3 * f]) ||
+            !std::isfinite(m.face_area[f]) || !std::isfinite(m.face_delta[f]) ||
+            !std::isfinite(m.face_weight[f]))
+            throw Fail{RQ_EINVAL, "nonfinite mesh face geometry"};
+    }
+    // each cell's faces ascending and every face listed by its cells
+    std::vector<int> seen(nf, 0);
+    int K = 0;
+    for (int c = 0; c < nc; ++c) {
+        const int a = m.cell_face_ptr[c], b = m.cell_face_ptr[c + 1];
+        if (b < a) throw Fail{RQ_EINVAL, "invalid cell face list"};
+        K = std::max(K, b - a);
+        for (int q = a; q < b; ++q) {
+            const int f = m.cell_face_idx[q];
+            if (f < 0 || f >= nf) throw Fail{RQ_EINVAL, "invalid cell face index"};
+            if (q > a && f < m.cell_face_idx[q - 1])
+                throw Fail{RQ_EINVAL, "cell face lists must be in ascending face order"};
+            if (m.owner[f] != c && m.neighbor[f] != c)
+                throw Fail{RQ_EINVAL, "inconsistent cell-face incidence"};
+            ++seen[f];
+        }
+        if (!std::isfinite(m.cell_volume[c]) || !finite3(&m.cell_center[3 * c]) ||
+            !std::isfinite(m.wall_distance[c]))
+            throw Fail{RQ_EINVAL, "nonfinite mesh cell geometry or wall distance"};
+    }
+    for (int f = 0; f < nf; ++f)
+        if (seen[f] != (f < nif ? 2 : 1)) throw Fail{RQ_EINVAL, "incomplete cell face lists"};
+    if (K > kSlots) throw Fail{RQ_EINVAL, "cells with more than 4 faces are not supported"};
+    // boundary faces: patches must list them in face order
+    std::vector<int> bpatch(nb, -1);
+    {
+        int expect = nif;
+        for (int pi = 0; pi < np; ++pi)
+            for (int q = m.patch_face_ptr[pi]; q < m.patch_face_ptr[pi + 1]; ++q) {
+                if (m.patch_face_idx[q] != expect)
+                    throw Fail{RQ_EINVAL,
+                               "patches must list the boundary faces in ascending face order"};
+                bpatch[expect - nif] = pi;
+                ++expect;
+            }
+        if (expect != nf) throw Fail{RQ_EINVAL, "every boundary face needs a patch"};
+    }
+    // boundary conditions (validatePatchBCs, BoundaryCondition.hpp:45-103)
+    const rq_bc_desc &bc = *bp;
+    for (int pi = 0; pi < np; ++pi) {
+        for (const int *t : {bc.u_type, bc.p_type, bc.k_type, bc.w_type})
+            if (t[pi] < 0 || t[pi] > 8) throw Fail{RQ_EINVAL, "unknown boundary condition type"};
+        const int ut = bc.u_type[pi];
+        if ((ut == BC_DIRICHLET || ut == BC_INLET || ut == BC_WALL_MOVING) &&
+            (!std::isfinite(bc.u_val[2 * pi]) || !std::isfinite(bc.u_val[2 * pi + 1])))
+            throw Fail{RQ_EINVAL, "active vector boundary values must be finite"};
+        if ((bc.p_type[pi] == BC_DIRICHLET || bc.p_type[pi] == BC_OUTLET) &&
+            !std::isfinite(bc.p_val[pi]))
+            throw Fail{RQ_EINVAL, "active scalar boundary values must be finite"};
+        for (auto pr : {std::make_pair(bc.k_type, bc.k_val), std::make_pair(bc.w_type, bc.w_val)})
+            if (pr.first[pi] == BC_DIRICHLET) {
+                if (!std::isfinite(pr.second[pi]))
+                    throw Fail{RQ_EINVAL, "active scalar boundary values must be finite"};
+                if (pr.second[pi] < 0)
+                    throw Fail{RQ_EINVAL, "SIMPLE prescribed turbulence must be nonnegative"};
+            }
+    }
+    // parameters (InferenceParameterSet::validate)
+    const rq_param_desc &P = *pp;
+    if (P.n_active <= 0)
+        throw Fail{RQ_EINVAL, "inference parameters require at least one active index"};
+    {
+        bool used[11] = {};
+        for (int i = 0; i < P.n_active; ++i) {
+            const int a = P.active_indices[i];
+            if (a < 0 || a >= 11) throw Fail{RQ_EINVAL, "inference parameter index must be in [0,10]"};
+            if (used[a]) throw Fail{RQ_EINVAL, "inference parameter indices must be unique"};
+            used[a] = true;
+        }
+    }
+
+    auto *E = new rq_engine();
+    try {
+        E->nc = nc;
+        E->nif = nif;
+        E->nf = nf;
+        E->nb = nb;
+        E->npatch = np;
+        E->K = K;
+        E->n_active = P.n_active;
+        E->active.assign(P.active_indices, P.active_indices + P.n_active);
+        E->base.assign(P.base, P.base + 11);
+        E->lower.assign(P.lower, P.lower + P.n_active);
+        E->upper.assign(P.upper, P.upper + P.n_active);
+        const rq_obs_desc &O = *op;
+        E->nobs = O.n_obs;
+        E->observed.assign(O.observed, O.observed + O.n_obs);
+        E->variance.assign(O.variance, O.variance + O.n_obs);
+        for (int i = 0; i < O.n_obs; ++i)
+            if (!std::isfinite(E->variance[i]) || E->variance[i] <= 0 ||
+                !std::isfinite(E->observed[i]))
+                throw Fail{RQ_EINVAL,
+                           "Gaussian measurements require finite values, nonnegative uncertainties "
+                           "and finite positive total variance"};
+
+        // ---- arena image: geometry in the cell/face records and the int
+        //      tables, each face constant computed with the reference's arithmetic
+        const int no = O.n_obs;
+        ArenaLayout L{};
+        build_layout(L, nc, nif, nf, nb, no, s.convergence_window);
+        std::vector<double> image(L.stride, 0.0);
+        double *CR = image.data(), *FR = image.data() + L.face;
+        int *IB = reinterpret_cast<int *>(image.data() + L.ibase);
+        int *ISL = IB + L.islot, *IFA = IB + L.iface, *IBF = IB + L.ibflag;
+        for (int64_t i = 0; i < 8 * (int64_t)nc; ++i) ISL[i] = -1;
+        for (int c = 0; c < nc; ++c) {
+            int s2 = 0, lastSelf = -1;
+            for (int q = m.cell_face_ptr[c]; q < m.cell_face_ptr[c + 1]; ++q, ++s2) {
+                const int f = m.cell_face_idx[q];
+                int side = 0;
+                if (f < nif) {
+                    if (m.owner[f] == m.neighbor[f]) {  // self-wrapped face: owner side first
+                        side = (lastSelf == f) ? 1 : 0;
+                        lastSelf = f;
+                    } else {
+                        side = m.owner[f] == c ? 0 : 1;
+                    }
+                    ISL[8 * c + 4 + s2] = side == 0 ? m.neighbor[f] : m.owner[f];
+                }
+                ISL[8 * c + s2] = f | (side << 30);
+            }
+        }
+        for (int f = 0; f < nf; ++f) {
+            IFA[2 * f] = m.owner[f];
+            IFA[2 * f + 1] = m.neighbor[f];
+        }
+        for (int b = 0; b < nb; ++b) {
+            const int t = m.patch_type[bpatch[b]];
+            IBF[b] = (t == PATCH_WALL ? 1 : 0) | (t == PATCH_OUTLET ? 2 : 0);
+        }
+        const double *CC = m.cell_center, *FC = m.face_center, *FN = m.face_normal;
+        const double *FSH = m.face_shift, *FD = m.face_d;
+        const bool linear = s.momentum_convection == CONV_LINEAR_UPWIND ||
+                            s.turbulence_convection == CONV_LINEAR_UPWIND;
+        const bool limited = s.momentum_convection == CONV_LIMITED ||
+                             s.turbulence_convection == CONV_LIMITED;
+        if (linear && limited)
+            throw Fail{RQ_EINVAL, "momentum and turbulence must use the same higher-order scheme"};
+        for (int f = 0; f < nf; ++f) {
+            double *R = FR + (int64_t)f * kNF;
+            const double area = m.face_area[f];
+            R[W_Sx] = FN[3 * f] * area;  // Vec3 Sf = normal * area
+            R[W_Sy] = FN[3 * f + 1] * area;
+            R[W_A] = area;
+            R[W_dlt] = std::max(m.face_delta[f], 1e-20);
+            R[W_w] = m.face_weight[f];
+            R[W_wm] = 1.0 - m.face_weight[f];
+            R[W_nx] = FN[3 * f];
+            R[W_ny] = FN[3 * f + 1];
+            const double dd = std::max(m.face_delta[f], 1e-30);
+            R[W_ex] = FD[3 * f] / dd;
+            R[W_ey] = FD[3 * f + 1] / dd;
+            if (f < nif) {
+                const int o = m.owner[f], n = m.neighbor[f];
+                if (linear) {
+                    // x_f - x_U (SIMPLESolver.cpp:517-518)
+                    R[W_dOx] = FC[3 * f] - CC[3 * o];
+                    R[W_dOy] = FC[3 * f + 1] - CC[3 * o + 1];
+                    R[W_dNx] = (FC[3 * f] - FSH[3 * f]) - CC[3 * n];
+                    R[W_dNy] = (FC[3 * f + 1] - FSH[3 * f + 1]) - CC[3 * n + 1];
+                } else {
+                    // x_D - x_U of the limited scheme (SIMPLESolver.cpp:521-523)
+                    R[W_dOx] = (CC[3 * n] + FSH[3 * f]) - CC[3 * o];
+                    R[W_dOy] = (CC[3 * n + 1] + FSH[3 * f + 1]) - CC[3 * o + 1];
+                    R[W_dNx] = (CC[3 * o] - FSH[3 * f]) - CC[3 * n];
+                    R[W_dNy] = (CC[3 * o + 1] - FSH[3 * f + 1]) - CC[3 * n + 1];
+                }
+            }
+        }
+        std::vector<double> sing, lap, grad;
+        const bool planar = planar_balance(m, bpatch, bc, sing, lap, grad);
+        for (int f = 0; f < nif && planar; ++f) FR[(int64_t)f * kNF + W_omg] = grad[f];
+        for (int c = 0; c < nc; ++c) {
+            double *R = CR + (int64_t)c * kNA;
+            R[V_vol] = m.cell_volume[c];
+            R[V_oms] = planar ? sing[c] : 0.0;
+            R[V_oml] = planar ? lap[c] : 0.0;
+            R[V_wd] = m.wall_distance[c];
+        }
+        // wall-constrained omega cells: the last wall face of each owner wins
+        // (SIMPLESolver::omegaWallValues, SIMPLESolver.cpp:665-684)
+        std::vector<double> wall_y(nc, std::numeric_limits<double>::quiet_NaN());
+        for (int pi = 0; pi < np; ++pi) {
+            if (m.patch_type[pi] != PATCH_WALL) continue;
+            for (int q = m.patch_face_ptr[pi]; q < m.patch_face_ptr[pi + 1]; ++q) {
+                const int f = m.patch_face_idx[q];
+                wall_y[m.owner[f]] = std::max(m.face_delta[f], 1e-20);
+            }
+        }
+        bool has_outlet = false;
+        for (int pi = 0; pi < np; ++pi)
+            if (m.patch_type[pi] == PATCH_OUTLET && m.patch_face_ptr[pi + 1] > m.patch_face_ptr[pi])
+                has_outlet = true;
+
+        // ---- cold tables (boundary conditions, observations) in one blob
+        auto vecd = [](const double *v, int n) { return std::vector<double>(v, v + n); };
+        auto veci = [](const int32_t *v, int n) { return std::vector<int>(v, v + n); };
+        Blob B;
+        DevMesh D{};
+        const size_t o_img = B.add(image);
+        size_t o_wy = B.add(wall_y), o_bp = B.add(bpatch);
+        size_t o_ut = B.add(veci(bc.u_type, np));
+        size_t o_ptp = B.add(veci(bc.p_type, np)), o_kt = B.add(veci(bc.k_type, np));
+        size_t o_wt = B.add(veci(bc.w_type, np));
+        std::vector<double> uvx(np), uvy(np);
+        for (int pi = 0; pi < np; ++pi) {
+            uvx[pi] = bc.u_val[2 * pi];
+            uvy[pi] = bc.u_val[2 * pi + 1];
+        }
+        size_t o_uvx = B.add(uvx);
+        size_t o_uvy = B.add(uvy), o_pv = B.add(vecd(bc.p_val, np)), o_kv = B.add(vecd(bc.k_val, np));
+        size_t o_wv = B.add(vecd(bc.w_val, np));
+        const int nnb = no > 0 ? O.nb_ptr[no] : 0;
+        std::vector<double> floors(no);
+        for (int i = 0; i < no; ++i) floors[i] = std::sqrt(O.variance[i]);
+        std::vector<int> nbptr(no + 1, 0);
+        if (no > 0) nbptr.assign(O.nb_ptr, O.nb_ptr + no + 1);
+        size_t o_ot = B.add(veci(O.type, no)), o_oc = B.add(veci(O.component, no));
+        size_t o_ocell = B.add(veci(O.cell, no)), o_onp = B.add(nbptr);
+        size_t o_onc = B.add(veci(O.nb_cell, nnb)), o_onf = B.add(veci(O.nb_face, nnb));
+        size_t o_ofa = B.add(veci(O.face_a, no)), o_ofb = B.add(veci(O.face_b, no));
+        size_t o_ox = B.add(vecd(O.offx, no)), o_oy = B.add(vecd(O.offy, no));
+        size_t o_axx = B.add(vecd(O.axx, no)), o_axy = B.add(vecd(O.axy, no));
+        size_t o_ayy = B.add(vecd(O.ayy, no)), o_det = B.add(vecd(O.det, no));
+        size_t o_ndx = B.add(vecd(O.nb_dx, nnb)), o_ndy = B.add(vecd(O.nb_dy, nnb));
+        size_t o_nw = B.add(vecd(O.nb_w, nnb)), o_t = B.add(vecd(O.t, no));
+        size_t o_rv = B.add(vecd(O.ref_velocity, no)), o_rp = B.add(vecd(O.ref_pressure, no));
+        size_t o_rr = B.add(vecd(O.ref_density, no)), o_fl = B.add(floors);
+        for (int i = 0; i < no; ++i) {
+            const int t = O.type[i];
+            if (t < 0 || t > 2) throw Fail{RQ_EINVAL, "unsupported observation type"};
+            if ((t == OBS_VELOCITY || t == OBS_PRESSURE) && (O.cell[i] < 0 || O.cell[i] >= nc))
+                throw Fail{RQ_EINVAL, "observation probe cell out of range"};
+            if (O.nb_ptr[i + 1] < O.nb_ptr[i]) throw Fail{RQ_EINVAL, "invalid observation stencil"};
+            for (int q = O.nb_ptr[i]; q < O.nb_ptr[i + 1]; ++q)
+                if (O.nb_cell[q] >= nc || (O.nb_cell[q] < 0 && (O.nb_face[q] < nif || O.nb_face[q] >= nf)))
+                    throw Fail{RQ_EINVAL, "observation stencil index out of range"};
+            if (t == OBS_SKIN_FRICTION && (O.face_a[i] >= nf || O.face_b[i] >= nf))
+                throw Fail{RQ_EINVAL, "skin-friction stencil face out of range"};
+        }
+
+        HIPCHECK(hipGetDevice(&E->device));
+        HIPCHECK(hipMalloc(&E->d_blob, B.host.size()));
+        HIPCHECK(hipMemcpy(E->d_blob, B.host.data(), B.host.size(), hipMemcpyHostToDevice));
+        char *b0 = E->d_blob;
+        auto I = [&](size_t o) { return reinterpret_cast<const int *>(b0 + o); };
+        auto Dd = [&](size_t o) { return reinterpret_cast<const double *>(b0 + o); };
+        D.img = Dd(o_img);
+        D.nc = nc; D.nif = nif; D.nf = nf; D.nb = nb; D.npatch = np;
+        D.has_outlet = has_outlet; D.planar = planar;
+        D.wall_y = Dd(o_wy); D.bpatch = I(o_bp);
+        D.u_type = I(o_ut); D.p_type = I(o_ptp); D.k_type = I(o_kt);
+        D.w_type = I(o_wt); D.u_valx = Dd(o_uvx); D.u_valy = Dd(o_uvy); D.p_val = Dd(o_pv);
+        D.k_val = Dd(o_kv); D.w_val = Dd(o_wv);
+        D.nobs = no;
+        D.ob_type = I(o_ot); D.ob_comp = I(o_oc); D.ob_cell = I(o_ocell); D.ob_nb_ptr = I(o_onp);
+        D.ob_nb_cell = I(o_onc); D.ob_nb_face = I(o_onf); D.ob_fa = I(o_ofa); D.ob_fb = I(o_ofb);
+        D.ob_offx = Dd(o_ox); D.ob_offy = Dd(o_oy); D.ob_axx = Dd(o_axx); D.ob_axy = Dd(o_axy);
+        D.ob_ayy = Dd(o_ayy); D.ob_det = Dd(o_det); D.ob_nb_dx = Dd(o_ndx);
+        D.ob_nb_dy = Dd(o_ndy); D.ob_nb_w = Dd(o_nw); D.ob_t = Dd(o_t); D.ob_refv = Dd(o_rv);
+        D.ob_refp = Dd(o_rp); D.ob_refrho = Dd(o_rr); D.ob_floor = Dd(o_fl);
+        D.lay = L;
+        E->lay = L;
+        HIPCHECK(hipMalloc(&E->d_mesh, sizeof(DevMesh)));
+        HIPCHECK(hipMemcpy(E->d_mesh, &D, sizeof(DevMesh), hipMemcpyHostToDevice));
+
+        // ---- settings and launch shape
+        DevSettings &S = E->settings;
+        S.max_iterations = s.max_iterations;
+        S.convergence_tol = s.convergence_tol;
+        S.divergence_limit = s.divergence_limit;
+        S.convergence_window = s.convergence_window;
+        S.monitor_interval = s.monitor_interval;
+        S.alpha_u = s.alpha_u;
+        S.alpha_p = s.alpha_p;
+        S.alpha_k = s.alpha_k;
+        S.alpha_omega = s.alpha_omega;
+        S.inner_iterations = s.inner_iterations;
+        S.inner_tolerance = s.inner_tolerance;
+        S.turb_start_iter = s.turb_start_iter;
+        S.turb_update_interval = s.turb_update_interval;
+        S.nut_floor_iters = s.nut_floor_iters;
+        S.momentum_convection = s.momentum_convection;
+        S.turbulence_convection = s.turbulence_convection;
+        S.body_force_x = s.body_force_x;
+        S.body_force_y = s.body_force_y;
+        S.k_min = s.k_min;
+        S.k_max = s.k_max;
+        S.omega_min = s.omega_min;
+        S.pressure_solver = s.pressure_solver;
+        S.momentum_solver = s.momentum_solver;
+        S.turbulence_solver = s.turbulence_solver;
+        S.sst_variant = s.sst_variant;
+        S.nu = nu;
+        S.init_ux = init[0];
+        S.init_uy = init[1];
+        S.init_p = init[2];
+        S.init_k = init[3];
+        S.init_w = init[4];
+        S.mixed_precision = s.mixed_precision;
+        if (s.block_size) E->block = s.block_size;
+        else E->block = nc <= 128 ? 64 : nc <= 4096 ? 256 : 512;
+        int maxLds = 0;
+        HIPCHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                       E->device));
+        const size_t arenaBytes = (size_t)E->lay.stride * sizeof(double);
+        const size_t staticLds = 4096;
+        E->lds = arenaBytes + staticLds <= (size_t)maxLds;
+        E->lds_bytes = E->lds ? arenaBytes : 0;
+        HIPCHECK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&E->ev0));
+        HIPCHECK(hipEventCreate(&E->ev1));
+    } catch (...) {
+        delete E;
+        throw;
+    }
+    return E;
+}
+
+template <class T>
+void grow(T *&ptr, size_t &cap, size_t n) {
+    if (n <= cap) return;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    HIPCHECK(hipMalloc(&ptr, n * sizeof(T)));
+    cap = n;
+}
+
+void write_row_error(const rq_batch_out *out, int64_t row, const std::string &msg) {
+    if (out->row_errors && out->row_error_stride)
+        std::snprintf(out->row_errors + row * out->row_error_stride, out->row_error_stride, "%s",
+                      msg.c_str());
+}
+
+int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
+                  int32_t workers, rq_batch_out *out) {
+    if (!E || !out || !out->status) throw Fail{RQ_EINVAL, "null argument"};
+    if (workers < 1) throw Fail{RQ_EINVAL, "workers must be positive"};
+    if (n_params != E->n_active)
+        throw Fail{RQ_EINVAL, "batch rows must all have n_parameters entries"};
+    if (n_rows < 0) throw Fail{RQ_EINVAL, "row count must be nonnegative"};
+    if (n_rows > 0 && !theta) throw Fail{RQ_EINVAL, "null coefficient rows"};
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const int nobs = E->nobs;
+    std::vector<int64_t> valid;
+    std::vector<DevSlot> slots;
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *row = theta + r * n_params;
+        if (out->row_errors && out->row_error_stride) out->row_errors[r * out->row_error_stride] = 0;
+        if (out->predictions)
+            for (int i = 0; i < nobs; ++i) out->predictions[r * nobs + i] = nan;
+        if (out->iterations) out->iterations[r] = 0;
+        if (out->final_worst_measure) out->final_worst_measure[r] = 0.0;
+        if (out->contraction_rate_u) out->contraction_rate_u[r] = 0.0;
+        if (out->stalled) out->stalled[r] = 0;
+        if (out->converged) out->converged[r] = 0;
+        if (out->krylov_iterations)
+            for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = 0;
+        // InferenceParameterSet::inBounds (core/include/InferenceParameters.hpp:127-140)
+        bool ok = true;
+        for (int i = 0; i < n_params; ++i) {
+            const double v = row[i];
+            if (v < E->lower[i] || v > E->upper[i] || std::isnan(v) || std::isinf(v)) ok = false;
+        }
+        if (!ok) {
+            out->status[r] = ST_INVALID;
+            if (out->loglik) out->loglik[r] = -1e30;
+            write_row_error(out, r, "coefficient row is nonfinite or out of bounds");
+            continue;
+        }
+        DevSlot sl{};
+        sl.mesh = E->d_mesh;
+        for (int i = 0; i < 11; ++i) sl.coeffs[i] = E->base[i];
+        for (int i = 0; i < n_params; ++i) sl.coeffs[E->active[i]] = row[i];
+        // SSTCoefficients::validate (core/include/SSTModel.hpp:29-45)
+        bool cok = true;
+        for (int i = 0; i < 11; ++i)
+            if (!std::isfinite(sl.coeffs[i]) || sl.coeffs[i] < 0) cok = false;
+        if (!cok || sl.coeffs[2] == 0. || sl.coeffs[8] == 0.) {
+            out->status[r] = ST_UNKNOWN;
+            if (out->loglik) out->loglik[r] = -1e6;
+            write_row_error(out, r, "SST coefficients must be finite and nonnegative, with "
+                                    "positive beta1 and betaStar");
+            continue;
+        }
+        sl.row = (int)slots.size();
+        valid.push_back(r);
+        slots.push_back(sl);
+    }
+    if (slots.empty()) return RQ_OK;
+    const int ns = (int)slots.size();
+    HIPCHECK(hipSetDevice(E->device));
+    grow(E->d_slots, E->cap_slots, ns);
+    grow(E->d_results, E->cap_results, ns);
+    grow(E->d_pred, E->cap_pred, (size_t)ns * std::max(nobs, 1));
+    const int64_t fstride = 10 * (int64_t)E->nc + E->nf;
+    if (out->fields) grow(E->d_fields, E->cap_fields, (size_t)ns * fstride);
+    if (!E->lds) {
+        grow(E->d_arena, E->cap_arena, (size_t)ns * E->lay.stride);
+        for (int i = 0; i < ns; ++i) slots[i].arena = E->d_arena + (size_t)i * E->lay.stride;
+    }
+    HIPCHECK(hipMemcpyAsync(E->d_slots, slots.data(), ns * sizeof(DevSlot), hipMemcpyHostToDevice,
+                            E->stream));
+    HIPCHECK(hipEventRecord(E->ev0, E->stream));
+    HIPCHECK(launch_simple_solve(E->block, E->lds, E->d_slots, ns, E->settings, E->d_results,
+                                 E->d_pred, std::max(nobs, 1), out->fields ? E->d_fields : nullptr,
+                                 fstride, E->lds_bytes, E->stream));
+    HIPCHECK(hipEventRecord(E->ev1, E->stream));
+    std::vector<DevResult> res(ns);
+    std::vector<double> pred((size_t)ns * std::max(nobs, 1));
+    HIPCHECK(hipMemcpyAsync(res.data(), E->d_results, ns * sizeof(DevResult),
+                            hipMemcpyDeviceToHost, E->stream));
+    HIPCHECK(hipMemcpyAsync(pred.data(), E->d_pred, pred.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, E->stream));
+    std::vector<double> fields;
+    if (out->fields) {
+        fields.resize((size_t)ns * fstride);
+        HIPCHECK(hipMemcpyAsync(fields.data(), E->d_fields, fields.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, E->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(E->stream));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+    E->last_ms = ms;
+    for (int i = 0; i < ns; ++i) {
+        const int64_t r = valid[i];
+        const DevResult &d = res[i];
+        int status = d.status;
+        double ll = -1e6;
+        std::string msg;
+        if (status == ST_UNKNOWN)
+            msg = d.error == ERR_NONFINITE_MONITOR
+                      ? "convergence monitor returned a nonfinite observation"
+                      : "linear system coefficients and source must be finite";
+        if (status == ST_CONVERGED || status == ST_UNCONVERGED) {
+            // ObservationOperator::logLikelihood (ObservationOperator.hpp:592-607)
+            double acc = 0.0;
+            for (int j = 0; j < nobs; ++j) {
+                const double pj = pred[(size_t)i * nobs + j];
+                if (!std::isfinite(pj)) {
+                    msg = "Likelihood predictions must be finite";
+                    break;
+                }
+                const double diff = pj - E->observed[j];
+                acc += -0.5 * diff * diff / E->variance[j];
+                if (!std::isfinite(acc)) {
+                    msg = "Gaussian log-likelihood exceeds finite range";
+                    break;
+                }
+            }
+            if (msg.empty()) {
+                ll = acc;
+                if (out->predictions)
+                    for (int j = 0; j < nobs; ++j)
+                        out->predictions[r * nobs + j] = pred[(size_t)i * nobs + j];
+            } else {
+                status = ST_UNKNOWN;
+            }
+        }
+        out->status[r] = status;
+        if (out->loglik) out->loglik[r] = ll;
+        if (out->iterations) out->iterations[r] = d.iters;
+        if (out->final_worst_measure) out->final_worst_measure[r] = d.worst;
+        if (out->contraction_rate_u) out->contraction_rate_u[r] = d.rate;
+        if (out->stalled) out->stalled[r] = (uint8_t)d.stalled;
+        if (out->converged) out->converged[r] = status == ST_CONVERGED;
+        if (out->krylov_iterations)
+            for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = d.krylov[e];
+        if (!msg.empty()) write_row_error(out, r, msg);
+        if (out->fields)
+            std::memcpy(out->fields + r * fstride, fields.data() + (size_t)i * fstride,
+                        fstride * sizeof(double));
+    }
+    return RQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rq_engine *rq_engine_create(const rq_mesh_desc *mesh, const rq_param_desc *params,
+                            const rq_obs_desc *obs, const rq_bc_desc *bcs, double nu,
+                            const rq_settings *settings, const double init[5], char *err,
+                            size_t errcap) {
+    try {
+        rq_engine *e = create_impl(mesh, params, obs, bcs, nu, settings, init);
+        set_err(err, errcap, "");
+        return e;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+    } catch (const std::bad_alloc &) {
+        set_err(err, errcap, "host allocation failed");
+    } catch (const std::exception &e) {
+        set_err(err, errcap, e.what());
+    }
+    return nullptr;
+}
+
+int rq_engine_evaluate(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
+                       int32_t workers, rq_batch_out *out, char *err, size_t errcap) {
+    try {
+        int rc = evaluate_impl(engine, theta, n_rows, n_params, workers, out);
+        set_err(err, errcap, "");
+        return rc;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+        return f.code;
+    } catch (const std::bad_alloc &) {
+        set_err(err, errcap, "host allocation failed");
+        return RQ_ENOMEM;
+    } catch (const std::exception &e) {
+        set_err(err, errcap, e.what());
+        return RQ_EDEVICE;
+    }
+}
+
+int32_t rq_engine_n_observations(const rq_engine *e) { return e ? e->nobs : -1; }
+int32_t rq_engine_n_parameters(const rq_engine *e) { return e ? e->n_active : -1; }
+int32_t rq_engine_n_cells(const rq_engine *e) { return e ? e->nc : -1; }
+int32_t rq_engine_n_faces(const rq_engine *e) { return e ? e->nf : -1; }
+
+int rq_engine_last_timing(const rq_engine *e, double *kernel_ms, int32_t *block_size,
+                          int32_t *arena_in_lds, int64_t *arena_bytes) {
+    if (!e) return RQ_EINVAL;
+    if (kernel_ms) *kernel_ms = e->last_ms;
+    if (block_size) *block_size = e->block;
+    if (arena_in_lds) *arena_in_lds = e->lds ? 1 : 0;
+    if (arena_bytes) *arena_bytes = e->lay.stride * (int64_t)sizeof(double);
+    return RQ_OK;
+}
+
+void rq_engine_destroy(rq_engine *e) { delete e; }
+
+const char *rq_build_info(void) {
+    return "rans_amd: gfx950 HIP, FP64 assembly and convergence, -O3 -ffp-contract=off"
+#ifdef __clang_version__
+           " clang " __clang_version__
+#endif
+        ;
+}
+
+}  // extern "C"

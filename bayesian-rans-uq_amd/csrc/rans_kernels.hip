@@ -1,0 +1,1516 @@
+// gfx950 kernels for the incompressible SIMPLE + SST k-omega forward solve.
+//
+// One workgroup owns one draw (one coefficient vector on one mesh) for the
+// whole solve: a persistent kernel runs every outer iteration, every Krylov
+// iteration and the FP64 convergence decision on the device. The draw's
+// fields, matrices and Krylov vectors live in an arena that is placed in LDS
+// when it fits (kLDS) and in HBM otherwise; the mesh tables are shared by
+// all draws on the mesh and are read through the caches.
+//
+// All work is cell-centric: a cell gathers the contributions of its faces in
+// the order of its face list, which is the face order the reference's face
+// loops accumulate in. So no atomics are needed, results do not depend on the
+// batch composition, and the FP64 assembly reproduces the reference's
+// rounding. Block reductions use 64-lane butterflies, then LDS across waves,
+// in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "rans_device.h"
+
+namespace rq {
+
+#define DEV __device__ __forceinline__
+
+DEV double smax(double a, double b) { return (a < b) ? b : a; }
+DEV double smin(double a, double b) { return (b < a) ? b : a; }
+
+// ----------------------------------------------------------------------------
+// block reductions. A wave reduces with DPP lane moves in a fixed tree (pairs,
+// quads, half rows, rows, then row broadcasts), the total is read from lane
+// 63 into a scalar register, so every lane holds the same bits. Waves then
+// combine through LDS in wave order.
+// ----------------------------------------------------------------------------
+template <int CTRL, int ROW_MASK = 0xf>
+DEV double dpp_move(double v, double old) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL,
+                                               ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL,
+                                               ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+DEV double lane63(double v) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+DEV double wave_sum(double v) {
+    v += dpp_move<0xB1>(v, 0.0);        // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E>(v, 0.0);        // quad_perm [2,3,0,1]
+    v += dpp_move<0x141>(v, 0.0);       // row_half_mirror
+    v += dpp_move<0x140>(v, 0.0);       // row_mirror
+    v += dpp_move<0x142, 0xa>(v, 0.0);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v, 0.0);  // row_bcast:31 into rows 2 and 3
+    return lane63(v);
+}
+DEV double wave_max(double v) {
+    v = smax(v, dpp_move<0xB1>(v, v));
+    v = smax(v, dpp_move<0x4E>(v, v));
+    v = smax(v, dpp_move<0x141>(v, v));
+    v = smax(v, dpp_move<0x140>(v, v));
+    v = smax(v, dpp_move<0x142, 0xa>(v, v));
+    v = smax(v, dpp_move<0x143, 0xc>(v, v));
+    return lane63(v);
+}
+
+template <int BS>
+struct Reducer {
+    static constexpr int NW = BS / 64;
+    double *buf;  // LDS scratch [2][NW][4]
+    int parity;
+
+    template <int NV, bool MAX>
+    DEV void run(double (&v)[NV]) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = MAX ? wave_max(v[i]) : wave_sum(v[i]);
+        if constexpr (NW > 1) {
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            double *b = buf + parity * (NW * 4);
+            if (lane == 0)
+#pragma unroll
+                for (int i = 0; i < NV; ++i) b[wave * 4 + i] = v[i];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double a = b[i];
+                for (int w = 1; w < NW; ++w) a = MAX ? smax(a, b[w * 4 + i]) : a + b[w * 4 + i];
+                v[i] = a;
+            }
+            parity ^= 1;
+        } else {
+            __syncthreads();  // orders the phase's LDS writes before later reads
+        }
+    }
+    template <int NV>
+    DEV void sum(double (&v)[NV]) { run<NV, false>(v); }
+    template <int NV>
+    DEV void max(double (&v)[NV]) { run<NV, true>(v); }
+};
+
+// Arena pointers carry their address space explicitly (LDS or global), so
+// every access compiles to ds_* or global_* instructions rather than flat.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v2i __attribute__((ext_vector_type(2)));
+template <bool kLDS>
+struct Space;
+template <>
+struct Space<true> {
+    typedef __attribute__((address_space(3))) double D;
+    typedef __attribute__((address_space(3))) const int I;
+    typedef __attribute__((address_space(3))) const v4i I4;
+    typedef __attribute__((address_space(3))) const v2i I2;
+};
+template <>
+struct Space<false> {
+    typedef __attribute__((address_space(1))) double D;
+    typedef __attribute__((address_space(1))) const int I;
+    typedef __attribute__((address_space(1))) const v4i I4;
+    typedef __attribute__((address_space(1))) const v2i I2;
+};
+
+// slot code: face id in the low 30 bits, side (0 owner, 1 neighbour) in bit 30
+DEV int slot_face(int code) { return code & 0x3fffffff; }
+DEV bool slot_own(int code) { return (code >> 30) == 0; }
+
+// scalar bookkeeping of one draw, owned by thread 0
+struct Book {
+    int done, status, error, finalIter, stalled, haveQPrev, turbEst, pad;
+    double lastK, lastOm, normUx0, normUy0, normP0, pfRef, worst, rate;
+    long long nEst[4];
+    long long kry[5];
+};
+
+struct Krylov {
+    int iterations, converged;
+    double initial;
+};
+
+template <int BS, bool kLDS>
+struct Draw {
+    const DevMesh &M;
+    const DevSettings &S;
+    const double *c;  // 11 coefficients (LDS)
+    Book *bk;
+    Reducer<BS> red;
+    int nc, nif, nf, nb;
+    bool planar, has_outlet;
+    const int tid = threadIdx.x;
+    using D = typename Space<kLDS>::D;
+    D *Cb, *Fb, *Bb;                            // record bases
+    typename Space<kLDS>::I4 *slot4;            // [nc][2]: slot codes, neighbour cells
+    typename Space<kLDS>::I2 *fon;              // [nf]: owner, neighbour
+    typename Space<kLDS>::I *bflag;             // [nb]
+    D *ringb, *q, *qprev;
+
+    DEV Draw(const DevMesh &m, const DevSettings &s, const double *coeffs, double *A, Book *b,
+             double *redbuf)
+        : M(m), S(s), c(coeffs), bk(b) {
+        red.buf = redbuf;
+        red.parity = 0;
+        nc = M.nc;
+        nif = M.nif;
+        nf = M.nf;
+        nb = M.nb;
+        planar = M.planar;
+        has_outlet = M.has_outlet;
+        const ArenaLayout &L = M.lay;
+        Cb = (D *)A;
+        Fb = (D *)(A + L.face);
+        Bb = (D *)(A + L.bnd);
+        const int *ib = reinterpret_cast<const int *>(A + L.ibase);
+        slot4 = (typename Space<kLDS>::I4 *)(ib + L.islot);
+        fon = (typename Space<kLDS>::I2 *)(ib + L.iface);
+        bflag = (typename Space<kLDS>::I *)(ib + L.ibflag);
+        ringb = (D *)(A + L.ring);
+        q = (D *)(A + L.q);
+        qprev = (D *)(A + L.qprev);
+    }
+
+    // record accessors: cell, face and boundary-face variables
+    DEV D &C(int i, int v) const { return Cb[i * kNA + v]; }
+    DEV D &Fc(int f, int v) const { return Fb[f * kNF + v]; }
+    DEV D &Bd(int b, int v) const { return Bb[b * kNB + v]; }
+
+    // off-diagonal coefficient of slot s of cell ci
+    DEV D &off(int ci, int s) const { return Cb[ci * kNA + V_off0 + s]; }
+
+    // ---------------------------------------------------------------- SST
+    // SSTModel pointwise functions (core/src/SSTModel.cpp:15-108)
+    DEV double sst_f1(double kk, double omega, double y, double CDkwv) const {
+        const double nu = S.nu;
+        double sqrtK = sqrt(smax(kk, 0.0));
+        double omSafe = smax(omega, 1e-20);
+        double ySafe = smax(y, 1e-20);
+        double y2s = ySafe * ySafe;
+        double term1 = sqrtK / (c[8] * omSafe * ySafe);
+        double term2 = 500.0 * nu / (y2s * omSafe);
+        double CDpos = smax(CDkwv, 1e-10);
+        double term3 = 4.0 * c[5] * smax(kk, 0.0) / (CDpos * y2s);
+        double arg1 = smin(smax(term1, term2), term3);
+        double a4 = arg1 * arg1 * arg1 * arg1;
+        return tanh(a4);
+    }
+    DEV double sst_f2(double kk, double omega, double y) const {
+        const double nu = S.nu;
+        double sqrtK = sqrt(smax(kk, 0.0));
+        double omSafe = smax(omega, 1e-20);
+        double ySafe = smax(y, 1e-20);
+        double y2 = ySafe * ySafe;
+        double term1 = 2.0 * sqrtK / (c[8] * omSafe * ySafe);
+        double term2 = 500.0 * nu / (y2 * omSafe);
+        double arg2 = smax(term1, term2);
+        return tanh(arg2 * arg2);
+    }
+    DEV double sst_nut(double kk, double omega, double Sv, double f2) const {
+        double kSafe = smax(kk, 0.0);
+        if (S.sst_variant == 1) return kSafe / smax(omega, 1e-20);
+        double denom = smax(c[9] * omega, Sv * f2);
+        denom = smax(denom, 1e-20);
+        return c[9] * kSafe / denom;
+    }
+    DEV double sst_pk(double nt, double Sv, double kk, double omega) const {
+        double raw = nt * Sv * Sv;
+        double lim = 10.0 * c[8] * smax(kk, 0.0) * smax(omega, 1e-20);
+        return smin(raw, lim);
+    }
+    DEV double sst_pomega(double nt, double Sv, double kk, double omega) const {
+        double S2 = Sv * Sv;
+        double lim = 10.0 * c[8] * smax(kk, 0.0) * smax(omega, 1e-20) / smax(nt, 1e-30);
+        return smin(S2, lim);
+    }
+    DEV double blend(double a, double b, double f1) const { return f1 * a + (1.0 - f1) * b; }
+    // SIMPLESolver::resolvedWallOmega (SIMPLESolver.cpp:155-161)
+    DEV double resolved_wall_omega(double y) const {
+        const double coefficient = planar ? 6.0 : 60.0;
+        return coefficient * S.nu / (c[2] * y * y);
+    }
+
+    // ----------------------------------------------------------- boundary
+    // applyVelocityBC / applyPressureBC / applyKBC / applyOmegaBC
+    // (core/include/BoundaryCondition.hpp:279-384)
+    DEV void bc_velocity() {
+        for (int b = tid; b < nb; b += BS) {
+            const int f = nif + b, pi = M.bpatch[b], o = fon[f].x;
+            switch (M.u_type[pi]) {
+            case BC_WALL_NOSLIP:
+                Bd(b, B_Ux) = 0.0;
+                Bd(b, B_Uy) = 0.0;
+                break;
+            case BC_WALL_MOVING:
+            case BC_INLET:
+            case BC_DIRICHLET:
+                Bd(b, B_Ux) = M.u_valx[pi];
+                Bd(b, B_Uy) = M.u_valy[pi];
+                break;
+            case BC_SYMMETRY: {
+                const double nx = Fc(f, W_nx), ny = Fc(f, W_ny);
+                const double d = C(o, V_Ux) * nx + C(o, V_Uy) * ny;
+                Bd(b, B_Ux) = C(o, V_Ux) - nx * d;
+                Bd(b, B_Uy) = C(o, V_Uy) - ny * d;
+                break;
+            }
+            default:
+                Bd(b, B_Ux) = C(o, V_Ux);
+                Bd(b, B_Uy) = C(o, V_Uy);
+            }
+        }
+    }
+    DEV void bc_pressure() {
+        for (int b = tid; b < nb; b += BS) {
+            const int f = nif + b, pi = M.bpatch[b];
+            const int t = M.p_type[pi];
+            Bd(b, B_p) = (t == BC_OUTLET || t == BC_DIRICHLET) ? M.p_val[pi] : C(fon[f].x, V_p);
+        }
+    }
+    DEV void bc_k() {
+        for (int b = tid; b < nb; b += BS) {
+            const int f = nif + b, pi = M.bpatch[b];
+            const int t = M.k_type[pi];
+            Bd(b, B_k) = t == BC_WALL_KOMEGA ? 0.0 : (t == BC_DIRICHLET ? M.k_val[pi] : C(fon[f].x, V_k));
+        }
+    }
+    DEV void bc_omega(double beta1) {
+        for (int b = tid; b < nb; b += BS) {
+            const int f = nif + b, pi = M.bpatch[b];
+            const int t = M.w_type[pi];
+            double v;
+            if (t == BC_WALL_KOMEGA) {
+                const double y1 = Fc(f, W_dlt);
+                v = 60.0 * S.nu / (beta1 * y1 * y1);
+            } else if (t == BC_DIRICHLET) {
+                v = M.w_val[pi];
+            } else {
+                v = C(fon[f].x, V_w);
+            }
+            Bd(b, B_w) = v;
+        }
+    }
+
+    DEV void slots(int ci, int (&code)[kSlots], int (&nb_)[kSlots]) const {
+        const v4i a = slot4[2 * ci], b = slot4[2 * ci + 1];
+        code[0] = a.x; code[1] = a.y; code[2] = a.z; code[3] = a.w;
+        nb_[0] = b.x; nb_[1] = b.y; nb_[2] = b.z; nb_[3] = b.w;
+    }
+
+    // ----------------------------------------------------------- gradients
+    // greenGaussGrad (core/include/Field.hpp:178-212) of NF fields at one
+    // cell, its faces gathered in face order.
+    template <int NF>
+    DEV void gg_cell(int ci, const int (&ph)[NF], const int (&phb)[NF], double (&gx)[NF],
+                     double (&gy)[NF]) const {
+        int code[kSlots], nbv[kSlots];
+        slots(ci, code, nbv);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) gx[i] = gy[i] = 0.0;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (code[s] < 0) continue;
+            const int f = slot_face(code[s]);
+            const double Sx = Fc(f, W_Sx), Sy = Fc(f, W_Sy);
+            if (f < nif) {
+                const v2i on = fon[f];
+                const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+                const bool own = slot_own(code[s]);
+#pragma unroll
+                for (int i = 0; i < NF; ++i) {
+                    const double phiF = C(on.x, ph[i]) * wt + C(on.y, ph[i]) * wmv;
+                    if (own) {
+                        gx[i] = gx[i] + Sx * phiF;
+                        gy[i] = gy[i] + Sy * phiF;
+                    } else {
+                        gx[i] = gx[i] - Sx * phiF;
+                        gy[i] = gy[i] - Sy * phiF;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NF; ++i) {
+                    const double pbv = Bd(f - nif, phb[i]);
+                    gx[i] = gx[i] + Sx * pbv;
+                    gy[i] = gy[i] + Sy * pbv;
+                }
+            }
+        }
+        const double V = C(ci, V_vol);
+        if (V > 1e-30)
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+                gx[i] = gx[i] / V;
+                gy[i] = gy[i] / V;
+            }
+    }
+
+    // U_f . S_f (SIMPLESolver.cpp:842-852)
+    DEV double interp_flux(int f) const {
+        if (f < nif) {
+            const v2i on = fon[f];
+            const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+            const double ufx = C(on.x, V_Ux) * wt + C(on.y, V_Ux) * wmv;
+            const double ufy = C(on.x, V_Uy) * wt + C(on.y, V_Uy) * wmv;
+            return (ufx * Fc(f, W_nx) + ufy * Fc(f, W_ny)) * Fc(f, W_A);
+        }
+        const int b = f - nif;
+        return (Bd(b, B_Ux) * Fc(f, W_nx) + Bd(b, B_Uy) * Fc(f, W_ny)) * Fc(f, W_A);
+    }
+
+    // F (phi_f^HO - phi_U) of the deferred convection correction at an
+    // internal face (SIMPLESolver.cpp:495-532), 0 when the face flux is 0.
+    // The host stores x_f - x_U in dO/dN for linear upwind and x_D - x_U
+    // for the limited scheme.
+    DEV double convection_corr(int f, v2i on, int ph, int gx, int gy, int scheme) const {
+        const double Fl = Fc(f, W_phi);
+        if (Fl == 0.0) return 0.0;
+        const bool ownerUp = Fl > 0.0;
+        const int up = ownerUp ? on.x : on.y, dn = ownerUp ? on.y : on.x;
+        const double dx = ownerUp ? Fc(f, W_dOx) : Fc(f, W_dNx);
+        const double dy = ownerUp ? Fc(f, W_dOy) : Fc(f, W_dNy);
+        const double pu = C(up, ph);
+        double phiF;
+        if (scheme == CONV_LINEAR_UPWIND) {
+            phiF = pu + (C(up, gx) * dx + C(up, gy) * dy);
+        } else {
+            const double dphi = C(dn, ph) - pu;
+            double psi = 0.0;
+            if (fabs(dphi) > 1e-300) {
+                const double r = 2.0 * (C(up, gx) * dx + C(up, gy) * dy) / dphi - 1.0;
+                psi = r > 0.0 ? (r * r + r) / (r * r + 1.0) : 0.0;
+            }
+            phiF = pu + 0.5 * psi * dphi;
+        }
+        return Fl * (phiF - pu);
+    }
+
+    // y = A x for row ci (LinearSystem::matvec order: diagonal, then faces)
+    DEV double spmv(int ci, int x) const {
+        const v4i nb4 = slot4[2 * ci + 1];
+        const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        double y = C(ci, V_diag) * C(ci, x);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) y += off(ci, s) * C(nbv[s], x);
+        return y;
+    }
+
+    DEV bool row_finite(int ci, double dg, double sr) const {
+        const v4i nb4 = slot4[2 * ci + 1];
+        const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        bool ok = isfinite(dg) && isfinite(sr);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) ok = ok && isfinite(off(ci, s));
+        return ok;
+    }
+
+    // ------------------------------------------------------ momentum row
+    // SIMPLESolver::assembleMomentum (SIMPLESolver.cpp:303-482) for one cell;
+    // returns the row's residual at the current component values, the
+    // unrelaxed diagonal in aPu and the admission flag in ok.
+    DEV double momentum_row(int ci, int comp, double &aPu, bool &ok) {
+        const double nu = S.nu;
+        const int Ubv = comp == 0 ? B_Ux : B_Uy;
+        const int Uc = comp == 0 ? V_Ux : V_Uy;
+        int code[kSlots], nbv[kSlots];
+        slots(ci, code, nbv);
+        double dg = 0.0, sr = 0.0;
+        const double nuTc = C(ci, V_nuT);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (code[s] < 0) continue;
+            const int f = slot_face(code[s]);
+            const double Sf = Fc(f, W_A);
+            const double delta = Fc(f, W_dlt);
+            const double mFlux = Fc(f, W_phi);
+            if (f < nif) {
+                const double nuEff_o = nu + (slot_own(code[s]) ? nuTc : C(nbv[s], V_nuT));
+                const double nuEff_n = nu + (slot_own(code[s]) ? C(nbv[s], V_nuT) : nuTc);
+                const double nuEff_f = Fc(f, W_w) * nuEff_o + Fc(f, W_wm) * nuEff_n;
+                const double Df = nuEff_f * Sf / delta;
+                const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
+                if (slot_own(code[s])) {
+                    dg += Df + cPos;
+                    off(ci, s) = -(Df + cNeg);
+                } else {
+                    dg += Df + cNeg;
+                    off(ci, s) = -(Df + cPos);
+                }
+            } else {
+                const int b = f - nif;
+                const bool wall = bflag[b] & 1;
+                const double nuEff = wall ? nu : nu + nuTc;
+                const double Db = nuEff * Sf / delta;
+                if (mFlux >= 0) {
+                    dg += mFlux + Db;
+                    sr += Db * Bd(b, Ubv);
+                } else {
+                    dg += Db;
+                    sr += (Db - mFlux) * Bd(b, Ubv);
+                }
+            }
+        }
+        const double V = C(ci, V_vol);
+        const double dpdx = comp == 0 ? C(ci, V_gPx) : C(ci, V_gPy);
+        sr -= dpdx * V;
+        const double fb = comp == 0 ? S.body_force_x : S.body_force_y;
+        if (fb != 0.0) sr += fb * V;
+        // transpose stress div(nuT (grad U)^T), zero wall coefficient
+        // (core/include/StressOperators.hpp:31-81)
+        {
+            const int ga = comp == 0 ? V_gUxx : V_gUxy;
+            const int gb = comp == 0 ? V_gUyx : V_gUyy;
+            const double gac = C(ci, ga), gbc = C(ci, gb);
+            double tsrc = 0.0;
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                if (code[s] < 0) continue;
+                const int f = slot_face(code[s]);
+                if (f < nif) {
+                    const bool own = slot_own(code[s]);
+                    const int n = nbv[s];
+                    const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+                    const double nuo = own ? nuTc : C(n, V_nuT), nun = own ? C(n, V_nuT) : nuTc;
+                    const double gan_ = C(n, ga), gbn_ = C(n, gb);
+                    const double gao = own ? gac : gan_, gan = own ? gan_ : gac;
+                    const double gbo = own ? gbc : gbn_, gbn = own ? gbn_ : gbc;
+                    const double cf = wt * nuo + wmv * nun;
+                    const double gxf = gao * wt + gan * wmv;
+                    const double gyf = gbo * wt + gbn * wmv;
+                    const double flux = cf * (gxf * Fc(f, W_nx) + gyf * Fc(f, W_ny)) * Fc(f, W_A);
+                    if (own) tsrc += flux;
+                    else tsrc -= flux;
+                } else {
+                    const bool wall = bflag[f - nif] & 1;
+                    const double cf = wall ? 0.0 : nuTc;
+                    const double flux = cf * (gac * Fc(f, W_nx) + gbc * Fc(f, W_ny)) * Fc(f, W_A);
+                    tsrc += flux;
+                }
+            }
+            sr += tsrc;
+        }
+        if (S.momentum_convection != CONV_UPWIND) {
+            const int gx = comp == 0 ? V_gUxx : V_gUyx;
+            const int gy = comp == 0 ? V_gUxy : V_gUyy;
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                if (code[s] < 0) continue;
+                const int f = slot_face(code[s]);
+                if (f >= nif) continue;
+                const double corr = convection_corr(f, fon[f], Uc, gx, gy, S.momentum_convection);
+                if (slot_own(code[s])) sr -= corr;
+                else sr += corr;
+            }
+        }
+        const double alphaU = S.alpha_u;
+        const double uc = C(ci, Uc);
+        aPu = dg;
+        sr += (1.0 - alphaU) / alphaU * dg * uc;
+        dg /= alphaU;
+        C(ci, V_diag) = dg;
+        C(ci, V_src) = sr;
+        if (comp == 0) C(ci, V_aPs) = dg;
+        ok = row_finite(ci, dg, sr) && isfinite(uc);
+        double y = dg * uc;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) y += off(ci, s) * C(nbv[s], Uc);
+        return sr - y;
+    }
+
+    // ---------------------------------------------------------- k / omega rows
+    // SIMPLESolver::assembleKEquation (SIMPLESolver.cpp:947-1023)
+    DEV void k_row(int ci, bool &ok) {
+        const double nu = S.nu;
+        int code[kSlots], nbv[kSlots];
+        slots(ci, code, nbv);
+        double dg = 0.0, sr = 0.0;
+        const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (code[s] < 0) continue;
+            const int f = slot_face(code[s]);
+            const double Sf = Fc(f, W_A);
+            const double delta = Fc(f, W_dlt);
+            const double mFlux = Fc(f, W_phi);
+            if (f < nif) {
+                const bool own = slot_own(code[s]);
+                const int n = nbv[s];
+                const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+                const double F1o = own ? F1c : C(n, V_F1), F1n = own ? C(n, V_F1) : F1c;
+                const double nto = own ? nuTc : C(n, V_nuT), ntn = own ? C(n, V_nuT) : nuTc;
+                const double F1f = wt * F1o + wmv * F1n;
+                const double sk = blend(c[0], c[4], F1f);
+                const double nuEff_o = nu + sk * nto;
+                const double nuEff_n = nu + sk * ntn;
+                const double nuEff_f = wt * nuEff_o + wmv * nuEff_n;
+                const double Df = nuEff_f * Sf / delta;
+                const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
+                if (own) {
+                    dg += Df + cPos;
+                    off(ci, s) = -(Df + cNeg);
+                } else {
+                    dg += Df + cNeg;
+                    off(ci, s) = -(Df + cPos);
+                }
+            } else {
+                const int b = f - nif;
+                const double sk = blend(c[0], c[4], F1c);
+                const double nuEff = nu + sk * nuTc;
+                const double Db = nuEff * Sf / delta;
+                if (mFlux >= 0) {
+                    dg += mFlux + Db;
+                    sr += Db * Bd(b, B_k);
+                } else {
+                    dg += Db;
+                    sr += (Db - mFlux) * Bd(b, B_k);
+                }
+            }
+        }
+        const double V = C(ci, V_vol);
+        sr += C(ci, V_Pk) * V;
+        const double destruction = c[8] * smax(C(ci, V_w), 1e-20);
+        dg += destruction * V;
+        if (S.turbulence_convection != CONV_UPWIND) {
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                if (code[s] < 0) continue;
+                const int f = slot_face(code[s]);
+                if (f >= nif) continue;
+                const double corr = convection_corr(f, fon[f], V_k, V_gKx, V_gKy, S.turbulence_convection);
+                if (slot_own(code[s])) sr -= corr;
+                else sr += corr;
+            }
+        }
+        const double aK = S.alpha_k;
+        const double kc = C(ci, V_k);
+        sr += (1.0 - aK) / aK * dg * kc;
+        dg /= aK;
+        C(ci, V_diag) = dg;
+        C(ci, V_src) = sr;
+        ok = row_finite(ci, dg, sr) && isfinite(kc);
+    }
+
+    // SIMPLESolver::assembleOmegaEquation (SIMPLESolver.cpp:1030-1169) and
+    // constrainOmega (SIMPLESolver.cpp:686-704); returns the constrained
+    // initial value of the omega solve.
+    DEV double omega_row(int ci, bool &ok) {
+        const double nu = S.nu;
+        int code[kSlots], nbv[kSlots];
+        slots(ci, code, nbv);
+        double dg = 0.0, sr = 0.0, balance = 0.0;
+        const double amplitude = 6 * nu / c[2];
+        const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
+        const double omsc = planar ? C(ci, V_oms) : 0.0;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (code[s] < 0) continue;
+            const int f = slot_face(code[s]);
+            const double Sf = Fc(f, W_A);
+            const double delta = Fc(f, W_dlt);
+            const double mFlux = Fc(f, W_phi);
+            if (f < nif) {
+                const bool own = slot_own(code[s]);
+                const int n = nbv[s];
+                const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+                const double F1o = own ? F1c : C(n, V_F1), F1n = own ? C(n, V_F1) : F1c;
+                const double nto = own ? nuTc : C(n, V_nuT), ntn = own ? C(n, V_nuT) : nuTc;
+                const double F1f = wt * F1o + wmv * F1n;
+                const double sw = blend(c[1], c[5], F1f);
+                const double nuEff_o = nu + sw * nto;
+                const double nuEff_n = nu + sw * ntn;
+                const double nuEff_f = wt * nuEff_o + wmv * nuEff_n;
+                const double Df = nuEff_f * Sf / delta;
+                const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
+                if (own) {
+                    dg += Df + cPos;
+                    off(ci, s) = -(Df + cNeg);
+                } else {
+                    dg += Df + cNeg;
+                    off(ci, s) = -(Df + cPos);
+                }
+                if (planar) {
+                    const double omsn = C(n, V_oms);
+                    const double so = own ? omsc : omsn, sn = own ? omsn : omsc;
+                    const double difference = (sn - so) / delta;
+                    const double correction =
+                        amplitude * Sf * ((nuEff_f - nu) * Fc(f, W_omg) - nuEff_f * difference);
+                    if (own) balance += correction;
+                    else balance -= correction;
+                }
+            } else {
+                const int b = f - nif;
+                const double sw = blend(c[1], c[5], F1c);
+                const double nuEff = nu + sw * nuTc;
+                const double Db = nuEff * Sf / delta;
+                if (mFlux >= 0) {
+                    dg += mFlux + Db;
+                    sr += Db * Bd(b, B_w);
+                } else {
+                    dg += Db;
+                    sr += (Db - mFlux) * Bd(b, B_w);
+                }
+            }
+        }
+        const double V = C(ci, V_vol);
+        const double wc = C(ci, V_w);
+        const double omC = smax(wc, 1e-20);
+        const double alphaB = blend(c[3], c[7], F1c);
+        const double production = sst_pomega(nuTc, C(ci, V_S), C(ci, V_k), wc);
+        sr += alphaB * production * V;
+        const double betaB = blend(c[2], c[6], F1c);
+        dg += betaB * omC * V;
+        sr += (1.0 - F1c) * C(ci, V_CDkw) * V;
+        if (planar) {
+            const double correction = balance + nu * (6 * nu / c[2]) * C(ci, V_oml) * V;
+            if (correction < 0) dg -= correction / omC;
+            else sr += correction;
+        }
+        if (S.turbulence_convection != CONV_UPWIND) {
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                if (code[s] < 0) continue;
+                const int f = slot_face(code[s]);
+                if (f >= nif) continue;
+                const double corr = convection_corr(f, fon[f], V_w, V_gWx, V_gWy, S.turbulence_convection);
+                if (slot_own(code[s])) sr -= corr;
+                else sr += corr;
+            }
+        }
+        const double aW = S.alpha_omega;
+        sr += (1.0 - aW) / aW * dg * wc;
+        dg /= aW;
+        const double wvc = C(ci, V_wallv);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (nbv[s] < 0) continue;
+            const bool own = slot_own(code[s]);
+            const double vn_ = C(nbv[s], V_wallv);
+            const double vo = own ? wvc : vn_, vn = own ? vn_ : wvc;
+            if (isfinite(vo)) {
+                if (!own) sr -= off(ci, s) * vo;
+                off(ci, s) = 0.0;
+            } else if (isfinite(vn)) {
+                if (own) sr -= off(ci, s) * vn;
+                off(ci, s) = 0.0;
+            }
+        }
+        if (isfinite(wvc)) sr = dg * wvc;
+        C(ci, V_diag) = dg;
+        C(ci, V_src) = sr;
+        const double x0 = isfinite(wvc) ? wvc : wc;
+        ok = row_finite(ci, dg, sr) && isfinite(x0);
+        return x0;
+    }
+
+    // ------------------------------------------------------------ Krylov
+    DEV double jac(double r, double d) const { return (fabs(d) > 1e-30) ? r / d : r; }
+
+    // BiCGSTABSolver::solve (LinearSolver.hpp:259-343). On entry kr holds
+    // b - A x and kx holds x; rr = r.r.
+    DEV Krylov bicgstab(double rr, int maxIter, double tol) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        for (int ci = tid; ci < nc; ci += BS) {
+            C(ci, V_krh) = C(ci, V_kr);
+            C(ci, V_kp) = 0.0;
+            C(ci, V_kv) = 0.0;
+        }
+        double rho = 1, alpha = 1, omega = 1;
+        double rhoN = rr;  // rh == r on entry, so rh.r == r.r
+        for (int it = 0; it < maxIter; ++it) {
+            if (fabs(rhoN) < 1e-30) break;
+            const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double pv = C(ci, V_kr) + beta * (C(ci, V_kp) - omega * C(ci, V_kv));
+                C(ci, V_kp) = pv;
+                C(ci, V_kph) = jac(pv, C(ci, V_diag));
+            }
+            __syncthreads();
+            double a1[1] = {0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double vv = spmv(ci, V_kph);
+                C(ci, V_kv) = vv;
+                a1[0] += C(ci, V_krh) * vv;
+            }
+            red.sum(a1);
+            const double rv = a1[0];
+            if (fabs(rv) < 1e-30) break;
+            alpha = rhoN / rv;
+            double a2[1] = {0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double sv = C(ci, V_kr) - alpha * C(ci, V_kv);
+                C(ci, V_ks) = sv;
+                C(ci, V_ksh) = jac(sv, C(ci, V_diag));
+                a2[0] += sv * sv;
+            }
+            red.sum(a2);
+            const double sn = sqrt(a2[0]);
+            if (sn / r0 < tol) {
+                for (int ci = tid; ci < nc; ci += BS) C(ci, V_kx) += alpha * C(ci, V_kph);
+                __syncthreads();
+                res.iterations = it + 1;
+                res.converged = 1;
+                return res;
+            }
+            double a3[2] = {0.0, 0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double tv = spmv(ci, V_ksh);
+                C(ci, V_kt) = tv;
+                a3[0] += tv * tv;
+                a3[1] += tv * C(ci, V_ks);
+            }
+            red.sum(a3);
+            const double tt = a3[0];
+            omega = (tt > 1e-30) ? a3[1] / tt : 0.0;
+            double a4[2] = {0.0, 0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                C(ci, V_kx) += alpha * C(ci, V_kph) + omega * C(ci, V_ksh);
+                const double rv2 = C(ci, V_ks) - omega * C(ci, V_kt);
+                C(ci, V_kr) = rv2;
+                a4[0] += rv2 * rv2;
+                a4[1] += C(ci, V_krh) * rv2;
+            }
+            red.sum(a4);
+            const double rn = sqrt(a4[0]);
+            res.iterations = it + 1;
+            if (rn / r0 < tol) {
+                res.converged = 1;
+                return res;
+            }
+            if (fabs(omega) < 1e-30) break;
+            rho = rhoN;
+            rhoN = a4[1];
+        }
+        return res;
+    }
+
+    // PCGSolver::solve (LinearSolver.hpp:191-244) from x = 0. On entry kr
+    // holds b and rr = b.b.
+    DEV Krylov pcg(double rr, int maxIter, double tol) {
+        const int z = V_krh, pv = V_kp, Ap = V_kv;
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        double a0[1] = {0.0};
+        for (int ci = tid; ci < nc; ci += BS) {
+            const double zv = jac(C(ci, V_kr), C(ci, V_diag));
+            C(ci, z) = zv;
+            C(ci, pv) = zv;
+            a0[0] += C(ci, V_kr) * zv;
+        }
+        red.sum(a0);
+        double rz = a0[0];
+        for (int it = 0; it < maxIter; ++it) {
+            double a1[1] = {0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double av = spmv(ci, pv);
+                C(ci, Ap) = av;
+                a1[0] += C(ci, pv) * av;
+            }
+            red.sum(a1);
+            const double pAp = a1[0];
+            if (fabs(pAp) < 1e-30) break;
+            const double alpha = rz / pAp;
+            double a2[2] = {0.0, 0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                C(ci, V_kx) += alpha * C(ci, pv);
+                const double rv = C(ci, V_kr) - alpha * C(ci, Ap);
+                C(ci, V_kr) = rv;
+                const double zv = jac(rv, C(ci, V_diag));
+                C(ci, z) = zv;
+                a2[0] += rv * rv;
+                a2[1] += rv * zv;
+            }
+            red.sum(a2);
+            const double rn = sqrt(a2[0]);
+            res.iterations = it + 1;
+            if (rn / r0 < tol) {
+                res.converged = 1;
+                return res;
+            }
+            const double rzNew = a2[1];
+            const double beta = rzNew / (rz + 1e-30);
+            for (int ci = tid; ci < nc; ci += BS) C(ci, pv) = C(ci, z) + beta * C(ci, pv);
+            __syncthreads();
+            rz = rzNew;
+        }
+        return res;
+    }
+
+    DEV Krylov solve(int kind, double rr, bool zeroStart) {
+        if (kind == SOLVER_PCG && zeroStart) return pcg(rr, S.inner_iterations, S.inner_tolerance);
+        return bicgstab(rr, S.inner_iterations, S.inner_tolerance);
+    }
+
+    // ------------------------------------------------------ contraction
+    // ContractionEstimate (SIMPLESolver.cpp:32-74) on a ring of 2m samples
+    DEV D *ring(int e) const { return ringb + e * 2 * S.convergence_window; }
+    DEV void ce_push(int e, double d) {
+        const int m2 = 2 * S.convergence_window;
+        ring(e)[bk->nEst[e] % m2] = d;
+        bk->nEst[e]++;
+    }
+    DEV double ce_at(int e, long long i) const { return ring(e)[i % (2 * S.convergence_window)]; }
+    DEV double ce_last(int e) const {
+        return bk->nEst[e] == 0 ? INFINITY : ce_at(e, bk->nEst[e] - 1);
+    }
+    DEV double ce_rate(int e) const {
+        const long long n = bk->nEst[e], m = S.convergence_window;
+        if (n < 2 * m) return NAN;
+        double dNew = 0.0, dOld = 0.0;
+        for (long long i = n - m; i < n; ++i) dNew = smax(dNew, ce_at(e, i));
+        for (long long i = n - 2 * m; i < n - m; ++i) dOld = smax(dOld, ce_at(e, i));
+        if (dOld <= 0.0) return dNew > 0.0 ? INFINITY : 0.0;
+        return pow(dNew / dOld, 1.0 / (double)m);
+    }
+    DEV double ce_remaining(int e, double negligible) const {
+        const long long n = bk->nEst[e], m = S.convergence_window;
+        if (n < 2 * m) return INFINITY;
+        double dNew = 0.0;
+        for (long long i = n - m; i < n; ++i) dNew = smax(dNew, ce_at(e, i));
+        if (dNew <= negligible) return dNew;
+        const double r = ce_rate(e);
+        if (!(r < 1.0)) return INFINITY;
+        return dNew * r / (1.0 - r);
+    }
+
+    // ------------------------------------------------------- observation
+    // PreparedObservation::evaluate (ObservationOperator.hpp:678-772)
+    DEV double observe(int i) const {
+        const int type = M.ob_type[i];
+        if (type == OBS_VELOCITY || type == OBS_PRESSURE) {
+            const bool pressure = type == OBS_PRESSURE;
+            const int comp = M.ob_comp[i];
+            const bool zero = !pressure && comp == 2;
+            const int cell = pressure ? V_p : (comp == 0 ? V_Ux : V_Uy);
+            const int bnd = pressure ? B_p : (comp == 0 ? B_Ux : B_Uy);
+            const double center = zero ? 0.0 : C(M.ob_cell[i], cell);
+            double bx = 0, by = 0;
+            for (int qq = M.ob_nb_ptr[i]; qq < M.ob_nb_ptr[i + 1]; ++qq) {
+                const double nbv = zero ? 0.0
+                                        : (M.ob_nb_cell[qq] < 0 ? Bd(M.ob_nb_face[qq] - nif, bnd)
+                                                                : C(M.ob_nb_cell[qq], cell));
+                const double difference = nbv - center;
+                bx += M.ob_nb_w[qq] * M.ob_nb_dx[qq] * difference;
+                by += M.ob_nb_w[qq] * M.ob_nb_dy[qq] * difference;
+            }
+            double gxv = 0, gyv = 0;
+            if (fabs(M.ob_det[i]) > 1e-14 * (M.ob_axx[i] * M.ob_ayy[i] + 1e-300)) {
+                gxv = (M.ob_ayy[i] * bx - M.ob_axy[i] * by) / M.ob_det[i];
+                gyv = (M.ob_axx[i] * by - M.ob_axy[i] * bx) / M.ob_det[i];
+            }
+            const double value = center + gxv * M.ob_offx[i] + gyv * M.ob_offy[i];
+            const double dynP = .5 * M.ob_refrho[i] * M.ob_refv[i] * M.ob_refv[i];
+            return pressure ? (value - M.ob_refp[i]) / smax(dynP, 1e-20) : value;
+        }
+        double value = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int f = pass == 0 ? M.ob_fa[i] : M.ob_fb[i];
+            if (f < 0) break;
+            const int ow = fon[f].x;
+            const double delta = Fc(f, W_dlt);
+            const double nx = Fc(f, W_nx), ny = Fc(f, W_ny);
+            const double un = C(ow, V_Ux) * nx + C(ow, V_Uy) * ny;
+            const double utx = C(ow, V_Ux) - nx * un, uty = C(ow, V_Uy) - ny * un;
+            const double tau = S.nu * sqrt(utx * utx + uty * uty) / delta;
+            const double dynP = 0.5 * M.ob_refv[i] * M.ob_refv[i];
+            const double cf = tau / smax(dynP, 1e-20);
+            value = pass == 0 ? cf : (1.0 - M.ob_t[i]) * value + M.ob_t[i] * cf;
+        }
+        return value;
+    }
+
+    DEV void fail(int code) {
+        if (tid == 0) {
+            bk->done = 1;
+            bk->status = ST_UNKNOWN;
+            bk->error = code;
+        }
+        __syncthreads();
+    }
+
+    // ----------------------------------------------------------- the solve
+    // SIMPLESolver::solve (SIMPLESolver.cpp:1193-1716) after initUniform
+    // (SIMPLESolver.cpp:267-298).
+    DEV void run() {
+        const double nu = S.nu;
+        for (int ci = tid; ci < nc; ci += BS) {
+            C(ci, V_Ux) = S.init_ux;
+            C(ci, V_Uy) = S.init_uy;
+            C(ci, V_p) = S.init_p;
+            C(ci, V_k) = S.init_k;
+            C(ci, V_F1) = 1.0;
+            C(ci, V_F2) = 1.0;
+            C(ci, V_Pk) = 0.0;
+            C(ci, V_CDkw) = 0.0;
+            const double y = smax(C(ci, V_wd), 1e-20);
+            const double om = smax(resolved_wall_omega(y), S.init_w);
+            C(ci, V_w) = om;
+            C(ci, V_nuT) = c[9] * S.init_k / smax(c[9] * om, 1e-20);
+            const double wy = M.wall_y[ci];
+            C(ci, V_wallv) = isfinite(wy) ? resolved_wall_omega(wy) : NAN;
+        }
+        if (tid == 0) {
+            bk->done = 0;
+            bk->status = ST_UNCONVERGED;
+            bk->error = ERR_NONE;
+            bk->finalIter = 0;
+            bk->stalled = 0;
+            bk->haveQPrev = 0;
+            bk->turbEst = 0;
+            bk->lastK = bk->lastOm = 1.0;
+            bk->normUx0 = bk->normUy0 = bk->normP0 = 1.0;
+            bk->pfRef = 0.0;
+            bk->worst = 0.0;
+            bk->rate = 0.0;
+            for (int e = 0; e < 4; ++e) bk->nEst[e] = 0;
+            for (int e = 0; e < 5; ++e) bk->kry[e] = 0;
+        }
+        __syncthreads();
+        bc_velocity();
+        bc_pressure();
+        bc_k();
+        bc_omega(0.075);
+        __syncthreads();
+        for (int f = tid; f < nf; f += BS) Fc(f, W_phi) = interp_flux(f);
+        __syncthreads();
+
+        const double tol = S.convergence_tol;
+        const double negligible = 0.01 * tol;
+        int turbEst = 0;
+        for (int iter = 0; iter < S.max_iterations; ++iter) {
+            for (int ci = tid; ci < nc; ci += BS) {
+                C(ci, V_UxP) = C(ci, V_Ux);
+                C(ci, V_UyP) = C(ci, V_Uy);
+            }
+            for (int f = tid; f < nif; f += BS) Fc(f, W_phiBar) = interp_flux(f);
+            const bool turbActive = iter >= S.turb_start_iter;
+            const bool turbUpdate =
+                turbActive && ((iter - S.turb_start_iter) % S.turb_update_interval == 0);
+            if (turbUpdate) {
+                if (!turbEst && iter >= S.turb_start_iter + S.nut_floor_iters) turbEst = 1;
+                const double nuTMin = turbEst ? 0.0 : 0.1 * nu;
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const int ph5[5] = {V_Ux, V_Uy, V_k, V_w, V_p};
+                    const int pb5[5] = {B_Ux, B_Uy, B_k, B_w, B_p};
+                    double gx[5], gy[5];
+                    gg_cell<5>(ci, ph5, pb5, gx, gy);
+                    C(ci, V_gUxx) = gx[0];
+                    C(ci, V_gUxy) = gy[0];
+                    C(ci, V_gUyx) = gx[1];
+                    C(ci, V_gUyy) = gy[1];
+                    C(ci, V_gKx) = gx[2];
+                    C(ci, V_gKy) = gy[2];
+                    C(ci, V_gWx) = gx[3];
+                    C(ci, V_gWy) = gy[3];
+                    C(ci, V_gPx) = gx[4];
+                    C(ci, V_gPy) = gy[4];
+                    // SSTModel::computeFields (SSTModel.cpp:208-240) and the
+                    // startup nuT floor (SIMPLESolver.cpp:1333-1337)
+                    const double S11 = gx[0], S22 = gy[1];
+                    const double S12 = 0.5 * (gy[0] + gx[1]);
+                    const double Sc = sqrt(2.0 * (S11 * S11 + S22 * S22 + 2.0 * (S12 * S12)));
+                    C(ci, V_S) = Sc;
+                    const double kc = smax(C(ci, V_k), 0.0);
+                    const double wc = smax(C(ci, V_w), 1e-20);
+                    const double y = smax(C(ci, V_wd), 1e-20);
+                    const double omSafe = smax(wc, 1e-20);
+                    const double cd = 2.0 * c[5] / omSafe * (gx[2] * gx[3] + gy[2] * gy[3]);
+                    C(ci, V_CDkw) = cd;
+                    C(ci, V_F1) = (S.sst_variant == 2) ? 1.0 : sst_f1(kc, wc, y, cd);
+                    const double f2 = sst_f2(kc, wc, y);
+                    C(ci, V_F2) = f2;
+                    const double nt = sst_nut(kc, wc, Sc, f2);
+                    C(ci, V_Pk) = sst_pk(nt, Sc, kc, wc);
+                    C(ci, V_nuT) = smax(nt, nuTMin);
+                }
+            } else {
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const int ph3[3] = {V_Ux, V_Uy, V_p};
+                    const int pb3[3] = {B_Ux, B_Uy, B_p};
+                    double gx[3], gy[3];
+                    gg_cell<3>(ci, ph3, pb3, gx, gy);
+                    C(ci, V_gUxx) = gx[0];
+                    C(ci, V_gUxy) = gy[0];
+                    C(ci, V_gUyx) = gx[1];
+                    C(ci, V_gUyy) = gy[1];
+                    C(ci, V_gPx) = gx[2];
+                    C(ci, V_gPy) = gy[2];
+                }
+            }
+            __syncthreads();
+
+            // ---- x momentum (SIMPLESolver.cpp:1342-1379)
+            double momResid, momScale;
+            Krylov rUx, rUy;
+            {
+                double a[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int ci = tid; ci < nc; ci += BS) {
+                    double aPu;
+                    bool ok;
+                    const double r = momentum_row(ci, 0, aPu, ok);
+                    const double ux = C(ci, V_Ux), uy = C(ci, V_Uy);
+                    C(ci, V_kr) = r;
+                    C(ci, V_kx) = ux;
+                    a[0] += fabs(r);
+                    const double contribution = aPu * sqrt(ux * ux + uy * uy);
+                    a[1] += contribution;
+                    a[2] += ok ? 0.0 : 1.0;
+                    a[3] += r * r;
+                }
+                red.sum(a);
+                momResid = a[0];
+                momScale = a[1];
+                if (a[2] != 0.0) {
+                    fail(ERR_NONFINITE_SYSTEM);
+                    break;
+                }
+                rUx = solve(S.momentum_solver, a[3], false);
+                for (int ci = tid; ci < nc; ci += BS) C(ci, V_Ux) = C(ci, V_kx);
+                __syncthreads();
+            }
+            // ---- y momentum: its transpose stress sees the new Ux
+            {
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const int ph1[1] = {V_Ux};
+                    const int pb1[1] = {B_Ux};
+                    double gx[1], gy[1];
+                    gg_cell<1>(ci, ph1, pb1, gx, gy);
+                    C(ci, V_gUxx) = gx[0];
+                    C(ci, V_gUxy) = gy[0];
+                }
+                __syncthreads();
+                double a[3] = {0.0, 0.0, 0.0};
+                for (int ci = tid; ci < nc; ci += BS) {
+                    double aPu;
+                    bool ok;
+                    const double r = momentum_row(ci, 1, aPu, ok);
+                    C(ci, V_kr) = r;
+                    C(ci, V_kx) = C(ci, V_Uy);
+                    a[0] += fabs(r);
+                    a[1] += ok ? 0.0 : 1.0;
+                    a[2] += r * r;
+                }
+                red.sum(a);
+                momResid = smax(momResid, a[0]);
+                if (a[1] != 0.0) {
+                    fail(ERR_NONFINITE_SYSTEM);
+                    break;
+                }
+                rUy = solve(S.momentum_solver, a[2], false);
+                for (int ci = tid; ci < nc; ci += BS) C(ci, V_Uy) = C(ci, V_kx);
+                __syncthreads();
+            }
+            bc_velocity();
+            __syncthreads();
+
+            // ---- predicted Rhie-Chow flux and the pressure correction
+            //      (SIMPLESolver.cpp:1410-1432, 717-830, 870-884)
+            const double relaxCorr = 1.0 - S.alpha_u;
+            double ms[1] = {0.0};
+            for (int f = tid; f < nf; f += BS) {
+                double m = interp_flux(f);
+                double coeff = 0.0;
+                if (f < nif) {
+                    const v2i on = fon[f];
+                    const int o = on.x, n = on.y;
+                    const double delta = Fc(f, W_dlt);
+                    const double ao = C(o, V_aPs), an = C(n, V_aPs);
+                    const double dP_o = (fabs(ao) > 1e-30) ? C(o, V_vol) / ao : 0.0;
+                    const double dP_n = (fabs(an) > 1e-30) ? C(n, V_vol) / an : 0.0;
+                    const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+                    const double dP_f = wt * dP_o + wmv * dP_n;
+                    const double gbx = C(o, V_gPx) * wt + C(n, V_gPx) * wmv;
+                    const double gby = C(o, V_gPy) * wt + C(n, V_gPy) * wmv;
+                    const double Af = Fc(f, W_A);
+                    m += -dP_f * Af * ((C(n, V_p) - C(o, V_p)) / delta - (gbx * Fc(f, W_ex) + gby * Fc(f, W_ey)));
+                    m += relaxCorr * (Fc(f, W_phi) - Fc(f, W_phiBar));
+                    coeff = dP_f * Af / delta;
+                } else if (bflag[f - nif] & 2) {
+                    const int o = fon[f].x;
+                    const double ao = C(o, V_aPs);
+                    const double dP_o = (fabs(ao) > 1e-30) ? C(o, V_vol) / ao : 0.0;
+                    coeff = dP_o * Fc(f, W_A) / Fc(f, W_dlt);
+                }
+                Fc(f, W_phiPred) = m;
+                Fc(f, W_pCoeff) = coeff;
+                ms[0] += fabs(m);
+            }
+            __syncthreads();
+            double a8[3] = {0.0, 0.0, 0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                int code[kSlots], nbv[kSlots];
+                slots(ci, code, nbv);
+                double dg = 0.0, sr = 0.0;
+#pragma unroll
+                for (int s = 0; s < kSlots; ++s) {
+                    if (code[s] < 0) continue;
+                    const int f = slot_face(code[s]);
+                    if (f < nif) {
+                        const double coeff = Fc(f, W_pCoeff);
+                        dg += coeff;
+                        off(ci, s) = -coeff;
+                        if (slot_own(code[s])) sr -= Fc(f, W_phiPred);
+                        else sr += Fc(f, W_phiPred);
+                    } else {
+                        sr -= Fc(f, W_phiPred);
+                        if (bflag[f - nif] & 2) dg += Fc(f, W_pCoeff);
+                    }
+                }
+                if (!has_outlet && ci == 0) dg += fabs(dg) > 1e-30 ? dg : 1.0;
+                C(ci, V_diag) = dg;
+                C(ci, V_src) = sr;
+                C(ci, V_kr) = sr;  // b - A x at x = 0
+                C(ci, V_kx) = 0.0;
+                a8[0] += fabs(sr);
+                a8[1] += sr * sr;
+                a8[2] += row_finite(ci, dg, sr) ? 0.0 : 1.0;
+            }
+            red.sum(ms);
+            red.sum(a8);
+            const double massFluxScale = ms[0];
+            const double contScaled = a8[0] / smax(massFluxScale, 1e-300);
+            if (a8[2] != 0.0) {
+                fail(ERR_NONFINITE_SYSTEM);
+                break;
+            }
+            const Krylov rP = solve(S.pressure_solver, a8[1], true);
+            for (int ci = tid; ci < nc; ci += BS) C(ci, V_pP) = C(ci, V_kx);
+            __syncthreads();
+            for (int b = tid; b < nb; b += BS) Bd(b, B_pP) = (bflag[b] & 2) ? 0.0 : C(fon[nif + b].x, V_pP);
+            __syncthreads();
+            // ---- correctVelocity / correctPressure (SIMPLESolver.cpp:916-940)
+            for (int ci = tid; ci < nc; ci += BS) {
+                const int ph1[1] = {V_pP};
+                const int pb1[1] = {B_pP};
+                double gx[1], gy[1];
+                gg_cell<1>(ci, ph1, pb1, gx, gy);
+                const double ac = C(ci, V_aPs);
+                const double rAP = (fabs(ac) > 1e-30) ? 1.0 / ac : 0.0;
+                const double V = C(ci, V_vol);
+                C(ci, V_Ux) -= rAP * gx[0] * V;
+                C(ci, V_Uy) -= rAP * gy[0] * V;
+                C(ci, V_p) += S.alpha_p * C(ci, V_pP);
+            }
+            __syncthreads();
+            bc_velocity();
+            bc_pressure();
+            for (int f = tid; f < nf; f += BS) {
+                const v2i on = fon[f];
+                if (f >= nif) Fc(f, W_phi) = Fc(f, W_phiPred) + Fc(f, W_pCoeff) * C(on.x, V_pP);
+                else Fc(f, W_phi) = Fc(f, W_phiPred) - Fc(f, W_pCoeff) * (C(on.y, V_pP) - C(on.x, V_pP));
+            }
+            __syncthreads();
+
+            // ---- turbulence transport (SIMPLESolver.cpp:1476-1538)
+            if (turbUpdate) {
+                double a9[2] = {0.0, 0.0};
+                for (int ci = tid; ci < nc; ci += BS) {
+                    bool ok;
+                    k_row(ci, ok);
+                    C(ci, V_kx) = C(ci, V_k);
+                    a9[1] += ok ? 0.0 : 1.0;
+                }
+                __syncthreads();
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const double r = C(ci, V_src) - spmv(ci, V_kx);
+                    C(ci, V_kr) = r;
+                    a9[0] += r * r;
+                }
+                red.sum(a9);
+                if (a9[1] != 0.0) {
+                    fail(ERR_NONFINITE_SYSTEM);
+                    break;
+                }
+                const Krylov rK = solve(S.turbulence_solver, a9[0], false);
+                double mk[2] = {1e-30, 0.0};
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const double kn = smax(S.k_min, smin(S.k_max, C(ci, V_kx)));
+                    mk[0] = smax(mk[0], fabs(kn));
+                    mk[1] = smax(mk[1], fabs(kn - C(ci, V_k)));
+                    C(ci, V_k) = kn;
+                }
+                __syncthreads();
+                bc_k();
+                __syncthreads();
+                double a10[2] = {0.0, 0.0};
+                for (int ci = tid; ci < nc; ci += BS) {
+                    bool ok;
+                    const double x0 = omega_row(ci, ok);
+                    C(ci, V_kx) = x0;
+                    a10[1] += ok ? 0.0 : 1.0;
+                }
+                __syncthreads();
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const double r = C(ci, V_src) - spmv(ci, V_kx);
+                    C(ci, V_kr) = r;
+                    a10[0] += r * r;
+                }
+                red.sum(a10);
+                if (a10[1] != 0.0) {
+                    fail(ERR_NONFINITE_SYSTEM);
+                    break;
+                }
+                const Krylov rW = solve(S.turbulence_solver, a10[0], false);
+                double mw[2] = {1e-30, 0.0};
+                for (int ci = tid; ci < nc; ci += BS) {
+                    double wn = smax(S.omega_min, smin(1e15, C(ci, V_kx)));
+                    const double wvc = C(ci, V_wallv);
+                    if (isfinite(wvc)) wn = wvc;
+                    mw[0] = smax(mw[0], fabs(wn));
+                    mw[1] = smax(mw[1], fabs(wn - C(ci, V_w)));
+                    C(ci, V_w) = wn;
+                }
+                __syncthreads();
+                bc_omega(c[2]);
+                double m4[4] = {mk[0], mk[1], mw[0], mw[1]};
+                red.max(m4);
+                if (tid == 0) {
+                    const double kChange = m4[1] / m4[0], omChange = m4[3] / m4[2];
+                    bk->lastK = kChange;
+                    bk->lastOm = omChange;
+                    ce_push(1, kChange);
+                    ce_push(2, omChange);
+                    bk->kry[3] += rK.iterations;
+                    bk->kry[4] += rW.iterations;
+                }
+            }
+
+            // ---- convergence bookkeeping (SIMPLESolver.cpp:1545-1705)
+            double mu[2] = {1e-300, 0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double ux = C(ci, V_Ux), uy = C(ci, V_Uy);
+                mu[0] = smax(mu[0], sqrt(ux * ux + uy * uy));
+                const double dx = ux - C(ci, V_UxP), dy = uy - C(ci, V_UyP);
+                mu[1] = smax(mu[1], sqrt(dx * dx + dy * dy));
+            }
+            red.max(mu);
+            const bool monitor =
+                M.nobs > 0 && iter % (S.monitor_interval > 1 ? S.monitor_interval : 1) == 0;
+            if (monitor) {
+                for (int i = tid; i < M.nobs; i += BS) q[i] = observe(i);
+                __syncthreads();
+            }
+            if (tid == 0) {
+                bk->turbEst = turbEst;
+                bk->kry[0] += rUx.iterations;
+                bk->kry[1] += rUy.iterations;
+                bk->kry[2] += rP.iterations;
+                bookkeeping(iter, turbActive, momResid, momScale, contScaled, massFluxScale,
+                            rUx.initial, rUy.initial, rP.initial, mu[0], mu[1], monitor,
+                            negligible, tol);
+            }
+            __syncthreads();
+            if (bk->done) break;
+        }
+        if (tid == 0 && !bk->done) {
+            bk->finalIter = S.max_iterations;
+            bk->stalled = isfinite(bk->rate) && bk->rate >= 1.0;
+            bk->status = ST_UNCONVERGED;
+        }
+        __syncthreads();
+    }
+
+    DEV void bookkeeping(int iter, bool turbActive, double momResid, double momScale,
+                         double contScaled, double massFluxScale, double eUx, double eUy,
+                         double eP, double uMax, double duMax, bool monitor, double negligible,
+                         double tol) {
+        const double momentumScaled = momResid / smax(momScale, 1e-300);
+        const double changeU = duMax / uMax;
+        ce_push(0, changeU);
+        const double remainingU = ce_remaining(0, negligible);
+        if (monitor) {
+            bool finite = true;
+            for (int i = 0; i < M.nobs; ++i) finite = finite && isfinite(q[i]);
+            if (!finite) {
+                bk->done = 1;
+                bk->status = ST_UNKNOWN;
+                bk->error = ERR_NONFINITE_MONITOR;
+                return;
+            }
+            if (bk->haveQPrev) {
+                double qMax = 0.0, dq = 0.0;
+                for (int i = 0; i < M.nobs; ++i) qMax = smax(qMax, fabs(q[i]));
+                for (int i = 0; i < M.nobs; ++i) {
+                    const double den = smax(smax(fabs(q[i]), M.ob_floor[i]), 1e-300);
+                    dq = smax(dq, fabs(q[i] - qprev[i]) / den);
+                }
+                ce_push(3, dq);
+            }
+            for (int i = 0; i < M.nobs; ++i) qprev[i] = q[i];
+            bk->haveQPrev = 1;
+        }
+        double worst = smax(smax(smax(momentumScaled, contScaled), changeU), remainingU);
+        if (turbActive) {
+            worst = smax(worst, bk->lastK);
+            worst = smax(worst, ce_remaining(1, negligible));
+            worst = smax(worst, bk->lastOm);
+            worst = smax(worst, ce_remaining(2, negligible));
+        }
+        if (M.nobs > 0) {
+            worst = smax(worst, ce_last(3));
+            worst = smax(worst, ce_remaining(3, negligible));
+        }
+        if (!isfinite(worst)) worst = INFINITY;
+        if (iter == 0) {
+            bk->normUx0 = smax(eUx, 1e-30);
+            bk->normUy0 = smax(eUy, 1e-30);
+            bk->normP0 = smax(eP, 1e-30);
+            bk->pfRef = massFluxScale;
+        } else if (iter < 50) {
+            bk->normP0 = smax(bk->normP0, eP);
+            bk->pfRef = smax(bk->pfRef, massFluxScale);
+        }
+        const double normMom0 = smax(bk->normUx0, bk->normUy0);
+        const double nUx = normMom0 < 1e-20 ? 0.0 : eUx / normMom0;
+        const double nUy = normMom0 < 1e-20 ? 0.0 : eUy / normMom0;
+        const double pref = smax(bk->normP0, bk->pfRef);
+        const double nP = pref < 1e-20 ? 0.0 : eP / pref;
+        double maxRes = smax(smax(nUx, nUy), nP);
+        const double eK = turbActive ? bk->lastK : 0.0, eOm = turbActive ? bk->lastOm : 0.0;
+        if (turbActive) maxRes = smax(smax(maxRes, bk->lastK), bk->lastOm);
+        if (!isfinite(eUx) || !isfinite(eUy) || !isfinite(eP) || !isfinite(eK) ||
+            !isfinite(eOm) || !isfinite(bk->lastK) || !isfinite(bk->lastOm) ||
+            !isfinite(momentumScaled) || !isfinite(contScaled) || !isfinite(changeU)) {
+            maxRes = INFINITY;
+            worst = INFINITY;
+        }
+        const bool turbScheduled = S.turb_start_iter < S.max_iterations;
+        bk->worst = worst;
+        bk->rate = ce_rate(0);
+        if (worst < tol && iter > 0 && (!turbScheduled || (turbActive && bk->turbEst))) {
+            bk->done = 1;
+            bk->status = ST_CONVERGED;
+            bk->finalIter = iter;
+            return;
+        }
+        if (isnan(maxRes) || isinf(maxRes) || maxRes > S.divergence_limit) {
+            bk->done = 1;
+            bk->status = ST_DIVERGED;
+            bk->finalIter = iter;
+        }
+    }
+};
+
+// ----------------------------------------------------------------------------
+// The persistent solve kernel: blockIdx.x -> one slot (draw). The draw's
+// arena starts as a copy of the mesh's pre-filled image (geometry and int
+// tables); in LDS mode it lives in dynamic LDS.
+// ----------------------------------------------------------------------------
+template <int BS, bool kLDS>
+__global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restrict__ slots,
+                                                          DevSettings settings,
+                                                          DevResult *__restrict__ results,
+                                                          double *__restrict__ predictions,
+                                                          int pred_stride, double *__restrict__ fields,
+                                                          int64_t field_stride) {
+    extern __shared__ double smem[];
+    __shared__ Book book;
+    __shared__ double redbuf[2 * (BS / 64) * 4];
+    __shared__ double coeffs[11];
+    __shared__ DevSettings sset;
+    const DevSlot &slot = slots[blockIdx.x];
+    const DevMesh &M = *slot.mesh;
+    if (threadIdx.x < 11) coeffs[threadIdx.x] = slot.coeffs[threadIdx.x];
+    if (threadIdx.x == 0) sset = settings;
+    double *arena = kLDS ? smem : slot.arena;
+    {
+        const int64_t n16 = M.lay.image_bytes / 16;
+        const uint4 *gsrc = reinterpret_cast<const uint4 *>(M.img);
+        uint4 *dst = reinterpret_cast<uint4 *>(arena);
+        for (int64_t i = threadIdx.x; i < n16; i += BS) dst[i] = gsrc[i];
+    }
+    __syncthreads();
+    Draw<BS, kLDS> d(M, sset, coeffs, arena, &book, redbuf);
+    d.run();
+    const int row = slot.row;
+    const bool observe = book.status == ST_CONVERGED || book.status == ST_UNCONVERGED;
+    for (int i = threadIdx.x; i < M.nobs; i += BS)
+        predictions[(int64_t)row * pred_stride + i] = observe ? d.observe(i) : NAN;
+    if (fields) {
+        double *out = fields + row * field_stride;
+        const int nc = M.nc;
+        const int vars[10] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
+        for (int a = 0; a < 10; ++a)
+            for (int ci = threadIdx.x; ci < nc; ci += BS)
+                out[a * nc + ci] = arena[ci * kNA + vars[a]];
+        for (int f = threadIdx.x; f < M.nf; f += BS)
+            out[10 * nc + f] = arena[M.lay.face + f * kNF + W_phi];
+    }
+    if (threadIdx.x == 0) {
+        DevResult &r = results[row];
+        r.status = book.status;
+        r.error = book.error;
+        r.iters = book.status == ST_UNKNOWN ? 0 : book.finalIter;
+        r.worst = book.status == ST_UNKNOWN ? 0.0 : book.worst;
+        r.rate = book.status == ST_UNKNOWN ? 0.0 : book.rate;
+        r.stalled = book.stalled;
+        r.converged = book.status == ST_CONVERGED;
+        r.diverged = book.status == ST_DIVERGED;
+        for (int e = 0; e < 5; ++e) r.krylov[e] = book.kry[e];
+    }
+}
+
+}  // namespace rq
+
+// ----------------------------------------------------------------------------
+// launch helpers (host side of this translation unit)
+// ----------------------------------------------------------------------------
+namespace rq {
+
+template <int BS, bool kLDS>
+static hipError_t launch_t(const DevSlot *slots, int nslots, const DevSettings &s,
+                           DevResult *results, double *pred, int pred_stride, double *fields,
+                           int64_t field_stride, size_t lds_bytes, hipStream_t stream) {
+    auto kern = simple_solve_kernel<BS, kLDS>;
+    if (kLDS) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nslots), dim3(BS), kLDS ? lds_bytes : 0, stream, slots, s,
+                       results, pred, pred_stride, fields, field_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int nslots,
+                               const DevSettings &s, DevResult *results, double *pred,
+                               int pred_stride, double *fields, int64_t field_stride,
+                               size_t lds_bytes, hipStream_t stream) {
+#define RQ_CASE(B)                                                                          \
+    case B:                                                                                 \
+        return lds ? launch_t<B, true>(slots, nslots, s, results, pred, pred_stride, fields, \
+                                       field_stride, lds_bytes, stream)                      \
+                   : launch_t<B, false>(slots, nslots, s, results, pred, pred_stride, fields,\
+                                        field_stride, lds_bytes, stream);
+    switch (block) {
+        RQ_CASE(64)
+        RQ_CASE(128)
+        RQ_CASE(256)
+        RQ_CASE(512)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef RQ_CASE
+}
+
+}  // namespace rq

@@ -1,0 +1,151 @@
+/* C-ABI of the MI355X forward-solve engine (SIMPLE + SST k-omega).
+ *
+ * This is the drop-in boundary for the reference's batch forward model:
+ *   ForwardBatch(mesh, params, observations, bcs, nu, settings, u, p, k, omega)
+ *     rans-uq/include/ForwardBatch.hpp:34-50  -> rq_engine_create
+ *   ForwardBatch::evaluate(theta, workers)
+ *     rans-uq/include/ForwardBatch.hpp:52-124 -> rq_engine_evaluate
+ *   (its Python binding: rans-uq/python/bindings.cpp:888-916)
+ * Plain pointers and sizes only; every error is a return code plus text in
+ * the caller's buffer (no exceptions cross this boundary).
+ *
+ * Status codes follow the reference's EvaluationStatus order
+ * (core/include/EvaluationTypes.hpp:6): 0 Converged, 1 Unconverged,
+ * 2 Diverged, 3 InvalidParameters, 4 Unknown.
+ */
+#ifndef RANS_AMD_H
+#define RANS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RQ_OK 0
+#define RQ_EINVAL 1   /* invalid argument (std::invalid_argument in the reference) */
+#define RQ_EDEVICE 2  /* HIP runtime failure or no device */
+#define RQ_ENOMEM 3   /* allocation failure (std::bad_alloc, fatal for a batch) */
+
+/* A 2D finite-volume mesh (all face normals in the x-y plane), internal faces
+ * first: the flattened Mesh of core/include/Mesh.hpp:52-268 with its wall
+ * distance (Mesh::computeWallDistance, core/src/Mesh.cpp:431-463). */
+typedef struct rq_mesh_desc {
+    int32_t n_cells, n_internal, n_faces, n_patches;
+    const int32_t *owner, *neighbor;         /* [n_faces], neighbor -1 on boundary */
+    const double *face_center, *face_normal; /* [3*n_faces] */
+    const double *face_area, *face_delta;    /* [n_faces] */
+    const double *face_d;                    /* [3*n_faces] owner -> neighbour centre */
+    const double *face_weight;               /* [n_faces] */
+    const double *face_shift;                /* [3*n_faces] periodic shift */
+    const double *cell_center;               /* [3*n_cells] */
+    const double *cell_volume, *wall_distance;
+    const int32_t *cell_face_ptr, *cell_face_idx; /* each cell's faces, ascending */
+    const int32_t *patch_type;               /* 0 wall 1 inlet 2 outlet 3 symmetry 4 other */
+    const int32_t *patch_face_ptr, *patch_face_idx;
+} rq_mesh_desc;
+
+/* FlowBoundaryConditions (core/include/BoundaryCondition.hpp:105-167):
+ * per-patch BCType codes (BoundaryCondition.hpp:13-23 order) and uniform
+ * values. */
+typedef struct rq_bc_desc {
+    const int32_t *u_type, *p_type, *k_type, *w_type; /* [n_patches] */
+    const double *u_val;                              /* [2*n_patches] */
+    const double *p_val, *k_val, *w_val;              /* [n_patches] */
+} rq_bc_desc;
+
+/* SolverSettings (core/include/FlowFields.hpp:70-241). Solvers: 0 PCG,
+ * 1 BiCGSTAB, 2 AMG. Schemes: 0 upwind, 1 linearUpwind, 2 limitedLinearUpwind. */
+typedef struct rq_settings {
+    int32_t max_iterations;
+    double convergence_tol, divergence_limit;
+    int32_t convergence_window, monitor_interval;
+    double alpha_u, alpha_p, alpha_k, alpha_omega;
+    int32_t inner_iterations;
+    double inner_tolerance;
+    int32_t turb_start_iter, turb_update_interval, nut_floor_iters;
+    int32_t momentum_convection, turbulence_convection;
+    double body_force_x, body_force_y;
+    double k_min, k_max, omega_min;
+    int32_t pressure_solver, momentum_solver, turbulence_solver;
+    int32_t sst_variant;
+    /* engine options with no reference counterpart */
+    int32_t mixed_precision; /* 1: FP32 Krylov vectors, FP64 defect correction */
+    int32_t block_size;      /* threads per draw; 0 = tuned default */
+} rq_settings;
+
+/* InferenceParameterSet (core/include/InferenceParameters.hpp:18-188). */
+typedef struct rq_param_desc {
+    int32_t n_active;
+    const int32_t *active_indices; /* into the 11 SST coefficients */
+    const double *base;            /* [11] values of the inactive coefficients */
+    const double *lower, *upper;   /* [n_active] bounds */
+} rq_param_desc;
+
+/* PreparedObservation stencils (core/include/ObservationOperator.hpp:611-669):
+ * type 0 velocity probe, 1 pressure tap, 2 skin friction. */
+typedef struct rq_obs_desc {
+    int32_t n_obs;
+    const int32_t *type, *component;
+    const double *observed, *variance;
+    const double *ref_velocity, *ref_pressure, *ref_density;
+    const int32_t *cell;
+    const double *offx, *offy, *axx, *axy, *ayy, *det;
+    const int32_t *nb_ptr, *nb_cell, *nb_face;
+    const double *nb_dx, *nb_dy, *nb_w;
+    const int32_t *face_a, *face_b;
+    const double *t;
+} rq_obs_desc;
+
+typedef struct rq_engine rq_engine;
+
+/* Per-row outputs of one batch (ForwardBatchRow, ForwardBatch.hpp:8-19).
+ * Any pointer except status may be NULL. predictions is n_rows x n_obs, NaN
+ * for rows without predictions. row_errors is n_rows x row_error_stride
+ * bytes of NUL-terminated text. fields, if given, receives per row 10
+ * n_cells blocks (Ux Uy p k omega nuT F1 F2 Pk CDkw) then n_faces face fluxes. */
+typedef struct rq_batch_out {
+    int32_t *status;
+    double *loglik, *predictions;
+    int32_t *iterations;
+    double *final_worst_measure, *contraction_rate_u;
+    uint8_t *stalled, *converged;
+    char *row_errors;
+    size_t row_error_stride;
+    double *fields;
+    int64_t *krylov_iterations; /* n_rows x 5: Ux, Uy, p', k, omega */
+} rq_batch_out;
+
+/* ForwardBatch constructor: copies every input (the caller may free or
+ * change its arrays afterwards). init = {Ux, Uy, p, k, omega}. */
+rq_engine *rq_engine_create(const rq_mesh_desc *mesh, const rq_param_desc *params,
+                            const rq_obs_desc *obs, const rq_bc_desc *bcs, double nu,
+                            const rq_settings *settings, const double init[5], char *err,
+                            size_t errcap);
+
+/* ForwardBatch::evaluate: theta is n_rows x n_params (row-major). All rows
+ * are solved on the GPU together; workers must be >= 1 and only keeps the
+ * reference's argument contract. */
+int rq_engine_evaluate(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
+                       int32_t workers, rq_batch_out *out, char *err, size_t errcap);
+
+int32_t rq_engine_n_observations(const rq_engine *engine);
+int32_t rq_engine_n_parameters(const rq_engine *engine);
+int32_t rq_engine_n_cells(const rq_engine *engine);
+int32_t rq_engine_n_faces(const rq_engine *engine);
+
+/* Device time of the last evaluate's solve kernel in milliseconds (HIP
+ * events on the engine's stream) and its launch shape. */
+int rq_engine_last_timing(const rq_engine *engine, double *kernel_ms, int32_t *block_size,
+                          int32_t *arena_in_lds, int64_t *arena_bytes);
+
+void rq_engine_destroy(rq_engine *engine);
+
+/* Library identity: compiler, target and build flags. */
+const char *rq_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
