@@ -59,7 +59,7 @@ class RqSettings(C.Structure):
                 ("omega_min", C.c_double), ("pressure_solver", C.c_int32),
                 ("momentum_solver", C.c_int32), ("turbulence_solver", C.c_int32),
                 ("sst_variant", C.c_int32), ("mixed_precision", C.c_int32),
-                ("block_size", C.c_int32)]
+                ("block_size", C.c_int32), ("profile", C.c_int32)]
 
 
 class RqParams(C.Structure):
@@ -80,7 +80,7 @@ class RqBatchOut(C.Structure):
     _fields_ = [("status", _ip), ("loglik", _dp), ("predictions", _dp), ("iterations", _ip),
                 ("final_worst_measure", _dp), ("contraction_rate_u", _dp), ("stalled", _u8p),
                 ("converged", _u8p), ("row_errors", C.c_char_p), ("row_error_stride", C.c_size_t),
-                ("fields", _dp), ("krylov_iterations", _i64p)]
+                ("fields", _dp), ("krylov_iterations", _i64p), ("phase_cycles", _i64p)]
 
 
 EXPORTS = ["rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
@@ -138,7 +138,8 @@ class _Keep:
         return a.ctypes.data_as(_ip)
 
 
-def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size: int = 0):
+def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size: int = 0,
+                    profile: bool = False):
     r = RqSettings()
     r.max_iterations, r.convergence_tol, r.divergence_limit = s.max_iterations, \
         s.convergence_tol, s.divergence_limit
@@ -157,6 +158,7 @@ def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size
     r.sst_variant = s.sst_variant
     r.mixed_precision = 1 if mixed_precision else 0
     r.block_size = block_size
+    r.profile = 1 if profile else 0
     return r
 
 
@@ -238,6 +240,7 @@ class ForwardBatchResult:
     krylov_iterations: np.ndarray
     fields: np.ndarray = None
     kernel_ms: float = 0.0
+    phase_cycles: np.ndarray = None
 
     @property
     def converged(self) -> np.ndarray:
@@ -251,7 +254,7 @@ class ForwardBatch:
                  obs_op: ObservationOperator, bcs: FlowBoundaryConditions, nu: float,
                  settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
-                 mixed_precision: bool = False, block_size: int = 0):
+                 mixed_precision: bool = False, block_size: int = 0, profile: bool = False):
         settings = settings if settings is not None else SolverSettings()
         settings.validate()
         param_set.validate()
@@ -283,7 +286,7 @@ class ForwardBatch:
         p.base = keep.d(param_set.base)
         p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
         o = obs_struct(prepared, keep)
-        s = settings_struct(settings, mixed_precision, block_size)
+        s = settings_struct(settings, mixed_precision, block_size, profile)
         init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
         err = C.create_string_buffer(512)
         h = self.lib.rq_engine_create(C.byref(m), C.byref(p), C.byref(o), C.byref(b), float(nu),
@@ -332,6 +335,7 @@ class ForwardBatch:
         stalled = np.zeros(n, np.uint8)
         conv = np.zeros(n, np.uint8)
         kry = np.zeros((n, 5), np.int64)
+        cyc = np.zeros((n, 16), np.int64)
         stride = 256
         errs = C.create_string_buffer(max(1, n * stride))
         fld = np.zeros((n, 10 * self.n_cells + self.n_faces)) if fields else None
@@ -348,6 +352,7 @@ class ForwardBatch:
         out.row_error_stride = stride
         out.fields = fld.ctypes.data_as(_dp) if fields else None
         out.krylov_iterations = kry.ctypes.data_as(_i64p)
+        out.phase_cycles = cyc.ctypes.data_as(_i64p)
         err = C.create_string_buffer(512)
         rc = self.lib.rq_engine_evaluate(self._h, theta.ctypes.data_as(_dp), n,
                                          self.n_parameters(), int(workers), C.byref(out), err, 512)
@@ -367,7 +372,7 @@ class ForwardBatch:
         timing = self.last_timing()
         return ForwardBatchResult(nobs, rows, pred, loglik, status, iters, errors,
                                   conv.astype(bool), worst, rate, stalled.astype(bool), kry,
-                                  fld, timing["kernel_ms"])
+                                  fld, timing["kernel_ms"], cyc)
 
 
 def build_info() -> str:

@@ -8,6 +8,7 @@ namespace rq {
 
 constexpr int kSlots = 4;          // faces per cell (ELL width): quadrilaterals and triangles
 constexpr int kMaxWindow = 256;    // largest contraction window supported
+constexpr int kPhases = 16;        // profiled phases of one outer iteration
 
 enum BcType { BC_DIRICHLET = 0, BC_NEUMANN, BC_INLET, BC_OUTLET, BC_WALL_NOSLIP,
               BC_WALL_KOMEGA, BC_WALL_MOVING, BC_SYMMETRY, BC_CYCLIC };
@@ -89,6 +90,7 @@ struct DevSettings {
     double nu;
     double init_ux, init_uy, init_p, init_k, init_w;
     int mixed_precision;   // 1: FP32 Krylov with FP64 defect correction
+    int profile;           // 1: accumulate per-phase clock counts in DevResult::cycles
 };
 
 // One draw to solve: its mesh, its arena and its coefficients.
@@ -104,6 +106,7 @@ struct DevResult {
     int status, iters, stalled, converged, diverged, error;
     double worst, rate;
     long long krylov[5];
+    long long cycles[kPhases];
 };
 
 }  // namespace rq

@@ -142,6 +142,7 @@ void validate_settings(const rq_settings &s) {
     if (s.mixed_precision != 0 && s.mixed_precision != 1)
         throw Fail{RQ_EINVAL, "mixed_precision must be 0 or 1"};
     if (s.mixed_precision) throw Fail{RQ_EINVAL, "mixed precision is not available in this engine build"};
+    if (s.profile != 0 && s.profile != 1) throw Fail{RQ_EINVAL, "profile must be 0 or 1"};
     if (s.block_size != 0 && s.block_size != 64 && s.block_size != 128 && s.block_size != 256 &&
         s.block_size != 512)
         throw Fail{RQ_EINVAL, "block_size must be 0, 64, 128, 256 or 512"};
@@ -558,6 +559,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.init_k = init[3];
         S.init_w = init[4];
         S.mixed_precision = s.mixed_precision;
+        S.profile = s.profile;
         if (s.block_size) E->block = s.block_size;
         else E->block = nc <= 128 ? 64 : nc <= 4096 ? 256 : 512;
         int maxLds = 0;
@@ -617,6 +619,8 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         if (out->converged) out->converged[r] = 0;
         if (out->krylov_iterations)
             for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = 0;
+        if (out->phase_cycles)
+            for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = 0;
         // InferenceParameterSet::inBounds (core/include/InferenceParameters.hpp:127-140)
         bool ok = true;
         for (int i = 0; i < n_params; ++i) {
@@ -727,6 +731,8 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         if (out->converged) out->converged[r] = status == ST_CONVERGED;
         if (out->krylov_iterations)
             for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = d.krylov[e];
+        if (out->phase_cycles)
+            for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = d.cycles[e];
         if (!msg.empty()) write_row_error(out, r, msg);
         if (out->fields)
             std::memcpy(out->fields + r * fstride, fields.data() + (size_t)i * fstride,

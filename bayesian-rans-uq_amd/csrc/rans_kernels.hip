@@ -129,6 +129,8 @@ struct Book {
     double lastK, lastOm, normUx0, normUy0, normP0, pfRef, worst, rate;
     long long nEst[4];
     long long kry[5];
+    long long tlast;
+    long long cyc[kPhases];
 };
 
 struct Krylov {
@@ -422,7 +424,7 @@ struct Draw {
         const int Uc = comp == 0 ? V_Ux : V_Uy;
         int code[kSlots], nbv[kSlots];
         slots(ci, code, nbv);
-        double dg = 0.0, sr = 0.0;
+        double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
         const double nuTc = C(ci, V_nuT);
 #pragma unroll
         for (int s = 0; s < kSlots; ++s) {
@@ -439,10 +441,10 @@ struct Draw {
                 const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
                 if (slot_own(code[s])) {
                     dg += Df + cPos;
-                    off(ci, s) = -(Df + cNeg);
+                    offv[s] = -(Df + cNeg);
                 } else {
                     dg += Df + cNeg;
-                    off(ci, s) = -(Df + cPos);
+                    offv[s] = -(Df + cPos);
                 }
             } else {
                 const int b = f - nif;
@@ -518,11 +520,15 @@ struct Draw {
         C(ci, V_diag) = dg;
         C(ci, V_src) = sr;
         if (comp == 0) C(ci, V_aPs) = dg;
-        ok = row_finite(ci, dg, sr) && isfinite(uc);
+        ok = isfinite(dg) && isfinite(sr) && isfinite(uc);
         double y = dg * uc;
 #pragma unroll
         for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) y += off(ci, s) * C(nbv[s], Uc);
+            if (nbv[s] >= 0) {
+                off(ci, s) = offv[s];
+                ok = ok && isfinite(offv[s]);
+                y += offv[s] * C(nbv[s], Uc);
+            }
         return sr - y;
     }
 
@@ -532,7 +538,7 @@ struct Draw {
         const double nu = S.nu;
         int code[kSlots], nbv[kSlots];
         slots(ci, code, nbv);
-        double dg = 0.0, sr = 0.0;
+        double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
         const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
 #pragma unroll
         for (int s = 0; s < kSlots; ++s) {
@@ -556,10 +562,10 @@ struct Draw {
                 const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
                 if (own) {
                     dg += Df + cPos;
-                    off(ci, s) = -(Df + cNeg);
+                    offv[s] = -(Df + cNeg);
                 } else {
                     dg += Df + cNeg;
-                    off(ci, s) = -(Df + cPos);
+                    offv[s] = -(Df + cPos);
                 }
             } else {
                 const int b = f - nif;
@@ -596,7 +602,13 @@ struct Draw {
         dg /= aK;
         C(ci, V_diag) = dg;
         C(ci, V_src) = sr;
-        ok = row_finite(ci, dg, sr) && isfinite(kc);
+        ok = isfinite(dg) && isfinite(sr) && isfinite(kc);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) {
+                off(ci, s) = offv[s];
+                ok = ok && isfinite(offv[s]);
+            }
     }
 
     // SIMPLESolver::assembleOmegaEquation (SIMPLESolver.cpp:1030-1169) and
@@ -606,7 +618,7 @@ struct Draw {
         const double nu = S.nu;
         int code[kSlots], nbv[kSlots];
         slots(ci, code, nbv);
-        double dg = 0.0, sr = 0.0, balance = 0.0;
+        double dg = 0.0, sr = 0.0, balance = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
         const double amplitude = 6 * nu / c[2];
         const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
         const double omsc = planar ? C(ci, V_oms) : 0.0;
@@ -632,10 +644,10 @@ struct Draw {
                 const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
                 if (own) {
                     dg += Df + cPos;
-                    off(ci, s) = -(Df + cNeg);
+                    offv[s] = -(Df + cNeg);
                 } else {
                     dg += Df + cNeg;
-                    off(ci, s) = -(Df + cPos);
+                    offv[s] = -(Df + cPos);
                 }
                 if (planar) {
                     const double omsn = C(n, V_oms);
@@ -696,18 +708,24 @@ struct Draw {
             const double vn_ = C(nbv[s], V_wallv);
             const double vo = own ? wvc : vn_, vn = own ? vn_ : wvc;
             if (isfinite(vo)) {
-                if (!own) sr -= off(ci, s) * vo;
-                off(ci, s) = 0.0;
+                if (!own) sr -= offv[s] * vo;
+                offv[s] = 0.0;
             } else if (isfinite(vn)) {
-                if (own) sr -= off(ci, s) * vn;
-                off(ci, s) = 0.0;
+                if (own) sr -= offv[s] * vn;
+                offv[s] = 0.0;
             }
         }
         if (isfinite(wvc)) sr = dg * wvc;
         C(ci, V_diag) = dg;
         C(ci, V_src) = sr;
         const double x0 = isfinite(wvc) ? wvc : wc;
-        ok = row_finite(ci, dg, sr) && isfinite(x0);
+        ok = isfinite(dg) && isfinite(sr) && isfinite(x0);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) {
+                off(ci, s) = offv[s];
+                ok = ok && isfinite(offv[s]);
+            }
         return x0;
     }
 
@@ -861,35 +879,43 @@ struct Draw {
     }
 
     // ------------------------------------------------------ contraction
-    // ContractionEstimate (SIMPLESolver.cpp:32-74) on a ring of 2m samples
+    // ContractionEstimate (SIMPLESolver.cpp:32-74): each estimator keeps its
+    // last 2m samples in an LDS ring (older samples never enter its rate or
+    // remainder). Window maxima are taken across the lanes of wave 0; a NaN
+    // sample counts as 0, as in the reference's max fold starting from 0.
     DEV D *ring(int e) const { return ringb + e * 2 * S.convergence_window; }
     DEV void ce_push(int e, double d) {
         const int m2 = 2 * S.convergence_window;
         ring(e)[bk->nEst[e] % m2] = d;
         bk->nEst[e]++;
     }
-    DEV double ce_at(int e, long long i) const { return ring(e)[i % (2 * S.convergence_window)]; }
-    DEV double ce_last(int e) const {
-        return bk->nEst[e] == 0 ? INFINITY : ce_at(e, bk->nEst[e] - 1);
+    // wave 0 only: maxima of the newest and the previous window
+    DEV void ce_windows(int e, double &dNew, double &dOld) const {
+        const long long n = bk->nEst[e];
+        const int m = S.convergence_window, m2 = 2 * m;
+        double nw = 0.0, ol = 0.0;
+        for (int j = tid; j < m2; j += 64) {
+            const long long age = ((n - 1 - j) % m2 + m2) % m2;
+            double v = ring(e)[j];
+            v = (v != v) ? 0.0 : v;
+            if (age < m) nw = smax(nw, v);
+            else ol = smax(ol, v);
+        }
+        dNew = wave_max(nw);
+        dOld = wave_max(ol);
     }
-    DEV double ce_rate(int e) const {
-        const long long n = bk->nEst[e], m = S.convergence_window;
+    DEV double ce_rate(long long n, double dNew, double dOld) const {
+        const long long m = S.convergence_window;
         if (n < 2 * m) return NAN;
-        double dNew = 0.0, dOld = 0.0;
-        for (long long i = n - m; i < n; ++i) dNew = smax(dNew, ce_at(e, i));
-        for (long long i = n - 2 * m; i < n - m; ++i) dOld = smax(dOld, ce_at(e, i));
         if (dOld <= 0.0) return dNew > 0.0 ? INFINITY : 0.0;
         return pow(dNew / dOld, 1.0 / (double)m);
     }
-    DEV double ce_remaining(int e, double negligible) const {
-        const long long n = bk->nEst[e], m = S.convergence_window;
+    DEV double ce_remaining(long long n, double dNew, double rate, double negligible) const {
+        const long long m = S.convergence_window;
         if (n < 2 * m) return INFINITY;
-        double dNew = 0.0;
-        for (long long i = n - m; i < n; ++i) dNew = smax(dNew, ce_at(e, i));
         if (dNew <= negligible) return dNew;
-        const double r = ce_rate(e);
-        if (!(r < 1.0)) return INFINITY;
-        return dNew * r / (1.0 - r);
+        if (!(rate < 1.0)) return INFINITY;
+        return dNew * rate / (1.0 - rate);
     }
 
     // ------------------------------------------------------- observation
@@ -938,6 +964,15 @@ struct Draw {
         return value;
     }
 
+    // per-phase clock accounting (thread 0, between block-wide barriers)
+    DEV void mark(int phase) {
+        if (S.profile && tid == 0) {
+            const long long now = clock64();
+            bk->cyc[phase] += now - bk->tlast;
+            bk->tlast = now;
+        }
+    }
+
     DEV void fail(int code) {
         if (tid == 0) {
             bk->done = 1;
@@ -983,6 +1018,8 @@ struct Draw {
             bk->rate = 0.0;
             for (int e = 0; e < 4; ++e) bk->nEst[e] = 0;
             for (int e = 0; e < 5; ++e) bk->kry[e] = 0;
+            for (int e = 0; e < kPhases; ++e) bk->cyc[e] = 0;
+            bk->tlast = clock64();
         }
         __syncthreads();
         bc_velocity();
@@ -1057,6 +1094,7 @@ struct Draw {
                 }
             }
             __syncthreads();
+            mark(0);
 
             // ---- x momentum (SIMPLESolver.cpp:1342-1379)
             double momResid, momScale;
@@ -1079,6 +1117,7 @@ struct Draw {
                 red.sum(a);
                 momResid = a[0];
                 momScale = a[1];
+                mark(1);
                 if (a[2] != 0.0) {
                     fail(ERR_NONFINITE_SYSTEM);
                     break;
@@ -1086,6 +1125,7 @@ struct Draw {
                 rUx = solve(S.momentum_solver, a[3], false);
                 for (int ci = tid; ci < nc; ci += BS) C(ci, V_Ux) = C(ci, V_kx);
                 __syncthreads();
+                mark(2);
             }
             // ---- y momentum: its transpose stress sees the new Ux
             {
@@ -1111,6 +1151,7 @@ struct Draw {
                 }
                 red.sum(a);
                 momResid = smax(momResid, a[0]);
+                mark(3);
                 if (a[1] != 0.0) {
                     fail(ERR_NONFINITE_SYSTEM);
                     break;
@@ -1118,6 +1159,7 @@ struct Draw {
                 rUy = solve(S.momentum_solver, a[2], false);
                 for (int ci = tid; ci < nc; ci += BS) C(ci, V_Uy) = C(ci, V_kx);
                 __syncthreads();
+                mark(4);
             }
             bc_velocity();
             __syncthreads();
@@ -1159,7 +1201,7 @@ struct Draw {
             for (int ci = tid; ci < nc; ci += BS) {
                 int code[kSlots], nbv[kSlots];
                 slots(ci, code, nbv);
-                double dg = 0.0, sr = 0.0;
+                double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int s = 0; s < kSlots; ++s) {
                     if (code[s] < 0) continue;
@@ -1167,7 +1209,7 @@ struct Draw {
                     if (f < nif) {
                         const double coeff = Fc(f, W_pCoeff);
                         dg += coeff;
-                        off(ci, s) = -coeff;
+                        offv[s] = -coeff;
                         if (slot_own(code[s])) sr -= Fc(f, W_phiPred);
                         else sr += Fc(f, W_phiPred);
                     } else {
@@ -1180,19 +1222,28 @@ struct Draw {
                 C(ci, V_src) = sr;
                 C(ci, V_kr) = sr;  // b - A x at x = 0
                 C(ci, V_kx) = 0.0;
+                bool okp = isfinite(dg) && isfinite(sr);
+#pragma unroll
+                for (int s = 0; s < kSlots; ++s)
+                    if (nbv[s] >= 0) {
+                        off(ci, s) = offv[s];
+                        okp = okp && isfinite(offv[s]);
+                    }
                 a8[0] += fabs(sr);
                 a8[1] += sr * sr;
-                a8[2] += row_finite(ci, dg, sr) ? 0.0 : 1.0;
+                a8[2] += okp ? 0.0 : 1.0;
             }
             red.sum(ms);
             red.sum(a8);
             const double massFluxScale = ms[0];
+            mark(5);
             const double contScaled = a8[0] / smax(massFluxScale, 1e-300);
             if (a8[2] != 0.0) {
                 fail(ERR_NONFINITE_SYSTEM);
                 break;
             }
             const Krylov rP = solve(S.pressure_solver, a8[1], true);
+            mark(6);
             for (int ci = tid; ci < nc; ci += BS) C(ci, V_pP) = C(ci, V_kx);
             __syncthreads();
             for (int b = tid; b < nb; b += BS) Bd(b, B_pP) = (bflag[b] & 2) ? 0.0 : C(fon[nif + b].x, V_pP);
@@ -1219,6 +1270,7 @@ struct Draw {
                 else Fc(f, W_phi) = Fc(f, W_phiPred) - Fc(f, W_pCoeff) * (C(on.y, V_pP) - C(on.x, V_pP));
             }
             __syncthreads();
+            mark(7);
 
             // ---- turbulence transport (SIMPLESolver.cpp:1476-1538)
             if (turbUpdate) {
@@ -1236,11 +1288,13 @@ struct Draw {
                     a9[0] += r * r;
                 }
                 red.sum(a9);
+                mark(8);
                 if (a9[1] != 0.0) {
                     fail(ERR_NONFINITE_SYSTEM);
                     break;
                 }
                 const Krylov rK = solve(S.turbulence_solver, a9[0], false);
+                mark(9);
                 double mk[2] = {1e-30, 0.0};
                 for (int ci = tid; ci < nc; ci += BS) {
                     const double kn = smax(S.k_min, smin(S.k_max, C(ci, V_kx)));
@@ -1265,11 +1319,13 @@ struct Draw {
                     a10[0] += r * r;
                 }
                 red.sum(a10);
+                mark(10);
                 if (a10[1] != 0.0) {
                     fail(ERR_NONFINITE_SYSTEM);
                     break;
                 }
                 const Krylov rW = solve(S.turbulence_solver, a10[0], false);
+                mark(11);
                 double mw[2] = {1e-30, 0.0};
                 for (int ci = tid; ci < nc; ci += BS) {
                     double wn = smax(S.omega_min, smin(1e15, C(ci, V_kx)));
@@ -1303,6 +1359,7 @@ struct Draw {
                 mu[1] = smax(mu[1], sqrt(dx * dx + dy * dy));
             }
             red.max(mu);
+            mark(12);
             const bool monitor =
                 M.nobs > 0 && iter % (S.monitor_interval > 1 ? S.monitor_interval : 1) == 0;
             if (monitor) {
@@ -1314,11 +1371,11 @@ struct Draw {
                 bk->kry[0] += rUx.iterations;
                 bk->kry[1] += rUy.iterations;
                 bk->kry[2] += rP.iterations;
-                bookkeeping(iter, turbActive, momResid, momScale, contScaled, massFluxScale,
-                            rUx.initial, rUy.initial, rP.initial, mu[0], mu[1], monitor,
-                            negligible, tol);
             }
-            __syncthreads();
+            bookkeeping(iter, turbActive, momResid, momScale, contScaled, massFluxScale,
+                        rUx.initial, rUy.initial, rP.initial, mu[0], mu[1], monitor, negligible,
+                        tol);
+            mark(13);
             if (bk->done) break;
         }
         if (tid == 0 && !bk->done) {
@@ -1329,84 +1386,122 @@ struct Draw {
         __syncthreads();
     }
 
+    // The convergence decision of SIMPLESolver::solve (SIMPLESolver.cpp:
+    // 1545-1705) for this iteration. Called by every thread; wave 0 does the
+    // work (lane-parallel window maxima and monitor norms, scalar logic in
+    // every lane, thread 0 writes the state).
     DEV void bookkeeping(int iter, bool turbActive, double momResid, double momScale,
                          double contScaled, double massFluxScale, double eUx, double eUy,
                          double eP, double uMax, double duMax, bool monitor, double negligible,
                          double tol) {
-        const double momentumScaled = momResid / smax(momScale, 1e-300);
-        const double changeU = duMax / uMax;
-        ce_push(0, changeU);
-        const double remainingU = ce_remaining(0, negligible);
-        if (monitor) {
-            bool finite = true;
-            for (int i = 0; i < M.nobs; ++i) finite = finite && isfinite(q[i]);
-            if (!finite) {
-                bk->done = 1;
-                bk->status = ST_UNKNOWN;
-                bk->error = ERR_NONFINITE_MONITOR;
-                return;
-            }
-            if (bk->haveQPrev) {
-                double qMax = 0.0, dq = 0.0;
-                for (int i = 0; i < M.nobs; ++i) qMax = smax(qMax, fabs(q[i]));
-                for (int i = 0; i < M.nobs; ++i) {
-                    const double den = smax(smax(fabs(q[i]), M.ob_floor[i]), 1e-300);
-                    dq = smax(dq, fabs(q[i] - qprev[i]) / den);
+        if (tid < 64) {
+            const bool lane0 = tid == 0;
+            const double momentumScaled = momResid / smax(momScale, 1e-300);
+            const double changeU = duMax / uMax;
+            if (lane0) ce_push(0, changeU);
+            bool qbad = false;
+            if (monitor) {
+                const int nobs = M.nobs;
+                const bool have = bk->haveQPrev;
+                double bad = 0.0, dq = 0.0;
+                for (int i = tid; i < nobs; i += 64) {
+                    const double qi = q[i];
+                    if (!isfinite(qi)) bad = 1.0;
+                    const double den = smax(smax(fabs(qi), M.ob_floor[i]), 1e-300);
+                    const double x = fabs(qi - qprev[i]) / den;
+                    if (have) dq = smax(dq, (x != x) ? 0.0 : x);
                 }
-                ce_push(3, dq);
+                bad = wave_max(bad);
+                dq = wave_max(dq);
+                if (bad != 0.0) {
+                    qbad = true;
+                    if (lane0) {
+                        bk->done = 1;
+                        bk->status = ST_UNKNOWN;
+                        bk->error = ERR_NONFINITE_MONITOR;
+                    }
+                } else {
+                    if (have && lane0) ce_push(3, dq);
+                    for (int i = tid; i < nobs; i += 64) qprev[i] = q[i];
+                    if (lane0) bk->haveQPrev = 1;
+                }
             }
-            for (int i = 0; i < M.nobs; ++i) qprev[i] = q[i];
-            bk->haveQPrev = 1;
+            // lane 0's ring pushes must be visible to the other lanes of the wave
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (!qbad) {
+                double dNew[4], dOld[4];
+                long long n[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    ce_windows(e, dNew[e], dOld[e]);
+                    n[e] = bk->nEst[e];
+                }
+                double rate[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) rate[e] = ce_rate(n[e], dNew[e], dOld[e]);
+                const double remainingU = ce_remaining(n[0], dNew[0], rate[0], negligible);
+                const double lastK = bk->lastK, lastOm = bk->lastOm;
+                double worst = smax(smax(smax(momentumScaled, contScaled), changeU), remainingU);
+                if (turbActive) {
+                    worst = smax(worst, lastK);
+                    worst = smax(worst, ce_remaining(n[1], dNew[1], rate[1], negligible));
+                    worst = smax(worst, lastOm);
+                    worst = smax(worst, ce_remaining(n[2], dNew[2], rate[2], negligible));
+                }
+                if (M.nobs > 0) {
+                    const int m2 = 2 * S.convergence_window;
+                    const double lastQ = n[3] == 0 ? INFINITY : ring(3)[(n[3] - 1) % m2];
+                    worst = smax(worst, lastQ);
+                    worst = smax(worst, ce_remaining(n[3], dNew[3], rate[3], negligible));
+                }
+                if (!isfinite(worst)) worst = INFINITY;
+                double normUx0 = bk->normUx0, normUy0 = bk->normUy0, normP0 = bk->normP0;
+                double pfRef = bk->pfRef;
+                if (iter == 0) {
+                    normUx0 = smax(eUx, 1e-30);
+                    normUy0 = smax(eUy, 1e-30);
+                    normP0 = smax(eP, 1e-30);
+                    pfRef = massFluxScale;
+                } else if (iter < 50) {
+                    normP0 = smax(normP0, eP);
+                    pfRef = smax(pfRef, massFluxScale);
+                }
+                const double normMom0 = smax(normUx0, normUy0);
+                const double nUx = normMom0 < 1e-20 ? 0.0 : eUx / normMom0;
+                const double nUy = normMom0 < 1e-20 ? 0.0 : eUy / normMom0;
+                const double pref = smax(normP0, pfRef);
+                const double nP = pref < 1e-20 ? 0.0 : eP / pref;
+                double maxRes = smax(smax(nUx, nUy), nP);
+                const double eK = turbActive ? lastK : 0.0, eOm = turbActive ? lastOm : 0.0;
+                if (turbActive) maxRes = smax(smax(maxRes, lastK), lastOm);
+                if (!isfinite(eUx) || !isfinite(eUy) || !isfinite(eP) || !isfinite(eK) ||
+                    !isfinite(eOm) || !isfinite(lastK) || !isfinite(lastOm) ||
+                    !isfinite(momentumScaled) || !isfinite(contScaled) || !isfinite(changeU)) {
+                    maxRes = INFINITY;
+                    worst = INFINITY;
+                }
+                const bool turbScheduled = S.turb_start_iter < S.max_iterations;
+                if (lane0) {
+                    bk->normUx0 = normUx0;
+                    bk->normUy0 = normUy0;
+                    bk->normP0 = normP0;
+                    bk->pfRef = pfRef;
+                    bk->worst = worst;
+                    bk->rate = rate[0];
+                    if (worst < tol && iter > 0 && (!turbScheduled || (turbActive && bk->turbEst))) {
+                        bk->done = 1;
+                        bk->status = ST_CONVERGED;
+                        bk->finalIter = iter;
+                    } else if (isnan(maxRes) || isinf(maxRes) || maxRes > S.divergence_limit) {
+                        bk->done = 1;
+                        bk->status = ST_DIVERGED;
+                        bk->finalIter = iter;
+                    }
+                }
+            }
         }
-        double worst = smax(smax(smax(momentumScaled, contScaled), changeU), remainingU);
-        if (turbActive) {
-            worst = smax(worst, bk->lastK);
-            worst = smax(worst, ce_remaining(1, negligible));
-            worst = smax(worst, bk->lastOm);
-            worst = smax(worst, ce_remaining(2, negligible));
-        }
-        if (M.nobs > 0) {
-            worst = smax(worst, ce_last(3));
-            worst = smax(worst, ce_remaining(3, negligible));
-        }
-        if (!isfinite(worst)) worst = INFINITY;
-        if (iter == 0) {
-            bk->normUx0 = smax(eUx, 1e-30);
-            bk->normUy0 = smax(eUy, 1e-30);
-            bk->normP0 = smax(eP, 1e-30);
-            bk->pfRef = massFluxScale;
-        } else if (iter < 50) {
-            bk->normP0 = smax(bk->normP0, eP);
-            bk->pfRef = smax(bk->pfRef, massFluxScale);
-        }
-        const double normMom0 = smax(bk->normUx0, bk->normUy0);
-        const double nUx = normMom0 < 1e-20 ? 0.0 : eUx / normMom0;
-        const double nUy = normMom0 < 1e-20 ? 0.0 : eUy / normMom0;
-        const double pref = smax(bk->normP0, bk->pfRef);
-        const double nP = pref < 1e-20 ? 0.0 : eP / pref;
-        double maxRes = smax(smax(nUx, nUy), nP);
-        const double eK = turbActive ? bk->lastK : 0.0, eOm = turbActive ? bk->lastOm : 0.0;
-        if (turbActive) maxRes = smax(smax(maxRes, bk->lastK), bk->lastOm);
-        if (!isfinite(eUx) || !isfinite(eUy) || !isfinite(eP) || !isfinite(eK) ||
-            !isfinite(eOm) || !isfinite(bk->lastK) || !isfinite(bk->lastOm) ||
-            !isfinite(momentumScaled) || !isfinite(contScaled) || !isfinite(changeU)) {
-            maxRes = INFINITY;
-            worst = INFINITY;
-        }
-        const bool turbScheduled = S.turb_start_iter < S.max_iterations;
-        bk->worst = worst;
-        bk->rate = ce_rate(0);
-        if (worst < tol && iter > 0 && (!turbScheduled || (turbActive && bk->turbEst))) {
-            bk->done = 1;
-            bk->status = ST_CONVERGED;
-            bk->finalIter = iter;
-            return;
-        }
-        if (isnan(maxRes) || isinf(maxRes) || maxRes > S.divergence_limit) {
-            bk->done = 1;
-            bk->status = ST_DIVERGED;
-            bk->finalIter = iter;
-        }
+        __syncthreads();
     }
 };
 
@@ -1466,6 +1561,7 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
         r.converged = book.status == ST_CONVERGED;
         r.diverged = book.status == ST_DIVERGED;
         for (int e = 0; e < 5; ++e) r.krylov[e] = book.kry[e];
+        for (int e = 0; e < kPhases; ++e) r.cycles[e] = book.cyc[e];
     }
 }
 

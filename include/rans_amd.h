@@ -73,6 +73,7 @@ typedef struct rq_settings {
     /* engine options with no reference counterpart */
     int32_t mixed_precision; /* 1: FP32 Krylov vectors, FP64 defect correction */
     int32_t block_size;      /* threads per draw; 0 = tuned default */
+    int32_t profile;         /* 1: per-phase clock counts in rq_batch_out.phase_cycles */
 } rq_settings;
 
 /* InferenceParameterSet (core/include/InferenceParameters.hpp:18-188). */
@@ -115,6 +116,7 @@ typedef struct rq_batch_out {
     size_t row_error_stride;
     double *fields;
     int64_t *krylov_iterations; /* n_rows x 5: Ux, Uy, p', k, omega */
+    int64_t *phase_cycles;      /* n_rows x 16 shader clocks per solve phase (profile = 1) */
 } rq_batch_out;
 
 /* ForwardBatch constructor: copies every input (the caller may free or

@@ -1,0 +1,193 @@
+"""Writes the golden fixtures under tests/golden/ from the reference solver.
+
+The reference's forward model is compiled from its own sources by
+``make -C oracle ref`` into oracle/_ref/ref_harness (a driver of this project
+around the reference's ForwardModel). This script feeds it meshes written in
+the reference's MSH2 format or built by the reference's own generators, and
+stores inputs and outputs as small JSON files. It needs the reference sources
+(only present where fixtures are made); tests read the JSON files only.
+
+    python tests/golden/make_fixtures.py
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "bayesian-rans-uq_amd"))
+from cfdr_amd import cases  # noqa: E402
+from cfdr_amd.mesh import Mesh, hill_bottom  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+RE_TAU = cases.RE_TAU
+
+
+def run_case(lines):
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as fh:
+        fh.write("\n".join(lines) + "\n")
+        path = fh.name
+    try:
+        out = subprocess.run([HARNESS, path], check=True, capture_output=True, text=True).stdout
+    finally:
+        os.unlink(path)
+    start = out.index("{")
+    return json.loads(out[start:].replace("NaN", "null").replace("-Infinity", "-1e999")
+                      .replace("Infinity", "1e999"))
+
+
+def read_dump(path):
+    with open(path, "rb") as fh:
+        buf = fh.read()
+    nc, nb, nf = struct.unpack_from("<3i", buf, 0)
+    data = np.frombuffer(buf, "<f8", offset=12)
+    names = ["Ux", "Uy", "p", "k", "omega", "nuT", "F1", "F2", "Pk", "CDkw"]
+    out, pos = {}, 0
+    for n in names:
+        out[n] = data[pos:pos + nc].tolist()
+        pos += nc
+    for n in ["Uxb", "Uyb", "pb", "kb", "omegab"]:
+        out[n] = data[pos:pos + nb].tolist()
+        pos += nb
+    out["flux"] = data[pos:pos + nf].tolist()
+    return out
+
+
+def settings_lines(s):
+    return [f"set max_iterations {s.max_iterations}", f"set convergence_tol {s.convergence_tol!r}",
+            f"set divergence_limit {s.divergence_limit!r}", f"set alpha_u {s.alpha_u!r}",
+            f"set alpha_p {s.alpha_p!r}", f"set alpha_k {s.alpha_k!r}",
+            f"set alpha_omega {s.alpha_omega!r}", f"set inner_iterations {s.inner_iterations}",
+            f"set inner_tolerance {s.inner_tolerance!r}", f"set turb_start_iter {s.turb_start_iter}",
+            f"set turb_update_interval {s.turb_update_interval}",
+            f"set body_force_x {s.body_force.x!r}", f"set pressure_solver {s.pressure_solver}",
+            f"set momentum_solver {s.momentum_solver}", f"set turbulence_solver {s.turbulence_solver}"]
+
+
+def problem_lines(prob, mesh_path, thetas, bc=(1.0, 1.0, 10.0), dump=None):
+    L = [f"mesh_file {mesh_path}", f"nu {prob.nu!r}", "bcs_channel %r %r %r" % bc]
+    L += settings_lines(prob.settings)
+    L.append("init %r %r 0 %r %r %r" % (prob.u_init.x, prob.u_init.y, prob.p_init, prob.k_init,
+                                        prob.omega_init))
+    for ob in prob.obs.observables:
+        L.append("obs_u %r %r %d %r %r" % (ob.location[0], ob.location[1], ob.component,
+                                           ob.observed, ob.sigma_obs))
+    L.append("params " + " ".join(str(i) for i in prob.params.active_indices))
+    for th in thetas:
+        L.append("theta " + " ".join(repr(float(v)) for v in th))
+    if dump:
+        L.append(f"dump {dump}")
+    return L
+
+
+def mesh_arrays(m: Mesh):
+    return {"owner": m.owner.tolist(), "neighbor": m.neighbor.tolist(),
+            "face_center": m.face_center.ravel().tolist(), "face_normal": m.face_normal.ravel().tolist(),
+            "face_area": m.face_area.tolist(), "face_delta": m.face_delta.tolist(),
+            "face_d": m.face_d.ravel().tolist(), "face_weight": m.face_weight.tolist(),
+            "face_shift": m.face_shift.ravel().tolist(), "cell_center": m.cell_center.ravel().tolist(),
+            "cell_volume": m.cell_volume.tolist(), "cell_face_ptr": m.cell_face_ptr.tolist(),
+            "cell_face_idx": m.cell_face_idx.tolist()}
+
+
+def write(name, obj):
+    with open(os.path.join(HERE, name), "w") as fh:
+        json.dump(obj, fh, indent=None, separators=(",", ":"))
+    print("wrote", name)
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    tmp = tempfile.mkdtemp()
+
+    # 1. generator meshes from the reference's own generators (bitwise geometry)
+    meshes = {}
+    pc = os.path.join(tmp, "pc.msh")
+    run_case(["mesh_periodic_channel 4 8 1.0 1.0 182.088 0.5", f"save_mesh {pc}", "nu 1",
+              "params 9 8"])
+    meshes["periodic_channel_4x8"] = {"args": [4, 8, 1.0, 1.0, 182.088, 0.5],
+                                      "mesh": mesh_arrays(Mesh.load_binary(pc))}
+    nx = 6
+    xs = [9.0 * i / nx for i in range(nx + 1)]
+    ys = [hill_bottom(x) for x in xs]
+    ys[nx] = ys[0]
+    hc = os.path.join(tmp, "hill.msh")
+    run_case(["mesh_curved %d " % (nx + 1) + " ".join("%r %r" % (x, y) for x, y in zip(xs, ys))
+              + " 3.035 4 5600 1.0", f"save_mesh {hc}", "nu 1", "params 9 8"])
+    meshes["hill_6x4"] = {"args": {"nx": 6, "ny": 4, "Lx": 9.0, "yTop": 3.035, "Re": 5600.0},
+                          "mesh": mesh_arrays(Mesh.load_binary(hc))}
+    write("ref_meshes.json", meshes)
+
+    # 2. small one-column channel: full solves and intermediate states
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
+    mp = os.path.join(tmp, "ch32.msh")
+    prob.mesh.save_binary(mp)
+    thetas = [[0.31, 0.09]] + cases.channel_draws(3).tolist()
+    res = run_case(problem_lines(prob, mp, thetas, dump=os.path.join(tmp, "ch32")))
+    rows = res["rows"]
+    rows[0]["fields"] = read_dump(os.path.join(tmp, "ch32_0.bin"))
+    write("ref_channel_small.json", {"case": "channel_problem(ny=32, ratio=1.15, n_probes=30, "
+                                             "tol=1e-6, max_iterations=40000)",
+                                     "thetas": thetas, "rows": rows})
+    steps = {}
+    for K in (1, 2, 6, 7, 30, 600):
+        p2 = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=K)
+        r = run_case(problem_lines(p2, mp, [[0.31, 0.09]], dump=os.path.join(tmp, f"st{K}")))
+        row = r["rows"][0]
+        row["fields"] = read_dump(os.path.join(tmp, f"st{K}_0.bin"))
+        steps[str(K)] = row
+    write("ref_channel_steps.json", {"case": "channel_problem(ny=32, ratio=1.15, n_probes=30, "
+                                             "tol=1e-6, max_iterations=K)", "theta": [0.31, 0.09],
+                                     "steps": steps})
+
+    # 3. configuration-1 channel and the first draws of the 64-draw sweep
+    theta64 = cases.channel_draws(64)
+    write("channel_theta64.json", {"seed": cases.CHANNEL_SEED, "theta": theta64.tolist()})
+    prob = cases.channel_problem()
+    mp = os.path.join(tmp, "ch128.msh")
+    prob.mesh.save_binary(mp)
+    thetas = [[0.31, 0.09]] + theta64[:3].tolist()
+    res = run_case(problem_lines(prob, mp, thetas))
+    write("ref_channel_c1.json", {"case": "channel_problem()", "thetas": thetas,
+                                  "rows": res["rows"]})
+
+    # 4. edge cases on the small channel (ForwardModel.cpp:45-135 conventions)
+    edge = {}
+    base = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
+    mp = os.path.join(tmp, "ch32.msh")
+    b2 = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
+    b2.settings.divergence_limit = 0.0
+    edge["divergence_limit_0"] = run_case(problem_lines(b2, mp, [[0.31, 0.09]]))["rows"][0]
+    b3 = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=50)
+    edge["max_iterations_50"] = run_case(problem_lines(b3, mp, [[0.31, 0.09]]))["rows"][0]
+    b4 = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
+    b4.obs.observables[0].observed = 1e308
+    edge["likelihood_overflow"] = run_case(problem_lines(b4, mp, [[0.31, 0.09]]))["rows"][0]
+    edge["out_of_bounds"] = run_case(problem_lines(base, mp, [[0.9, 0.09], [0.31, 5.0]]))["rows"]
+    write("ref_channel_edges.json", edge)
+
+    # 5. small periodic hill, both pressure solvers, capped run
+    for ps in ("AMG", "PCG"):
+        hp = cases.hill_problem(nx=30, ny=16, max_iterations=400, pressure_solver=ps)
+        mp = os.path.join(tmp, "hill.msh")
+        hp.mesh.save_binary(mp)
+        res = run_case(problem_lines(hp, mp, [[0.31, 0.09, 0.85, 0.5]],
+                                     bc=(1.0, 1e-4, 10.0), dump=os.path.join(tmp, "hill")))
+        row = res["rows"][0]
+        row["fields"] = read_dump(os.path.join(tmp, "hill_0.bin"))
+        write(f"ref_hill_small_{ps.lower()}.json",
+              {"case": f"hill_problem(nx=30, ny=16, max_iterations=400, pressure_solver='{ps}')",
+               "theta": [0.31, 0.09, 0.85, 0.5], "row": row})
+
+
+if __name__ == "__main__":
+    main()
