@@ -1,0 +1,274 @@
+"""Throughput benchmark of the forward-solve engine on one MI355X.
+
+A step is one batched evaluate of the 64-draw channel sweep (1x128
+geometric channel, ratio 1.05, tol 1e-8, FP64): all 64 coefficient draws
+run in one persistent-kernel launch. The line printed is
+
+  value        converged solves/s of the batched path (whole job, one GPU)
+  plain        the same draws solved one launch per draw (the plain per-solve
+               GPU path), on a bounded sample of draws
+  roofline     algorithmic bytes of the executed SIMPLE/Krylov stages per
+               launch / kernel time (HIP events on the engine stream), against
+               the 8 TB/s HBM peak; traffic is read from the committed
+               rocprofv3 --pmc summary when one exists
+  cpu_baseline the CPU FP64 oracle on the host cores, a bounded sample of the
+               same draws, one draw per thread
+  max_rel_err_vs_oracle  probe error of the GPU rows against the oracle rows
+               of that sample (statuses must match)
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bayesian-rans-uq_amd"))
+
+HBM_PEAK_GBS = 8000.0
+S_VAL, S_IDX = 8, 4  # FP64 values, int32 indices
+
+
+# ------------------------------------------------------------ byte model
+def _gg(N, F, B):
+    """One Green-Gauss gradient: field on cells and boundary faces, face
+    weight and S (3 doubles), owner/neighbour, gradient out (2 per cell)."""
+    return (N + B) * S_VAL + F * 3 * S_VAL + 2 * F * S_IDX + 2 * N * S_VAL
+
+
+def _assembly(N, F, k_in):
+    """One equation's LDU assembly: face geometry (S, delta, w), owner and
+    neighbour, k_in cell inputs, the face flux, diag/source/upper/lower out."""
+    return F * 3 * S_VAL + 2 * F * S_IDX + k_in * N * S_VAL + F * S_VAL + (2 * N + 2 * F) * S_VAL
+
+
+def _spmv(N, F):
+    return (3 * N + 2 * F) * S_VAL + 2 * F * S_IDX
+
+
+def bicgstab_iteration_bytes(N, F):
+    return 2 * _spmv(N, F) + 2 * 3 * N * S_VAL + 9 * N * S_VAL + 14 * N * S_VAL
+
+
+def pcg_iteration_bytes(N, F):
+    return _spmv(N, F) + 3 * N * S_VAL + 5 * N * S_VAL + 9 * N * S_VAL
+
+
+def outer_iteration_bytes(N, F, B, turbulence):
+    """Compulsory traffic of one SIMPLE outer iteration as the engine runs it
+    (Krylov iterations excluded): each distinct array a stage reads, counted
+    once, plus each array it writes."""
+    b = 0
+    b += 4 * N * S_VAL                                   # U -> U_prev copy
+    b += F * 3 * S_VAL + 2 * F * S_IDX + 4 * N * S_VAL + F * S_VAL   # interpolated flux
+    n_grad = 5 if turbulence else 3                      # U(2), p (+k, omega)
+    b += n_grad * _gg(N, F, B)
+    if turbulence:
+        b += 17 * N * S_VAL                              # SST closure sweep
+    b += 2 * _assembly(N, F, 9)                          # x/y momentum
+    b += _gg(N, F, B)                                    # grad Ux for the y transpose stress
+    b += _assembly(N, F, 7)                              # Rhie-Chow flux + p' assembly
+    b += _gg(N, F, B) + 9 * N * S_VAL                    # grad p', U/p correction
+    b += F * 3 * S_VAL + 2 * F * S_IDX + 2 * N * S_VAL   # flux correction
+    if turbulence:
+        b += _assembly(N, F, 8) + _assembly(N, F, 10)    # k, omega
+        b += 2 * 3 * N * S_VAL                           # clamps and change norms
+    b += 4 * N * S_VAL                                   # convergence norms
+    return b
+
+
+def solve_bytes(N, F, B, iters, krylov, turb_start, settings):
+    """Algorithmic bytes of one solve from its executed iteration counts."""
+    turb_iters = 0
+    if iters > turb_start:
+        turb_iters = (iters - turb_start + settings.turb_update_interval - 1) \
+            // settings.turb_update_interval
+    plain_iters = iters - turb_iters
+    b = plain_iters * outer_iteration_bytes(N, F, B, False)
+    b += turb_iters * outer_iteration_bytes(N, F, B, True)
+    per = {"BiCGSTAB": bicgstab_iteration_bytes(N, F), "PCG": pcg_iteration_bytes(N, F)}
+    solvers = [settings.momentum_solver, settings.momentum_solver, settings.pressure_solver,
+               settings.turbulence_solver, settings.turbulence_solver]
+    for count, name in zip(krylov, solvers):
+        b += int(count) * per[name]
+    return b
+
+
+# --------------------------------------------------------------- helpers
+def _cores():
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = os.cpu_count() or 1
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, min(n, 16))
+
+
+def _traffic_from_profile():
+    path = os.path.join(ROOT, "profiles", "simple_solve_kernel_channel_sweep64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(prob, theta, cores):
+    """The CPU FP64 oracle, one draw per thread (the ctypes call releases the
+    GIL), on `cores` draws of the sweep."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
+                                prob.init_vector())
+    oracle_py.lib()
+    rows = theta[:cores]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        res = list(ex.map(lambda th: O.evaluate(prob.coefficients(th)), rows))
+    dt = time.perf_counter() - t0
+    conv = sum(1 for r in res if r["status"] == 0)
+    return res, {"value": conv / dt, "unit": "converged solves/s", "cores": cores,
+                 "kind": "port",
+                 "sample": f"{len(rows)} draws of the 64-draw sweep, one per thread, "
+                           f"{dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--draws", type=int, default=64)
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(draws are independent: several GPUs would each run their own batch)",
+              file=sys.stderr)
+        return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+
+    from cfdr_amd import cases, engine
+
+    prob = cases.channel_problem()
+    theta = cases.channel_draws(args.draws)
+    mesh = prob.mesh
+    N, F, B = mesh.n_cells, mesh.n_internal, mesh.n_boundary
+    batch = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
+                                prob.settings, prob.u_init, prob.p_init, prob.k_init,
+                                prob.omega_init, block_size=args.block_size)
+    print(f"# workload: {args.draws} draws, {N} cells, build {engine.build_info()}",
+          file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        batch.evaluate(theta)
+        print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    # evaluate() returns after its stream has synchronised: the timed region
+    # starts and ends with the device idle.
+    step_ms, kernel_ms, results = [], [], []
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        r = batch.evaluate(theta)
+        step_ms.append(1e3 * (time.perf_counter() - t0))
+        kernel_ms.append(r.kernel_ms)
+        results.append(r)
+        print(f"# step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms", file=sys.stderr, flush=True)
+    total_s = time.perf_counter() - t_all
+    last = results[-1]
+    converged = int(sum(int(np.sum(r.convergence_mask)) for r in results))
+    value = converged / total_s
+    for r in results[1:]:
+        if not np.array_equal(r.predictions, last.predictions, equal_nan=True):
+            raise SystemExit("bench.py: repeated steps differ")
+
+    # roofline of the solve kernel (the only kernel of a step)
+    launch_bytes = sum(solve_bytes(N, F, B, int(last.iteration_counts[i]),
+                                   last.krylov_iterations[i], prob.settings.turb_start_iter,
+                                   prob.settings)
+                       for i in range(len(theta)) if last.statuses[i] in (0, 1))
+    kernel_avg_s = statistics.mean(kernel_ms) / 1e3
+    achieved = launch_bytes / kernel_avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": _traffic_from_profile(),
+                "kernel": "simple_solve_kernel", "kernel_ms": round(1e3 * kernel_avg_s, 3),
+                "algorithmic_bytes_per_launch": int(launch_bytes)}
+
+    plain = None
+    if not args.no_plain and args.plain_sample > 0:
+        lat = []
+        conv_plain = 0
+        for i in range(args.plain_sample):
+            t0 = time.perf_counter()
+            r1 = batch.evaluate(theta[i:i + 1])
+            lat.append(time.perf_counter() - t0)
+            conv_plain += int(r1.convergence_mask[0])
+            if not np.array_equal(r1.predictions[0], last.predictions[i], equal_nan=True):
+                raise SystemExit("bench.py: a draw differs alone and in the batch")
+        plain = {"value": conv_plain / sum(lat), "unit": "converged solves/s",
+                 "sample": f"first {args.plain_sample} draws, one launch each",
+                 "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 1)}
+
+    cpu = None
+    err = None
+    if not args.no_cpu_baseline:
+        rows, cpu = cpu_baseline(prob, theta, _cores())
+        errs = []
+        for i, o in enumerate(rows):
+            if o["status"] != last.statuses[i]:
+                raise SystemExit(f"bench.py: draw {i} status {last.statuses[i]} vs oracle "
+                                 f"{o['status']}")
+            if o["status"] in (0, 1):
+                p, q = last.predictions[i], o["predictions"]
+                errs.append(float(np.max(np.abs(p - q) / np.abs(q))))
+        err = max(errs) if errs else None
+
+    line = {
+        "metric": "converged SST solves/s (served vs plain) at <=1e-6 rel err; p50/p99 latency",
+        "value": round(value, 4),
+        "unit": "converged solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(statistics.mean(step_ms), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "64-draw channel sweep: Student-t(4) draws of (a1, beta*), seed "
+                               "20261010, 1x128 geometric channel (ratio 1.05), Re_tau 182.088, "
+                               "tol 1e-8, PCG pressure / BiCGSTAB transport, 30 probes",
+                   "draws": int(len(theta)), "cells": N, "path": "batched persistent kernel"},
+        "converged": converged,
+        "statuses": {str(s): int(np.sum(last.statuses == s)) for s in np.unique(last.statuses)},
+        "iterations_mean": float(np.mean(last.iteration_counts)),
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 1),
+                       "p99": round(float(np.percentile(step_ms, 99)), 1),
+                       "note": "batch step latency: every draw of a step completes together"},
+        "plain": plain,
+        "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "max_rel_err_vs_oracle": err,
+        "block_size": batch.last_timing()["block_size"],
+        "arena_in_lds": batch.last_timing()["arena_in_lds"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,203 @@
+"""GPU parity of the gfx950 engine, called through the C-ABI
+(include/rans_amd.h via cfdr_amd.engine), against the fixtures written by the
+reference solver and against the CPU oracle on the same seeded inputs.
+
+Tolerances: the converged/failed status and the iteration count must match
+exactly; observation values (velocity probes) agree within REL_OBS = 1e-6
+relative, the bar the forward-solve contract sets. Field comparisons of
+intermediate states use FIELD_TOL (scaled by the field's magnitude). The
+engine follows the reference's FP64 operation order, so the observed
+differences are at round-off level (1e-15..1e-13); the stated bounds are the
+contract, not the observation.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cfdr_amd import cases, engine
+from fixtures import FIELD_NAMES, load, max_rel, max_scaled, small_channel, small_hill
+
+pytestmark = pytest.mark.gpu
+
+REL_OBS = 1e-6
+FIELD_TOL = 1e-9
+
+
+def _batch(prob, **kw):
+    return engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
+                               prob.u_init, prob.p_init, prob.k_init, prob.omega_init, **kw)
+
+
+def _oracle(prob):
+    import oracle_py
+    return oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
+                                   prob.init_vector())
+
+
+def _fields(res, i, nc):
+    f = res.fields[i]
+    out = {name: f[j * nc:(j + 1) * nc] for j, name in enumerate(FIELD_NAMES)}
+    out["flux"] = f[10 * nc:]
+    return out
+
+
+def _check_row(res, i, row):
+    assert res.statuses[i] == row["status"], (i, res.errors[i])
+    assert res.iteration_counts[i] == row["iters"]
+    assert bool(res.stalled[i]) == row["stalled"]
+    assert bool(res.convergence_mask[i]) == (row["status"] == 0)
+    if row["status"] in (0, 1):
+        assert max_rel(res.predictions[i], row["predictions"]) <= REL_OBS
+        assert res.log_likelihoods[i] == pytest.approx(row["loglik"], rel=REL_OBS)
+    else:
+        assert np.all(np.isnan(res.predictions[i]))
+        assert res.log_likelihoods[i] == row["loglik"]
+
+
+def test_small_channel_rows_match_reference():
+    d = load("ref_channel_small.json")
+    prob = small_channel()
+    res = _batch(prob).evaluate(np.asarray(d["thetas"]), fields=True)
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+        assert res.final_worst_measures[i] == pytest.approx(row["final_worst_measure"], rel=1e-4)
+        assert res.contraction_rates[i] == pytest.approx(row["contraction_rate_u"], rel=1e-6)
+    f = _fields(res, 0, prob.mesh.n_cells)
+    for name in FIELD_NAMES + ["flux"]:
+        assert max_scaled(f[name], d["rows"][0]["fields"][name]) <= FIELD_TOL, name
+
+
+@pytest.mark.parametrize("K", ["1", "2", "6", "7", "30", "600"])
+def test_intermediate_states_match_reference(K):
+    """State after K iterations: initialisation, startup nuT floor, first
+    turbulence updates, planar omega balance, Krylov warm starts."""
+    d = load("ref_channel_steps.json")
+    row = d["steps"][K]
+    prob = small_channel(max_iterations=int(K))
+    res = _batch(prob).evaluate(np.asarray([d["theta"]]), fields=True)
+    assert res.statuses[0] == 1 and res.iteration_counts[0] == int(K)
+    f = _fields(res, 0, prob.mesh.n_cells)
+    for name in FIELD_NAMES + ["flux"]:
+        assert max_scaled(f[name], row["fields"][name]) <= FIELD_TOL, name
+    assert max_rel(res.predictions[0], row["predictions"]) <= REL_OBS
+
+
+def test_status_edge_cases_match_reference():
+    """ForwardModel.cpp:45-135 conventions on the GPU path."""
+    e = load("ref_channel_edges.json")
+    prob = small_channel()
+    prob.settings.divergence_limit = 0.0
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    _check_row(res, 0, e["divergence_limit_0"])
+    assert res.final_worst_measures[0] == math.inf
+    prob = small_channel(max_iterations=50)
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    _check_row(res, 0, e["max_iterations_50"])
+    prob = small_channel()
+    prob.obs.observables[0].observed = 1e308
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    row = e["likelihood_overflow"]
+    assert (res.statuses[0], res.iteration_counts[0], res.log_likelihoods[0]) == \
+        (row["status"], row["iters"], row["loglik"])
+    assert np.all(np.isnan(res.predictions[0])) and not res.convergence_mask[0]
+    assert "log-likelihood" in res.errors[0]
+    prob = small_channel()
+    res = _batch(prob).evaluate(np.asarray([[0.9, 0.09], [0.31, 0.09], [0.31, 5.0],
+                                            [float("nan"), 0.09]]))
+    for i in (0, 2, 3):
+        _check_row(res, i, e["out_of_bounds"][0])
+        assert "out of bounds" in res.errors[i]
+    small = load("ref_channel_small.json")
+    _check_row(res, 1, small["rows"][0])
+
+
+def test_config1_channel_rows_match_reference():
+    """1x128 geometric channel, tol 1e-8: default coefficients and the first
+    three draws of the seeded 64-draw sweep."""
+    d = load("ref_channel_c1.json")
+    res = _batch(cases.channel_problem()).evaluate(np.asarray(d["thetas"]))
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+
+
+def test_three_column_channel_matches_oracle():
+    """Three-column mesh: four faces per cell, distinct periodic neighbours."""
+    prob = cases.channel_problem(ny=32, nx=3, ratio=1.15, n_probes=30, tol=1e-6,
+                                 max_iterations=40000)
+    theta = cases.channel_draws(3)
+    res = _batch(prob).evaluate(theta)
+    O = _oracle(prob)
+    for i in range(len(theta)):
+        o = O.evaluate(prob.coefficients(theta[i]))
+        assert res.statuses[i] == o["status"] and res.iteration_counts[i] == o["iters"]
+        assert max_rel(res.predictions[i], o["predictions"]) <= REL_OBS
+
+
+@pytest.mark.parametrize("solver", ["PCG"])
+def test_small_hill_matches_reference(solver):
+    """30x16 periodic hill (curved, non-orthogonal faces, arena in HBM),
+    400 capped iterations."""
+    d = load(f"ref_hill_small_{solver.lower()}.json")
+    prob = small_hill(solver)
+    b = _batch(prob)
+    res = b.evaluate(np.asarray([d["theta"]]), fields=True)
+    _check_row(res, 0, d["row"])
+    f = _fields(res, 0, prob.mesh.n_cells)
+    for name in ("Ux", "Uy", "p", "k", "omega", "nuT", "flux"):
+        assert max_scaled(f[name], d["row"]["fields"][name]) <= FIELD_TOL, name
+
+
+def test_batch_results_independent_of_batch_composition():
+    """Each row is bitwise the same alone, inside a batch and in another order
+    (test_batch_determinism.cpp:49-63)."""
+    prob = small_channel()
+    theta = cases.channel_draws(6)
+    b = _batch(prob)
+    full = b.evaluate(theta)
+    rev = b.evaluate(theta[::-1].copy())
+    for i in range(len(theta)):
+        alone = b.evaluate(theta[i:i + 1])
+        for other, j in ((alone, 0), (rev, len(theta) - 1 - i)):
+            assert other.statuses[j] == full.statuses[i]
+            assert other.iteration_counts[j] == full.iteration_counts[i]
+            assert np.array_equal(other.predictions[j], full.predictions[i])
+            assert other.log_likelihoods[j] == full.log_likelihoods[i]
+
+
+def test_block_sizes_agree():
+    """Launch shape changes only the reduction grouping."""
+    prob = small_channel()
+    theta = cases.channel_draws(2)
+    base = _batch(prob, block_size=64).evaluate(theta)
+    for bs in (128, 256):
+        r = _batch(prob, block_size=bs).evaluate(theta)
+        assert np.array_equal(r.statuses, base.statuses)
+        assert max_rel(r.predictions, base.predictions) <= REL_OBS
+
+
+def test_empty_batch_and_argument_errors():
+    prob = small_channel()
+    b = _batch(prob)
+    r = b.evaluate(np.zeros((0, 2)))
+    assert r.predictions.shape == (0, prob.obs.n_obs())
+    with pytest.raises(ValueError, match="workers"):
+        b.evaluate(np.asarray([[0.31, 0.09]]), workers=0)
+    with pytest.raises(ValueError, match="n_parameters"):
+        b.evaluate(np.asarray([[0.31, 0.09, 0.5]]))
+
+
+def test_sweep_draws_match_oracle():
+    """Eight draws of the seeded channel sweep on the small channel against the
+    oracle run on the same inputs."""
+    prob = small_channel()
+    theta = cases.channel_draws(8)
+    res = _batch(prob).evaluate(theta)
+    O = _oracle(prob)
+    for i in range(len(theta)):
+        o = O.evaluate(prob.coefficients(theta[i]))
+        assert res.statuses[i] == o["status"] and res.iteration_counts[i] == o["iters"]
+        assert max_rel(res.predictions[i], o["predictions"]) <= REL_OBS
+        assert res.krylov_iterations[i].tolist() == o["krylov_iters"]
