@@ -39,19 +39,33 @@ enum FaceVar {
     W_phi, W_phiBar, W_phiPred, W_pCoeff, W_Sx, W_Sy, W_A, W_dlt, W_w, W_wm, W_nx, W_ny, W_ex,
     W_ey, W_dOx, W_dOy, W_dNx, W_dNy, W_omg, W_COUNT
 };
-enum BndVar { B_Ux, B_Uy, B_p, B_k, B_w, B_pP, B_COUNT };
+// boundary-face record: face values, then the prescribed values of the
+// face's patch (velocity, pressure, k, omega)
+enum BndVar { B_Ux, B_Uy, B_p, B_k, B_w, B_pP, B_uvx, B_uvy, B_pv, B_kv, B_wv, B_COUNT };
+// observation record: probe stencil (offset, 2x2 least-squares system),
+// skin-friction blend, references and the monitor floor
+enum ObsVar { O_offx, O_offy, O_axx, O_axy, O_ayy, O_det, O_t, O_refv, O_refp, O_refrho, O_floor,
+              O_COUNT };
 constexpr int kNA = V_COUNT | 1;
 constexpr int kNF = W_COUNT | 1;
 constexpr int kNB = B_COUNT | 1;
+constexpr int kNO = O_COUNT | 1;
+// boundary flag word: bit 0 wall, bit 1 outlet, then the patch's BC types
+// for velocity, pressure, k and omega in 4-bit fields from bit 4
+constexpr int kFlagWall = 1, kFlagOutlet = 2;
+constexpr int bc_type(int flag, int field) { return (flag >> (4 + 4 * field)) & 15; }
 
 // Offsets in doubles inside one draw's arena; the int tables start at
 // ibase (in doubles) and their offsets are in ints from there.
 struct ArenaLayout {
     int64_t stride;                  // doubles per draw
     int64_t face, bnd, ring, q, qprev;
+    int64_t obs, onb;                // [nobs][kNO] probe records, [nnb][3] stencil dx, dy, w
     int64_t ibase;                   // start of the int tables (doubles)
     int64_t islot, iface, ibflag;    // in ints: [nc][8] slot codes then neighbours,
                                      // [nf][2] owner/neighbour, [nb] boundary flags
+    int64_t iobs, ionb;              // in ints: [nobs][8] type|component<<4, cell,
+                                     // stencil begin/end, friction faces a/b; [nnb][2] cell, face
     int64_t image_bytes;             // bytes of the pre-filled image copied at launch
 };
 
@@ -61,16 +75,7 @@ struct DevMesh {
     int nc, nif, nf, nb, npatch;
     int has_outlet, planar;
     const double *wall_y;        // [nc] wall-face distance of the constrained cell, NaN otherwise
-    const int *bpatch;           // [nb] patch of each boundary face
-    const int *u_type, *p_type, *k_type, *w_type;                  // [npatch]
-    const double *u_valx, *u_valy, *p_val, *k_val, *w_val;         // [npatch]
-    // prepared observations
-    int nobs;
-    const int *ob_type, *ob_comp, *ob_cell, *ob_nb_ptr, *ob_nb_cell, *ob_nb_face;
-    const int *ob_fa, *ob_fb;
-    const double *ob_offx, *ob_offy, *ob_axx, *ob_axy, *ob_ayy, *ob_det;
-    const double *ob_nb_dx, *ob_nb_dy, *ob_nb_w, *ob_t, *ob_refv, *ob_refp, *ob_refrho;
-    const double *ob_floor;
+    int nobs;                    // prepared observations (their stencils are in the image)
     ArenaLayout lay;  // per-draw arena layout for this mesh
 };
 

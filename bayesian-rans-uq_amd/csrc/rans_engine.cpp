@@ -199,7 +199,8 @@ bool planar_balance(const rq_mesh_desc &m, const std::vector<int> &bpatch, const
     return true;
 }
 
-int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, int window) {
+int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, int nnb,
+                     int window) {
     (void)nif;
     int64_t o = (int64_t)nc * kNA;
     L.face = o;
@@ -212,6 +213,10 @@ int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, 
     o += std::max(nobs, 1);
     L.qprev = o;
     o += std::max(nobs, 1);
+    L.obs = o;
+    o += (int64_t)std::max(nobs, 1) * kNO;
+    L.onb = o;
+    o += 3 * (int64_t)std::max(nnb, 1);
     o = (o + 1) & ~int64_t(1);
     L.ibase = o;
     int64_t ni = 0;
@@ -221,6 +226,10 @@ int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, 
     ni += 2 * (int64_t)nf;
     L.ibflag = ni;
     ni += std::max(nb, 1);
+    L.iobs = ni;
+    ni += 8 * (int64_t)std::max(nobs, 1);
+    L.ionb = ni;
+    ni += 2 * (int64_t)std::max(nnb, 1);
     o += (ni + 1) / 2;
     o = (o + 1) & ~int64_t(1);  // 16-byte multiple
     L.stride = o;
@@ -357,8 +366,24 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         // ---- arena image: geometry in the cell/face records and the int
         //      tables, each face constant computed with the reference's arithmetic
         const int no = O.n_obs;
+        const int nnb = no > 0 ? O.nb_ptr[no] : 0;
+        for (int i = 0; i < no; ++i) {
+            const int t = O.type[i];
+            if (t < 0 || t > 2) throw Fail{RQ_EINVAL, "unsupported observation type"};
+            if ((t == OBS_VELOCITY || t == OBS_PRESSURE) && (O.cell[i] < 0 || O.cell[i] >= nc))
+                throw Fail{RQ_EINVAL, "observation probe cell out of range"};
+            if (O.component[i] < 0 || O.component[i] > 2)
+                throw Fail{RQ_EINVAL, "observation component out of range"};
+            if (O.nb_ptr[i + 1] < O.nb_ptr[i] || O.nb_ptr[i] < 0)
+                throw Fail{RQ_EINVAL, "invalid observation stencil"};
+            for (int q = O.nb_ptr[i]; q < O.nb_ptr[i + 1]; ++q)
+                if (O.nb_cell[q] >= nc || (O.nb_cell[q] < 0 && (O.nb_face[q] < nif || O.nb_face[q] >= nf)))
+                    throw Fail{RQ_EINVAL, "observation stencil index out of range"};
+            if (t == OBS_SKIN_FRICTION && (O.face_a[i] >= nf || O.face_b[i] >= nf))
+                throw Fail{RQ_EINVAL, "skin-friction stencil face out of range"};
+        }
         ArenaLayout L{};
-        build_layout(L, nc, nif, nf, nb, no, s.convergence_window);
+        build_layout(L, nc, nif, nf, nb, no, nnb, s.convergence_window);
         std::vector<double> image(L.stride, 0.0);
         double *CR = image.data(), *FR = image.data() + L.face;
         int *IB = reinterpret_cast<int *>(image.data() + L.ibase);
@@ -386,8 +411,47 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
             IFA[2 * f + 1] = m.neighbor[f];
         }
         for (int b = 0; b < nb; ++b) {
-            const int t = m.patch_type[bpatch[b]];
-            IBF[b] = (t == PATCH_WALL ? 1 : 0) | (t == PATCH_OUTLET ? 2 : 0);
+            const int pi = bpatch[b];
+            const int t = m.patch_type[pi];
+            IBF[b] = (t == PATCH_WALL ? kFlagWall : 0) | (t == PATCH_OUTLET ? kFlagOutlet : 0) |
+                     (bc.u_type[pi] << 4) | (bc.p_type[pi] << 8) | (bc.k_type[pi] << 12) |
+                     (bc.w_type[pi] << 16);
+            double *R = image.data() + L.bnd + (int64_t)b * kNB;
+            R[B_uvx] = bc.u_val[2 * pi];
+            R[B_uvy] = bc.u_val[2 * pi + 1];
+            R[B_pv] = bc.p_val[pi];
+            R[B_kv] = bc.k_val[pi];
+            R[B_wv] = bc.w_val[pi];
+        }
+        // prepared observations: one record per probe plus its stencil entries
+        for (int i = 0; i < no; ++i) {
+            double *R = image.data() + L.obs + (int64_t)i * kNO;
+            R[O_offx] = O.offx[i];
+            R[O_offy] = O.offy[i];
+            R[O_axx] = O.axx[i];
+            R[O_axy] = O.axy[i];
+            R[O_ayy] = O.ayy[i];
+            R[O_det] = O.det[i];
+            R[O_t] = O.t[i];
+            R[O_refv] = O.ref_velocity[i];
+            R[O_refp] = O.ref_pressure[i];
+            R[O_refrho] = O.ref_density[i];
+            R[O_floor] = std::sqrt(O.variance[i]);
+            int *Q = IB + L.iobs + 8 * (int64_t)i;
+            Q[0] = O.type[i] | (O.component[i] << 4);
+            Q[1] = O.cell[i];
+            Q[2] = O.nb_ptr[i];
+            Q[3] = O.nb_ptr[i + 1];
+            Q[4] = O.face_a[i];
+            Q[5] = O.face_b[i];
+        }
+        for (int q = 0; q < nnb; ++q) {
+            double *R = image.data() + L.onb + 3 * (int64_t)q;
+            R[0] = O.nb_dx[q];
+            R[1] = O.nb_dy[q];
+            R[2] = O.nb_w[q];
+            IB[L.ionb + 2 * q] = O.nb_cell[q];
+            IB[L.ionb + 2 * q + 1] = O.nb_face[q];
         }
         const double *CC = m.cell_center, *FC = m.face_center, *FN = m.face_normal;
         const double *FSH = m.face_shift, *FD = m.face_d;
@@ -453,73 +517,22 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
             if (m.patch_type[pi] == PATCH_OUTLET && m.patch_face_ptr[pi + 1] > m.patch_face_ptr[pi])
                 has_outlet = true;
 
-        // ---- cold tables (boundary conditions, observations) in one blob
-        auto vecd = [](const double *v, int n) { return std::vector<double>(v, v + n); };
-        auto veci = [](const int32_t *v, int n) { return std::vector<int>(v, v + n); };
+        // ---- device copy: the arena image and the initial wall-omega table
         Blob B;
         DevMesh D{};
         const size_t o_img = B.add(image);
-        size_t o_wy = B.add(wall_y), o_bp = B.add(bpatch);
-        size_t o_ut = B.add(veci(bc.u_type, np));
-        size_t o_ptp = B.add(veci(bc.p_type, np)), o_kt = B.add(veci(bc.k_type, np));
-        size_t o_wt = B.add(veci(bc.w_type, np));
-        std::vector<double> uvx(np), uvy(np);
-        for (int pi = 0; pi < np; ++pi) {
-            uvx[pi] = bc.u_val[2 * pi];
-            uvy[pi] = bc.u_val[2 * pi + 1];
-        }
-        size_t o_uvx = B.add(uvx);
-        size_t o_uvy = B.add(uvy), o_pv = B.add(vecd(bc.p_val, np)), o_kv = B.add(vecd(bc.k_val, np));
-        size_t o_wv = B.add(vecd(bc.w_val, np));
-        const int nnb = no > 0 ? O.nb_ptr[no] : 0;
-        std::vector<double> floors(no);
-        for (int i = 0; i < no; ++i) floors[i] = std::sqrt(O.variance[i]);
-        std::vector<int> nbptr(no + 1, 0);
-        if (no > 0) nbptr.assign(O.nb_ptr, O.nb_ptr + no + 1);
-        size_t o_ot = B.add(veci(O.type, no)), o_oc = B.add(veci(O.component, no));
-        size_t o_ocell = B.add(veci(O.cell, no)), o_onp = B.add(nbptr);
-        size_t o_onc = B.add(veci(O.nb_cell, nnb)), o_onf = B.add(veci(O.nb_face, nnb));
-        size_t o_ofa = B.add(veci(O.face_a, no)), o_ofb = B.add(veci(O.face_b, no));
-        size_t o_ox = B.add(vecd(O.offx, no)), o_oy = B.add(vecd(O.offy, no));
-        size_t o_axx = B.add(vecd(O.axx, no)), o_axy = B.add(vecd(O.axy, no));
-        size_t o_ayy = B.add(vecd(O.ayy, no)), o_det = B.add(vecd(O.det, no));
-        size_t o_ndx = B.add(vecd(O.nb_dx, nnb)), o_ndy = B.add(vecd(O.nb_dy, nnb));
-        size_t o_nw = B.add(vecd(O.nb_w, nnb)), o_t = B.add(vecd(O.t, no));
-        size_t o_rv = B.add(vecd(O.ref_velocity, no)), o_rp = B.add(vecd(O.ref_pressure, no));
-        size_t o_rr = B.add(vecd(O.ref_density, no)), o_fl = B.add(floors);
-        for (int i = 0; i < no; ++i) {
-            const int t = O.type[i];
-            if (t < 0 || t > 2) throw Fail{RQ_EINVAL, "unsupported observation type"};
-            if ((t == OBS_VELOCITY || t == OBS_PRESSURE) && (O.cell[i] < 0 || O.cell[i] >= nc))
-                throw Fail{RQ_EINVAL, "observation probe cell out of range"};
-            if (O.nb_ptr[i + 1] < O.nb_ptr[i]) throw Fail{RQ_EINVAL, "invalid observation stencil"};
-            for (int q = O.nb_ptr[i]; q < O.nb_ptr[i + 1]; ++q)
-                if (O.nb_cell[q] >= nc || (O.nb_cell[q] < 0 && (O.nb_face[q] < nif || O.nb_face[q] >= nf)))
-                    throw Fail{RQ_EINVAL, "observation stencil index out of range"};
-            if (t == OBS_SKIN_FRICTION && (O.face_a[i] >= nf || O.face_b[i] >= nf))
-                throw Fail{RQ_EINVAL, "skin-friction stencil face out of range"};
-        }
+        size_t o_wy = B.add(wall_y);
 
         HIPCHECK(hipGetDevice(&E->device));
         HIPCHECK(hipMalloc(&E->d_blob, B.host.size()));
         HIPCHECK(hipMemcpy(E->d_blob, B.host.data(), B.host.size(), hipMemcpyHostToDevice));
         char *b0 = E->d_blob;
-        auto I = [&](size_t o) { return reinterpret_cast<const int *>(b0 + o); };
         auto Dd = [&](size_t o) { return reinterpret_cast<const double *>(b0 + o); };
         D.img = Dd(o_img);
         D.nc = nc; D.nif = nif; D.nf = nf; D.nb = nb; D.npatch = np;
         D.has_outlet = has_outlet; D.planar = planar;
-        D.wall_y = Dd(o_wy); D.bpatch = I(o_bp);
-        D.u_type = I(o_ut); D.p_type = I(o_ptp); D.k_type = I(o_kt);
-        D.w_type = I(o_wt); D.u_valx = Dd(o_uvx); D.u_valy = Dd(o_uvy); D.p_val = Dd(o_pv);
-        D.k_val = Dd(o_kv); D.w_val = Dd(o_wv);
+        D.wall_y = Dd(o_wy);
         D.nobs = no;
-        D.ob_type = I(o_ot); D.ob_comp = I(o_oc); D.ob_cell = I(o_ocell); D.ob_nb_ptr = I(o_onp);
-        D.ob_nb_cell = I(o_onc); D.ob_nb_face = I(o_onf); D.ob_fa = I(o_ofa); D.ob_fb = I(o_ofb);
-        D.ob_offx = Dd(o_ox); D.ob_offy = Dd(o_oy); D.ob_axx = Dd(o_axx); D.ob_axy = Dd(o_axy);
-        D.ob_ayy = Dd(o_ayy); D.ob_det = Dd(o_det); D.ob_nb_dx = Dd(o_ndx);
-        D.ob_nb_dy = Dd(o_ndy); D.ob_nb_w = Dd(o_nw); D.ob_t = Dd(o_t); D.ob_refv = Dd(o_rv);
-        D.ob_refp = Dd(o_rp); D.ob_refrho = Dd(o_rr); D.ob_floor = Dd(o_fl);
         D.lay = L;
         E->lay = L;
         HIPCHECK(hipMalloc(&E->d_mesh, sizeof(DevMesh)));
@@ -561,7 +574,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.mixed_precision = s.mixed_precision;
         S.profile = s.profile;
         if (s.block_size) E->block = s.block_size;
-        else E->block = nc <= 128 ? 64 : nc <= 4096 ? 256 : 512;
+        else E->block = nc <= 64 ? 64 : nc <= 128 ? 128 : nc <= 256 ? 256 : 512;
         int maxLds = 0;
         HIPCHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                        E->device));

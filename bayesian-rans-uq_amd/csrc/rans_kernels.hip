@@ -127,7 +127,7 @@ DEV bool slot_own(int code) { return (code >> 30) == 0; }
 struct Book {
     int done, status, error, finalIter, stalled, haveQPrev, turbEst, pad;
     double lastK, lastOm, normUx0, normUy0, normP0, pfRef, worst, rate;
-    long long nEst[4];
+    int nEst[4], pos[4];  // samples pushed, ring slot of the newest sample
     long long kry[5];
     long long tlast;
     long long cyc[kPhases];
@@ -144,6 +144,7 @@ struct Draw {
     const DevSettings &S;
     const double *c;  // 11 coefficients (LDS)
     Book *bk;
+    Krylov *kres;  // LDS slot broadcasting a wave-resident solve's result
     Reducer<BS> red;
     int nc, nif, nf, nb;
     bool planar, has_outlet;
@@ -153,11 +154,13 @@ struct Draw {
     typename Space<kLDS>::I4 *slot4;            // [nc][2]: slot codes, neighbour cells
     typename Space<kLDS>::I2 *fon;              // [nf]: owner, neighbour
     typename Space<kLDS>::I *bflag;             // [nb]
-    D *ringb, *q, *qprev;
+    D *ringb, *q, *qprev, *Ob, *Onb;
+    typename Space<kLDS>::I *obi, *onbi;  // probe ints, stencil cell/face pairs
+    int nobs;
 
     DEV Draw(const DevMesh &m, const DevSettings &s, const double *coeffs, double *A, Book *b,
-             double *redbuf)
-        : M(m), S(s), c(coeffs), bk(b) {
+             double *redbuf, Krylov *kr)
+        : M(m), S(s), c(coeffs), bk(b), kres(kr) {
         red.buf = redbuf;
         red.parity = 0;
         nc = M.nc;
@@ -175,17 +178,23 @@ struct Draw {
         fon = (typename Space<kLDS>::I2 *)(ib + L.iface);
         bflag = (typename Space<kLDS>::I *)(ib + L.ibflag);
         ringb = (D *)(A + L.ring);
+        Ob = (D *)(A + L.obs);
+        Onb = (D *)(A + L.onb);
+        obi = (typename Space<kLDS>::I *)(ib + L.iobs);
+        onbi = (typename Space<kLDS>::I *)(ib + L.ionb);
+        nobs = M.nobs;
         q = (D *)(A + L.q);
         qprev = (D *)(A + L.qprev);
     }
 
     // record accessors: cell, face and boundary-face variables
-    DEV D &C(int i, int v) const { return Cb[i * kNA + v]; }
-    DEV D &Fc(int f, int v) const { return Fb[f * kNF + v]; }
-    DEV D &Bd(int b, int v) const { return Bb[b * kNB + v]; }
+    // (record indices stay below 2^23, so 24-bit multiplies are exact)
+    DEV D &C(int i, int v) const { return Cb[__mul24(i, kNA) + v]; }
+    DEV D &Fc(int f, int v) const { return Fb[__mul24(f, kNF) + v]; }
+    DEV D &Bd(int b, int v) const { return Bb[__mul24(b, kNB) + v]; }
 
     // off-diagonal coefficient of slot s of cell ci
-    DEV D &off(int ci, int s) const { return Cb[ci * kNA + V_off0 + s]; }
+    DEV D &off(int ci, int s) const { return Cb[__mul24(ci, kNA) + V_off0 + s]; }
 
     // ---------------------------------------------------------------- SST
     // SSTModel pointwise functions (core/src/SSTModel.cpp:15-108)
@@ -243,8 +252,8 @@ struct Draw {
     // (core/include/BoundaryCondition.hpp:279-384)
     DEV void bc_velocity() {
         for (int b = tid; b < nb; b += BS) {
-            const int f = nif + b, pi = M.bpatch[b], o = fon[f].x;
-            switch (M.u_type[pi]) {
+            const int f = nif + b, o = fon[f].x, fl = bflag[b];
+            switch (bc_type(fl, 0)) {
             case BC_WALL_NOSLIP:
                 Bd(b, B_Ux) = 0.0;
                 Bd(b, B_Uy) = 0.0;
@@ -252,8 +261,8 @@ struct Draw {
             case BC_WALL_MOVING:
             case BC_INLET:
             case BC_DIRICHLET:
-                Bd(b, B_Ux) = M.u_valx[pi];
-                Bd(b, B_Uy) = M.u_valy[pi];
+                Bd(b, B_Ux) = Bd(b, B_uvx);
+                Bd(b, B_Uy) = Bd(b, B_uvy);
                 break;
             case BC_SYMMETRY: {
                 const double nx = Fc(f, W_nx), ny = Fc(f, W_ny);
@@ -270,28 +279,25 @@ struct Draw {
     }
     DEV void bc_pressure() {
         for (int b = tid; b < nb; b += BS) {
-            const int f = nif + b, pi = M.bpatch[b];
-            const int t = M.p_type[pi];
-            Bd(b, B_p) = (t == BC_OUTLET || t == BC_DIRICHLET) ? M.p_val[pi] : C(fon[f].x, V_p);
+            const int f = nif + b, t = bc_type(bflag[b], 1);
+            Bd(b, B_p) = (t == BC_OUTLET || t == BC_DIRICHLET) ? Bd(b, B_pv) : C(fon[f].x, V_p);
         }
     }
     DEV void bc_k() {
         for (int b = tid; b < nb; b += BS) {
-            const int f = nif + b, pi = M.bpatch[b];
-            const int t = M.k_type[pi];
-            Bd(b, B_k) = t == BC_WALL_KOMEGA ? 0.0 : (t == BC_DIRICHLET ? M.k_val[pi] : C(fon[f].x, V_k));
+            const int f = nif + b, t = bc_type(bflag[b], 2);
+            Bd(b, B_k) = t == BC_WALL_KOMEGA ? 0.0 : (t == BC_DIRICHLET ? Bd(b, B_kv) : C(fon[f].x, V_k));
         }
     }
     DEV void bc_omega(double beta1) {
         for (int b = tid; b < nb; b += BS) {
-            const int f = nif + b, pi = M.bpatch[b];
-            const int t = M.w_type[pi];
+            const int f = nif + b, t = bc_type(bflag[b], 3);
             double v;
             if (t == BC_WALL_KOMEGA) {
                 const double y1 = Fc(f, W_dlt);
                 v = 60.0 * S.nu / (beta1 * y1 * y1);
             } else if (t == BC_DIRICHLET) {
-                v = M.w_val[pi];
+                v = Bd(b, B_wv);
             } else {
                 v = C(fon[f].x, V_w);
             }
@@ -873,7 +879,248 @@ struct Draw {
         return res;
     }
 
+    // ------------------------------------------- wave-resident Krylov
+    // For meshes of at most 64*CPL cells one wave runs the whole Krylov
+    // solve: lane l owns cells l, l+64, ..., the vectors stay in registers,
+    // only the vector the matrix multiplies goes through the arena (so the
+    // neighbours' values can be gathered), and dot products are single-wave
+    // DPP reductions with no block barrier. Each per-cell operation is the one
+    // of the block-wide solvers above, in the same order.
+    DEV static void wave_sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    template <int CPL>
+    struct Rows {
+        double dg[CPL], of[CPL][kSlots];
+        int nbr[CPL][kSlots];
+        bool act[CPL];
+    };
+
+    template <int CPL>
+    DEV void load_rows(Rows<CPL> &R) const {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int ci = tid + 64 * j;
+            R.act[j] = ci < nc;
+            const v4i nb4 = R.act[j] ? slot4[2 * ci + 1] : v4i{-1, -1, -1, -1};
+            R.nbr[j][0] = nb4.x;
+            R.nbr[j][1] = nb4.y;
+            R.nbr[j][2] = nb4.z;
+            R.nbr[j][3] = nb4.w;
+            R.dg[j] = R.act[j] ? C(ci, V_diag) : 0.0;
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) R.of[j][s] = R.nbr[j][s] >= 0 ? off(ci, s) : 0.0;
+        }
+    }
+
+    // y = A x for the lane's rows; xs holds the lane's own entries of x and
+    // the arena variable xv all of x (LinearSystem::matvec order)
+    template <int CPL>
+    DEV void spmv_wave(const Rows<CPL> &R, const double (&xs)[CPL], int xv, double (&y)[CPL]) const {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            double a = R.dg[j] * xs[j];
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s)
+                if (R.nbr[j][s] >= 0) a += R.of[j][s] * C(R.nbr[j][s], xv);
+            y[j] = a;
+        }
+    }
+
+    template <int CPL>
+    DEV void publish(const Rows<CPL> &R, const double (&xs)[CPL], int xv) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            if (R.act[j]) C(tid + 64 * j, xv) = xs[j];
+        wave_sync();
+    }
+
+    template <int CPL>
+    DEV void store_x(const Rows<CPL> &R, const double (&x)[CPL]) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            if (R.act[j]) C(tid + 64 * j, V_kx) = x[j];
+    }
+
+    // BiCGSTABSolver::solve (LinearSolver.hpp:259-343), wave 0 only
+    template <int CPL>
+    DEV Krylov bicgstab_wave(double rr, int maxIter, double tol) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        Rows<CPL> R;
+        load_rows(R);
+        double r[CPL], rh[CPL], p[CPL], v[CPL], x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int ci = tid + 64 * j;
+            r[j] = R.act[j] ? C(ci, V_kr) : 0.0;
+            x[j] = R.act[j] ? C(ci, V_kx) : 0.0;
+            rh[j] = r[j];
+            p[j] = 0.0;
+            v[j] = 0.0;
+        }
+        double rho = 1, alpha = 1, omega = 1;
+        double rhoN = rr;
+        for (int it = 0; it < maxIter; ++it) {
+            if (fabs(rhoN) < 1e-30) break;
+            const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
+            double ph[CPL];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                p[j] = r[j] + beta * (p[j] - omega * v[j]);
+                ph[j] = jac(p[j], R.dg[j]);
+            }
+            publish(R, ph, V_kph);
+            spmv_wave(R, ph, V_kph, v);
+            double a1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) a1 += rh[j] * v[j];
+            const double rv = wave_sum(a1);
+            if (fabs(rv) < 1e-30) break;
+            alpha = rhoN / rv;
+            double s[CPL], sh[CPL], a2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                s[j] = r[j] - alpha * v[j];
+                sh[j] = jac(s[j], R.dg[j]);
+                a2 += s[j] * s[j];
+            }
+            const double sn = sqrt(wave_sum(a2));
+            if (sn / r0 < tol) {
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) x[j] += alpha * ph[j];
+                store_x(R, x);
+                res.iterations = it + 1;
+                res.converged = 1;
+                return res;
+            }
+            publish(R, sh, V_ksh);
+            double t[CPL];
+            spmv_wave(R, sh, V_ksh, t);
+            double tt = 0.0, ts = 0.0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                tt += t[j] * t[j];
+                ts += t[j] * s[j];
+            }
+            tt = wave_sum(tt);
+            ts = wave_sum(ts);
+            omega = (tt > 1e-30) ? ts / tt : 0.0;
+            double rr2 = 0.0, rhr = 0.0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                x[j] += alpha * ph[j] + omega * sh[j];
+                r[j] = s[j] - omega * t[j];
+                rr2 += r[j] * r[j];
+                rhr += rh[j] * r[j];
+            }
+            const double rn = sqrt(wave_sum(rr2));
+            rhr = wave_sum(rhr);
+            res.iterations = it + 1;
+            if (rn / r0 < tol) {
+                store_x(R, x);
+                res.converged = 1;
+                return res;
+            }
+            if (fabs(omega) < 1e-30) break;
+            rho = rhoN;
+            rhoN = rhr;
+        }
+        store_x(R, x);
+        return res;
+    }
+
+    // PCGSolver::solve (LinearSolver.hpp:191-244) from x = 0, wave 0 only
+    template <int CPL>
+    DEV Krylov pcg_wave(double rr, int maxIter, double tol) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        Rows<CPL> R;
+        load_rows(R);
+        double r[CPL], z[CPL], p[CPL], x[CPL], a0 = 0.0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int ci = tid + 64 * j;
+            r[j] = R.act[j] ? C(ci, V_kr) : 0.0;
+            x[j] = R.act[j] ? C(ci, V_kx) : 0.0;
+            z[j] = jac(r[j], R.dg[j]);
+            p[j] = z[j];
+            a0 += r[j] * z[j];
+        }
+        double rz = wave_sum(a0);
+        for (int it = 0; it < maxIter; ++it) {
+            publish(R, p, V_kp);
+            double ap[CPL];
+            spmv_wave(R, p, V_kp, ap);
+            double a1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) a1 += p[j] * ap[j];
+            const double pAp = wave_sum(a1);
+            if (fabs(pAp) < 1e-30) break;
+            const double alpha = rz / pAp;
+            double rr2 = 0.0, rzn = 0.0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                x[j] += alpha * p[j];
+                r[j] = r[j] - alpha * ap[j];
+                z[j] = jac(r[j], R.dg[j]);
+                rr2 += r[j] * r[j];
+                rzn += r[j] * z[j];
+            }
+            const double rn = sqrt(wave_sum(rr2));
+            const double rzNew = wave_sum(rzn);
+            res.iterations = it + 1;
+            if (rn / r0 < tol) {
+                store_x(R, x);
+                res.converged = 1;
+                return res;
+            }
+            const double beta = rzNew / (rz + 1e-30);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) p[j] = z[j] + beta * p[j];
+            rz = rzNew;
+        }
+        store_x(R, x);
+        return res;
+    }
+
+    template <int CPL>
+    DEV Krylov solve_wave(int kind, double rr, bool zeroStart) {
+        if (kind == SOLVER_PCG && zeroStart)
+            return pcg_wave<CPL>(rr, S.inner_iterations, S.inner_tolerance);
+        return bicgstab_wave<CPL>(rr, S.inner_iterations, S.inner_tolerance);
+    }
+
+    // Every thread calls; the result is the same in every thread.
+    // The wave-resident path is compiled into the launch shapes that small
+    // meshes use (up to 256 threads).
     DEV Krylov solve(int kind, double rr, bool zeroStart) {
+        if constexpr (BS <= 256) {
+            if (nc <= 256) {
+                if (tid < 64) {
+                    Krylov r;
+                    if (nc <= 64) r = solve_wave<1>(kind, rr, zeroStart);
+                    else if (nc <= 128) r = solve_wave<2>(kind, rr, zeroStart);
+                    else r = solve_wave<4>(kind, rr, zeroStart);
+                    if (tid == 0) *kres = r;
+                }
+                __syncthreads();
+                return *kres;
+            }
+        }
         if (kind == SOLVER_PCG && zeroStart) return pcg(rr, S.inner_iterations, S.inner_tolerance);
         return bicgstab(rr, S.inner_iterations, S.inner_tolerance);
     }
@@ -886,16 +1133,20 @@ struct Draw {
     DEV D *ring(int e) const { return ringb + e * 2 * S.convergence_window; }
     DEV void ce_push(int e, double d) {
         const int m2 = 2 * S.convergence_window;
-        ring(e)[bk->nEst[e] % m2] = d;
+        const int p = bk->nEst[e] == 0 ? 0 : (bk->pos[e] + 1 == m2 ? 0 : bk->pos[e] + 1);
+        ring(e)[p] = d;
+        bk->pos[e] = p;
         bk->nEst[e]++;
     }
-    // wave 0 only: maxima of the newest and the previous window
+    DEV double ce_last(int e) const { return bk->nEst[e] == 0 ? INFINITY : ring(e)[bk->pos[e]]; }
+    // wave 0 only: maxima of the newest and the previous window (a full ring)
     DEV void ce_windows(int e, double &dNew, double &dOld) const {
-        const long long n = bk->nEst[e];
+        const int p = bk->pos[e];
         const int m = S.convergence_window, m2 = 2 * m;
         double nw = 0.0, ol = 0.0;
         for (int j = tid; j < m2; j += 64) {
-            const long long age = ((n - 1 - j) % m2 + m2) % m2;
+            int age = p - j;
+            age = age < 0 ? age + m2 : age;
             double v = ring(e)[j];
             v = (v != v) ? 0.0 : v;
             if (age < m) nw = smax(nw, v);
@@ -904,14 +1155,14 @@ struct Draw {
         dNew = wave_max(nw);
         dOld = wave_max(ol);
     }
-    DEV double ce_rate(long long n, double dNew, double dOld) const {
-        const long long m = S.convergence_window;
+    DEV double ce_rate(int n, double dNew, double dOld) const {
+        const int m = S.convergence_window;
         if (n < 2 * m) return NAN;
         if (dOld <= 0.0) return dNew > 0.0 ? INFINITY : 0.0;
         return pow(dNew / dOld, 1.0 / (double)m);
     }
-    DEV double ce_remaining(long long n, double dNew, double rate, double negligible) const {
-        const long long m = S.convergence_window;
+    DEV double ce_remaining(int n, double dNew, double rate, double negligible) const {
+        const int m = S.convergence_window;
         if (n < 2 * m) return INFINITY;
         if (dNew <= negligible) return dNew;
         if (!(rate < 1.0)) return INFINITY;
@@ -921,35 +1172,39 @@ struct Draw {
     // ------------------------------------------------------- observation
     // PreparedObservation::evaluate (ObservationOperator.hpp:678-772)
     DEV double observe(int i) const {
-        const int type = M.ob_type[i];
+        const D *R = Ob + __mul24(i, kNO);
+        const typename Space<kLDS>::I *Q = obi + 8 * i;
+        const int type = Q[0] & 15;
         if (type == OBS_VELOCITY || type == OBS_PRESSURE) {
             const bool pressure = type == OBS_PRESSURE;
-            const int comp = M.ob_comp[i];
+            const int comp = Q[0] >> 4;
             const bool zero = !pressure && comp == 2;
             const int cell = pressure ? V_p : (comp == 0 ? V_Ux : V_Uy);
             const int bnd = pressure ? B_p : (comp == 0 ? B_Ux : B_Uy);
-            const double center = zero ? 0.0 : C(M.ob_cell[i], cell);
+            const double center = zero ? 0.0 : C(Q[1], cell);
             double bx = 0, by = 0;
-            for (int qq = M.ob_nb_ptr[i]; qq < M.ob_nb_ptr[i + 1]; ++qq) {
-                const double nbv = zero ? 0.0
-                                        : (M.ob_nb_cell[qq] < 0 ? Bd(M.ob_nb_face[qq] - nif, bnd)
-                                                                : C(M.ob_nb_cell[qq], cell));
+            for (int qq = Q[2]; qq < Q[3]; ++qq) {
+                const int nbc = onbi[2 * qq], nbf = onbi[2 * qq + 1];
+                const double ndx = Onb[3 * qq], ndy = Onb[3 * qq + 1], nw = Onb[3 * qq + 2];
+                const double nbv = zero ? 0.0 : (nbc < 0 ? Bd(nbf - nif, bnd) : C(nbc, cell));
                 const double difference = nbv - center;
-                bx += M.ob_nb_w[qq] * M.ob_nb_dx[qq] * difference;
-                by += M.ob_nb_w[qq] * M.ob_nb_dy[qq] * difference;
+                bx += nw * ndx * difference;
+                by += nw * ndy * difference;
             }
             double gxv = 0, gyv = 0;
-            if (fabs(M.ob_det[i]) > 1e-14 * (M.ob_axx[i] * M.ob_ayy[i] + 1e-300)) {
-                gxv = (M.ob_ayy[i] * bx - M.ob_axy[i] * by) / M.ob_det[i];
-                gyv = (M.ob_axx[i] * by - M.ob_axy[i] * bx) / M.ob_det[i];
+            const double det = R[O_det], axx = R[O_axx], axy = R[O_axy], ayy = R[O_ayy];
+            if (fabs(det) > 1e-14 * (axx * ayy + 1e-300)) {
+                gxv = (ayy * bx - axy * by) / det;
+                gyv = (axx * by - axy * bx) / det;
             }
-            const double value = center + gxv * M.ob_offx[i] + gyv * M.ob_offy[i];
-            const double dynP = .5 * M.ob_refrho[i] * M.ob_refv[i] * M.ob_refv[i];
-            return pressure ? (value - M.ob_refp[i]) / smax(dynP, 1e-20) : value;
+            const double value = center + gxv * R[O_offx] + gyv * R[O_offy];
+            const double refv = R[O_refv];
+            const double dynP = .5 * R[O_refrho] * refv * refv;
+            return pressure ? (value - R[O_refp]) / smax(dynP, 1e-20) : value;
         }
         double value = 0.0;
         for (int pass = 0; pass < 2; ++pass) {
-            const int f = pass == 0 ? M.ob_fa[i] : M.ob_fb[i];
+            const int f = pass == 0 ? Q[4] : Q[5];
             if (f < 0) break;
             const int ow = fon[f].x;
             const double delta = Fc(f, W_dlt);
@@ -957,9 +1212,10 @@ struct Draw {
             const double un = C(ow, V_Ux) * nx + C(ow, V_Uy) * ny;
             const double utx = C(ow, V_Ux) - nx * un, uty = C(ow, V_Uy) - ny * un;
             const double tau = S.nu * sqrt(utx * utx + uty * uty) / delta;
-            const double dynP = 0.5 * M.ob_refv[i] * M.ob_refv[i];
+            const double refv = R[O_refv];
+            const double dynP = 0.5 * refv * refv;
             const double cf = tau / smax(dynP, 1e-20);
-            value = pass == 0 ? cf : (1.0 - M.ob_t[i]) * value + M.ob_t[i] * cf;
+            value = pass == 0 ? cf : (1.0 - R[O_t]) * value + R[O_t] * cf;
         }
         return value;
     }
@@ -1016,7 +1272,7 @@ struct Draw {
             bk->pfRef = 0.0;
             bk->worst = 0.0;
             bk->rate = 0.0;
-            for (int e = 0; e < 4; ++e) bk->nEst[e] = 0;
+            for (int e = 0; e < 4; ++e) bk->nEst[e] = bk->pos[e] = 0;
             for (int e = 0; e < 5; ++e) bk->kry[e] = 0;
             for (int e = 0; e < kPhases; ++e) bk->cyc[e] = 0;
             bk->tlast = clock64();
@@ -1361,9 +1617,9 @@ struct Draw {
             red.max(mu);
             mark(12);
             const bool monitor =
-                M.nobs > 0 && iter % (S.monitor_interval > 1 ? S.monitor_interval : 1) == 0;
+                nobs > 0 && iter % (S.monitor_interval > 1 ? S.monitor_interval : 1) == 0;
             if (monitor) {
-                for (int i = tid; i < M.nobs; i += BS) q[i] = observe(i);
+                for (int i = tid; i < nobs; i += BS) q[i] = observe(i);
                 __syncthreads();
             }
             if (tid == 0) {
@@ -1401,13 +1657,12 @@ struct Draw {
             if (lane0) ce_push(0, changeU);
             bool qbad = false;
             if (monitor) {
-                const int nobs = M.nobs;
-                const bool have = bk->haveQPrev;
+                                const bool have = bk->haveQPrev;
                 double bad = 0.0, dq = 0.0;
                 for (int i = tid; i < nobs; i += 64) {
                     const double qi = q[i];
                     if (!isfinite(qi)) bad = 1.0;
-                    const double den = smax(smax(fabs(qi), M.ob_floor[i]), 1e-300);
+                    const double den = smax(smax(fabs(qi), Ob[__mul24(i, kNO) + O_floor]), 1e-300);
                     const double x = fabs(qi - qprev[i]) / den;
                     if (have) dq = smax(dq, (x != x) ? 0.0 : x);
                 }
@@ -1430,32 +1685,15 @@ struct Draw {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
             if (!qbad) {
-                double dNew[4], dOld[4];
-                long long n[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    ce_windows(e, dNew[e], dOld[e]);
-                    n[e] = bk->nEst[e];
-                }
-                double rate[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) rate[e] = ce_rate(n[e], dNew[e], dOld[e]);
-                const double remainingU = ce_remaining(n[0], dNew[0], rate[0], negligible);
+                // The window maxima and rates only decide anything when every
+                // last-sample term is already below tol (the worst measure is
+                // their maximum, or non-finite), and they are reported only at
+                // the final iteration: otherwise the full measure is skipped.
                 const double lastK = bk->lastK, lastOm = bk->lastOm;
-                double worst = smax(smax(smax(momentumScaled, contScaled), changeU), remainingU);
-                if (turbActive) {
-                    worst = smax(worst, lastK);
-                    worst = smax(worst, ce_remaining(n[1], dNew[1], rate[1], negligible));
-                    worst = smax(worst, lastOm);
-                    worst = smax(worst, ce_remaining(n[2], dNew[2], rate[2], negligible));
-                }
-                if (M.nobs > 0) {
-                    const int m2 = 2 * S.convergence_window;
-                    const double lastQ = n[3] == 0 ? INFINITY : ring(3)[(n[3] - 1) % m2];
-                    worst = smax(worst, lastQ);
-                    worst = smax(worst, ce_remaining(n[3], dNew[3], rate[3], negligible));
-                }
-                if (!isfinite(worst)) worst = INFINITY;
+                const double lastQ = nobs > 0 ? ce_last(3) : 0.0;
+                double cheap = smax(smax(momentumScaled, contScaled), changeU);
+                if (turbActive) cheap = smax(smax(cheap, lastK), lastOm);
+                if (nobs > 0) cheap = smax(cheap, lastQ);
                 double normUx0 = bk->normUx0, normUy0 = bk->normUy0, normP0 = bk->normP0;
                 double pfRef = bk->pfRef;
                 if (iter == 0) {
@@ -1475,11 +1713,41 @@ struct Draw {
                 double maxRes = smax(smax(nUx, nUy), nP);
                 const double eK = turbActive ? lastK : 0.0, eOm = turbActive ? lastOm : 0.0;
                 if (turbActive) maxRes = smax(smax(maxRes, lastK), lastOm);
-                if (!isfinite(eUx) || !isfinite(eUy) || !isfinite(eP) || !isfinite(eK) ||
+                const bool nonfinite =
+                    !isfinite(eUx) || !isfinite(eUy) || !isfinite(eP) || !isfinite(eK) ||
                     !isfinite(eOm) || !isfinite(lastK) || !isfinite(lastOm) ||
-                    !isfinite(momentumScaled) || !isfinite(contScaled) || !isfinite(changeU)) {
-                    maxRes = INFINITY;
-                    worst = INFINITY;
+                    !isfinite(momentumScaled) || !isfinite(contScaled) || !isfinite(changeU);
+                if (nonfinite) maxRes = INFINITY;
+                const bool diverging = isnan(maxRes) || isinf(maxRes) || maxRes > S.divergence_limit;
+                const bool last = iter + 1 >= S.max_iterations;
+                double worst = INFINITY, rate0 = bk->rate;
+                if (!(cheap >= tol) || diverging || last) {
+                    double dNew[4], dOld[4];
+                    int n[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        n[e] = bk->nEst[e];
+                        if (n[e] >= 2 * S.convergence_window) ce_windows(e, dNew[e], dOld[e]);
+                        else dNew[e] = dOld[e] = 0.0;
+                    }
+                    rate0 = ce_rate(n[0], dNew[0], dOld[0]);
+                    const double remainingU = ce_remaining(n[0], dNew[0], rate0, negligible);
+                    worst = smax(smax(smax(momentumScaled, contScaled), changeU), remainingU);
+                    if (turbActive) {
+                        worst = smax(worst, lastK);
+                        worst = smax(worst, ce_remaining(n[1], dNew[1],
+                                                         ce_rate(n[1], dNew[1], dOld[1]), negligible));
+                        worst = smax(worst, lastOm);
+                        worst = smax(worst, ce_remaining(n[2], dNew[2],
+                                                         ce_rate(n[2], dNew[2], dOld[2]), negligible));
+                    }
+                    if (nobs > 0) {
+                        worst = smax(worst, lastQ);
+                        worst = smax(worst, ce_remaining(n[3], dNew[3],
+                                                         ce_rate(n[3], dNew[3], dOld[3]), negligible));
+                    }
+                    if (!isfinite(worst)) worst = INFINITY;
+                    if (nonfinite) worst = INFINITY;
                 }
                 const bool turbScheduled = S.turb_start_iter < S.max_iterations;
                 if (lane0) {
@@ -1488,12 +1756,12 @@ struct Draw {
                     bk->normP0 = normP0;
                     bk->pfRef = pfRef;
                     bk->worst = worst;
-                    bk->rate = rate[0];
+                    bk->rate = rate0;
                     if (worst < tol && iter > 0 && (!turbScheduled || (turbActive && bk->turbEst))) {
                         bk->done = 1;
                         bk->status = ST_CONVERGED;
                         bk->finalIter = iter;
-                    } else if (isnan(maxRes) || isinf(maxRes) || maxRes > S.divergence_limit) {
+                    } else if (diverging) {
                         bk->done = 1;
                         bk->status = ST_DIVERGED;
                         bk->finalIter = iter;
@@ -1522,6 +1790,7 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
     __shared__ double redbuf[2 * (BS / 64) * 4];
     __shared__ double coeffs[11];
     __shared__ DevSettings sset;
+    __shared__ Krylov kslot;
     const DevSlot &slot = slots[blockIdx.x];
     const DevMesh &M = *slot.mesh;
     if (threadIdx.x < 11) coeffs[threadIdx.x] = slot.coeffs[threadIdx.x];
@@ -1534,7 +1803,7 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
         for (int64_t i = threadIdx.x; i < n16; i += BS) dst[i] = gsrc[i];
     }
     __syncthreads();
-    Draw<BS, kLDS> d(M, sset, coeffs, arena, &book, redbuf);
+    Draw<BS, kLDS> d(M, sset, coeffs, arena, &book, redbuf, &kslot);
     d.run();
     const int row = slot.row;
     const bool observe = book.status == ST_CONVERGED || book.status == ST_UNCONVERGED;
