@@ -11,6 +11,7 @@
 //   mesh_file <path.msh>                       MSH2 mesh (core/src/Mesh.cpp:817-982)
 //   mesh_periodic_channel nx ny Lx h ReTau yplus
 //   mesh_curved nNodes x0 yb0 x1 yb1 ... yTop ny Re yplus
+//   mesh_channel nx ny Lx Ly                   inlet/outlet channel (makeChannel2D)
 //   save_mesh <path.msh>
 //   nu <value>
 //   bcs_channel Uin kIn omIn
@@ -18,6 +19,7 @@
 //   init Ux Uy Uz p k omega
 //   obs_u x y comp observed sigma
 //   obs_cf patch x y observed sigma refVel
+//   obs_p x y observed sigma refVel refP refRho
 //   params idx...
 //   theta v...                                 one coefficient row (repeatable)
 //   dump <prefix>                              binary field dump per row
@@ -65,6 +67,8 @@ void setKey(SolverSettings &s, const std::string &k, const std::string &v) {
     else if (k == "turbulence_convection") s.turbulenceConvection = v;
     else if (k == "k_min") s.kMin = d();
     else if (k == "omega_min") s.omegaMin = d();
+    else if (k == "k_max") s.kMax = d();
+    else if (k == "sst_variant") s.sstVariant = i();
     else throw std::runtime_error("unknown setting " + k);
 }
 
@@ -150,6 +154,12 @@ int main(int argc, char **argv) {
             ls >> yTop >> ny >> re >> yp;
             mesh = Mesh::makeCurvedChannelPeriodic2D(xs, ys, yTop, ny, re, yp);
             haveMesh = true;
+        } else if (cmd == "mesh_channel") {
+            int nx, ny;
+            double Lx, Ly;
+            ls >> nx >> ny >> Lx >> Ly;
+            mesh = Mesh::makeChannel2D(nx, ny, Lx, Ly);
+            haveMesh = true;
         } else if (cmd == "save_mesh") {
             ls >> savePath;
         } else if (cmd == "nu") {
@@ -173,6 +183,10 @@ int main(int argc, char **argv) {
             double x, y, o, sg, rv;
             ls >> patch >> x >> y >> o >> sg >> rv;
             obs.addSkinFriction(patch, Vec3(x, y, 0.0), o, sg, rv);
+        } else if (cmd == "obs_p") {
+            double x, y, o, sg, rv, rp, rr;
+            ls >> x >> y >> o >> sg >> rv >> rp >> rr;
+            obs.addPressureTap(Vec3(x, y, 0.0), o, sg, rv, 0.0, rp, rr);
         } else if (cmd == "params") {
             int v;
             while (ls >> v) params.push_back(v);

@@ -28,6 +28,36 @@ def small_hill(pressure_solver):
     return cases.hill_problem(nx=30, ny=16, max_iterations=400, pressure_solver=pressure_solver)
 
 
+# velocity probes (x, y, component, observed) and pressure taps (x, y) of the
+# developing inlet/outlet channel
+INOUT_PROBES = [(3.0, 0.5, 0, 1.0), (5.5, 0.1, 0, 1.0), (5.5, 0.02, 0, 1.0), (2.0, 0.3, 1, 0.0)]
+INOUT_TAPS = [(1.0, 0.5), (5.0, 0.5)]
+
+
+def inout_channel(momentum_scheme, turbulence_scheme):
+    """The developing channel of ref_channel_inout.json: the reference's
+    makeChannel2D(24, 12, 6, 1) mesh (MSH2 fixture), inlet U 1, k 1e-3,
+    omega 5, nu 1e-3, tol 1e-5, PCG pressure."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    mesh = Mesh.load_binary(os.path.join(GOLDEN, "ref_channel_inout_24x12.msh"))
+    mesh.compute_wall_distance()
+    bcs = FlowBoundaryConditions.channel_defaults(mesh, 1.0, 0.001, 5.0)
+    s = SolverSettings(max_iterations=6000, convergence_tol=1e-5, pressure_solver="PCG",
+                       alpha_u=0.7, alpha_p=0.3, momentum_convection=momentum_scheme,
+                       turbulence_convection=turbulence_scheme)
+    obs = ObservationOperator()
+    for x, y, comp, observed in INOUT_PROBES:
+        obs.add_velocity_profile((x, y), comp, observed, 0.1)
+    for x, y in INOUT_TAPS:
+        obs.add_pressure_tap((x, y), 0.0, 0.1, 1.0, 0.0, 0.0, 1.0)
+    obs.add_skin_friction("bottom", (4.0, 0.0), 0.005, 0.001, 1.0)
+    return cases.Problem(mesh=mesh, bcs=bcs, settings=s, obs=obs,
+                         params=InferenceParameterSet.a1_betaStar(), nu=0.001,
+                         u_init=Vec3(1.0, 0.0, 0.0), p_init=0.0, k_init=0.001, omega_init=5.0)
+
+
 def max_rel(a, b):
     """Largest relative difference, measured against b (0 when both are empty)."""
     a = np.asarray(a, float)

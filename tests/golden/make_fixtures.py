@@ -27,6 +27,9 @@ sys.path.insert(0, os.path.join(ROOT, "bayesian-rans-uq_amd"))
 from cfdr_amd import cases  # noqa: E402
 from cfdr_amd.mesh import Mesh, hill_bottom  # noqa: E402
 
+sys.path.insert(0, os.path.dirname(HERE))
+from fixtures import INOUT_PROBES, INOUT_TAPS  # noqa: E402
+
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 RE_TAU = cases.RE_TAU
 
@@ -72,14 +75,23 @@ def settings_lines(s):
             f"set momentum_solver {s.momentum_solver}", f"set turbulence_solver {s.turbulence_solver}"]
 
 
-def problem_lines(prob, mesh_path, thetas, bc=(1.0, 1.0, 10.0), dump=None):
+def problem_lines(prob, mesh_path, thetas, bc=(1.0, 1.0, 10.0), dump=None, extra=()):
     L = [f"mesh_file {mesh_path}", f"nu {prob.nu!r}", "bcs_channel %r %r %r" % bc]
     L += settings_lines(prob.settings)
+    L += list(extra)
     L.append("init %r %r 0 %r %r %r" % (prob.u_init.x, prob.u_init.y, prob.p_init, prob.k_init,
                                         prob.omega_init))
     for ob in prob.obs.observables:
-        L.append("obs_u %r %r %d %r %r" % (ob.location[0], ob.location[1], ob.component,
-                                           ob.observed, ob.sigma_obs))
+        if ob.type == 0:
+            L.append("obs_u %r %r %d %r %r" % (ob.location[0], ob.location[1], ob.component,
+                                               ob.observed, ob.sigma_obs))
+        elif ob.type == 1:
+            L.append("obs_p %r %r %r %r %r %r %r" % (ob.location[0], ob.location[1], ob.observed,
+                                                     ob.sigma_obs, ob.ref_velocity,
+                                                     ob.ref_pressure, ob.ref_density))
+        else:
+            L.append("obs_cf %s %r %r %r %r %r" % (ob.patch_name, ob.location[0], ob.location[1],
+                                                   ob.observed, ob.sigma_obs, ob.ref_velocity))
     L.append("params " + " ".join(str(i) for i in prob.params.active_indices))
     for th in thetas:
         L.append("theta " + " ".join(repr(float(v)) for v in th))
@@ -187,6 +199,51 @@ def main():
         write(f"ref_hill_small_{ps.lower()}.json",
               {"case": f"hill_problem(nx=30, ny=16, max_iterations=400, pressure_solver='{ps}')",
                "theta": [0.31, 0.09, 0.85, 0.5], "row": row})
+
+    # 6. closure variants on the small channel, with a skin-friction
+    #    observation next to the velocity probes
+    var = {}
+    for name, variant in (("sst_no_limiter", 1), ("k_omega", 2)):
+        vp = cases.channel_problem(ny=32, ratio=1.15, n_probes=10, tol=1e-6, max_iterations=40000)
+        vp.obs.add_skin_friction("bottom", (0.5, 0.0), 0.005, 0.001, 15.0)
+        vp.settings.sst_variant = variant
+        row = run_case(problem_lines(vp, os.path.join(tmp, "ch32.msh"), [[0.31, 0.09]],
+                                     extra=[f"set sst_variant {variant}"]))["rows"][0]
+        var[name] = {"sst_variant": variant, "row": row}
+    write("ref_channel_variants.json", {"case": "channel_problem(ny=32, ratio=1.15, n_probes=10, "
+                                                "tol=1e-6, max_iterations=40000) + skin friction "
+                                                "on 'bottom' at x=0.5 (cf 0.005, sigma 0.001, "
+                                                "U_ref 15)", "variants": var})
+
+    # 7. developing channel with inlet and outlet (the reference's makeChannel2D
+    #    mesh, kept as an MSH2 fixture), pressure taps and skin friction, with
+    #    each convection scheme (streamwise-invariant channels do not exercise them)
+    io_mesh = os.path.join(HERE, "ref_channel_inout_24x12.msh")
+    base = ["nu 0.001", "bcs_channel 1.0 0.001 5.0", "set max_iterations 6000",
+            "set convergence_tol 1e-5", "set pressure_solver PCG", "set alpha_u 0.7",
+            "set alpha_p 0.3", "init 1 0 0 0 0.001 5.0"]
+    base += ["obs_u %r %r %d %r 0.1" % o for o in INOUT_PROBES]
+    base += ["obs_p %r %r 0.0 0.1 1.0 0.0 1.0" % o for o in INOUT_TAPS]
+    base += ["obs_cf bottom 4.0 0.0 0.005 0.001 1.0", "params 9 8"]
+    out = {"case": "makeChannel2D(24, 12, 6, 1), nu 1e-3, inlet U 1, k 1e-3, omega 5, "
+                   "tol 1e-5, PCG pressure, alpha_u 0.7, alpha_p 0.3",
+           "mesh": os.path.basename(io_mesh), "schemes": {}}
+    # Linear upwind in the k and omega equations overshoots into the k clip on
+    # this mesh, which makes the iteration depend discontinuously on round-off
+    # (a 1e-14 change of the initial state moves the probes by 1e-5): its case
+    # pairs linear-upwind momentum with upwind turbulence transport instead.
+    for name, mom, turb in (("limitedLinearUpwind", "limitedLinearUpwind", "limitedLinearUpwind"),
+                            ("linearUpwind", "linearUpwind", "upwind"),
+                            ("upwind", "upwind", "upwind")):
+        lines = ["mesh_channel 24 12 6.0 1.0", f"save_mesh {io_mesh}"] + base
+        lines += [f"set momentum_convection {mom}", f"set turbulence_convection {turb}",
+                  "theta 0.31 0.09", "theta 0.33 0.095", f"dump {os.path.join(tmp, 'io')}"]
+        res = run_case(lines)
+        res["rows"][0]["fields"] = read_dump(os.path.join(tmp, "io_0.bin"))
+        out["schemes"][name] = {"thetas": [[0.31, 0.09], [0.33, 0.095]],
+                                "momentum_convection": mom, "turbulence_convection": turb,
+                                "rows": res["rows"]}
+    write("ref_channel_inout.json", out)
 
 
 if __name__ == "__main__":

@@ -20,7 +20,8 @@ import numpy as np
 import pytest
 
 from cfdr_amd import cases, engine
-from fixtures import FIELD_NAMES, load, max_rel, max_scaled, small_channel, small_hill
+from fixtures import (FIELD_NAMES, inout_channel, load, max_rel, max_scaled, small_channel,
+                      small_hill)
 
 pytestmark = pytest.mark.gpu
 
@@ -162,6 +163,30 @@ def test_small_hill_matches_reference(solver):
     f = _fields(res, 0, prob.mesh.n_cells)
     for name in ("Ux", "Uy", "p", "k", "omega", "nuT", "flux"):
         assert max_scaled(f[name], d["row"]["fields"][name]) <= FIELD_TOL, name
+
+
+@pytest.mark.parametrize("name", ["sst_no_limiter", "k_omega"])
+def test_closure_variants_match_reference(name):
+    v = load("ref_channel_variants.json")["variants"][name]
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=10, tol=1e-6, max_iterations=40000)
+    prob.obs.add_skin_friction("bottom", (0.5, 0.0), 0.005, 0.001, 15.0)
+    prob.settings.sst_variant = v["sst_variant"]
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    _check_row(res, 0, v["row"])
+
+
+@pytest.mark.parametrize("scheme", ["limitedLinearUpwind", "linearUpwind", "upwind"])
+def test_inlet_outlet_channel_matches_reference(scheme):
+    """Inlet/outlet patches, pressure taps, skin friction, each scheme; the
+    288-cell mesh keeps its arena in HBM."""
+    d = load("ref_channel_inout.json")["schemes"][scheme]
+    prob = inout_channel(d["momentum_convection"], d["turbulence_convection"])
+    res = _batch(prob).evaluate(np.asarray(d["thetas"]), fields=True)
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+    f = _fields(res, 0, prob.mesh.n_cells)
+    for name in FIELD_NAMES + ["flux"]:
+        assert max_scaled(f[name], d["rows"][0]["fields"][name]) <= 1e-6, name
 
 
 def test_batch_results_independent_of_batch_composition():

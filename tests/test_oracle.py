@@ -28,7 +28,7 @@ import oracle_py
 from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
-from fixtures import FIELD_NAMES, load, max_rel, small_channel, small_hill
+from fixtures import FIELD_NAMES, inout_channel, load, max_rel, small_channel, small_hill
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -310,6 +310,31 @@ def test_config1_channel_rows():
     P = _oracle(prob)
     for theta, row in list(zip(d["thetas"], d["rows"]))[:2]:
         _check_row(P.evaluate(prob.coefficients(theta)), row)
+
+
+@pytest.mark.parametrize("name", ["sst_no_limiter", "k_omega"])
+def test_closure_variants(name):
+    """SST NoLimiter and k-omega variants, with a skin-friction observation."""
+    v = load("ref_channel_variants.json")["variants"][name]
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=10, tol=1e-6, max_iterations=40000)
+    prob.obs.add_skin_friction("bottom", (0.5, 0.0), 0.005, 0.001, 15.0)
+    prob.settings.sst_variant = v["sst_variant"]
+    _check_row(_oracle(prob).evaluate(prob.coefficients([0.31, 0.09])), v["row"])
+
+
+@pytest.mark.parametrize("scheme", ["limitedLinearUpwind", "linearUpwind", "upwind"])
+def test_inlet_outlet_channel(scheme):
+    """Developing channel with inlet/outlet patches (outlet pressure, no p'
+    pin), pressure taps and skin friction, for each convection scheme."""
+    d = load("ref_channel_inout.json")["schemes"][scheme]
+    prob = inout_channel(d["momentum_convection"], d["turbulence_convection"])
+    P = _oracle(prob)
+    for i, (theta, row) in enumerate(zip(d["thetas"], d["rows"])):
+        r = P.evaluate(prob.coefficients(theta), fields=i == 0)
+        _check_row(r, row)
+        if i == 0:
+            for name in FIELD_NAMES + ["flux"]:
+                assert np.array_equal(r[name], np.asarray(row["fields"][name])), name
 
 
 def test_channel_draws_fixture():
