@@ -17,6 +17,11 @@ run in one persistent-kernel launch. The line printed is
                of that sample (statuses must match)
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+
+Profiling helpers: --max-iterations caps every solve (a short, fixed amount of
+work for counter runs); --copy-calibration MIB runs only a device-to-device
+copy of that size, whose FETCH_SIZE/WRITE_SIZE readings scale the counters of
+the solve kernel to bytes.
 """
 from __future__ import annotations
 
@@ -139,6 +144,27 @@ def cpu_baseline(prob, theta, cores):
                            f"{dt:.1f} s wall"}
 
 
+def copy_calibration(mib):
+    """Three device-to-device copies of `mib` MiB through the HIP runtime."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), n) != 0:
+            raise SystemExit("bench.py: hipMalloc failed")
+    if hip.hipMemset(a, 1, n) != 0:
+        raise SystemExit("bench.py: hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, n, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise SystemExit("bench.py: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": mib << 20, "copies": 3}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,7 +175,11 @@ def main():
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--max-iterations", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(draws are independent: several GPUs would each run their own batch)",
@@ -162,6 +192,8 @@ def main():
     from cfdr_amd import cases, engine
 
     prob = cases.channel_problem()
+    if args.max_iterations:
+        prob.settings.max_iterations = args.max_iterations
     theta = cases.channel_draws(args.draws)
     mesh = prob.mesh
     N, F, B = mesh.n_cells, mesh.n_internal, mesh.n_boundary
@@ -189,6 +221,8 @@ def main():
     last = results[-1]
     converged = int(sum(int(np.sum(r.convergence_mask)) for r in results))
     value = converged / total_s
+    if args.max_iterations:
+        print("# capped solves: value counts converged rows only", file=sys.stderr, flush=True)
     for r in results[1:]:
         if not np.array_equal(r.predictions, last.predictions, equal_nan=True):
             raise SystemExit("bench.py: repeated steps differ")

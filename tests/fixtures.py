@@ -58,6 +58,20 @@ def inout_channel(momentum_scheme, turbulence_scheme):
                          u_init=Vec3(1.0, 0.0, 0.0), p_init=0.0, k_init=0.001, omega_init=5.0)
 
 
+def symmetry_channel():
+    """1x32 geometric channel (ratio 1.15) whose top patch is a symmetry
+    plane, 5000 capped iterations at tol 1e-6."""
+    from cfdr_amd.config import FlowBoundaryConditions
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=12, tol=1e-6, max_iterations=5000)
+    mesh = prob.mesh
+    for p in mesh.patches:
+        if p.name == "top":
+            p.type = "symmetry"
+    mesh.compute_wall_distance()
+    prob.bcs = FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1.0, 10.0)
+    return prob
+
+
 def max_rel(a, b):
     """Largest relative difference, measured against b (0 when both are empty)."""
     a = np.asarray(a, float)

@@ -28,7 +28,7 @@ from cfdr_amd import cases  # noqa: E402
 from cfdr_amd.mesh import Mesh, hill_bottom  # noqa: E402
 
 sys.path.insert(0, os.path.dirname(HERE))
-from fixtures import INOUT_PROBES, INOUT_TAPS  # noqa: E402
+from fixtures import INOUT_PROBES, INOUT_TAPS, symmetry_channel  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 RE_TAU = cases.RE_TAU
@@ -244,6 +244,17 @@ def main():
                                 "momentum_convection": mom, "turbulence_convection": turb,
                                 "rows": res["rows"]}
     write("ref_channel_inout.json", out)
+
+    # 8. symmetry patch: the small channel with its top wall relabelled as a
+    #    symmetry plane (slip top, zero-gradient k and omega)
+    sp = symmetry_channel()
+    sm = os.path.join(tmp, "sym.msh")
+    sp.mesh.save_binary(sm)
+    res = run_case(problem_lines(sp, sm, [[0.31, 0.09]], dump=os.path.join(tmp, "sym")))
+    row = res["rows"][0]
+    row["fields"] = read_dump(os.path.join(tmp, "sym_0.bin"))
+    write("ref_channel_symmetry.json", {"case": "symmetry_channel() of tests/fixtures.py",
+                                        "theta": [0.31, 0.09], "row": row})
 
 
 if __name__ == "__main__":

@@ -21,7 +21,7 @@ import pytest
 
 from cfdr_amd import cases, engine
 from fixtures import (FIELD_NAMES, inout_channel, load, max_rel, max_scaled, small_channel,
-                      small_hill)
+                      small_hill, symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -187,6 +187,16 @@ def test_inlet_outlet_channel_matches_reference(scheme):
     f = _fields(res, 0, prob.mesh.n_cells)
     for name in FIELD_NAMES + ["flux"]:
         assert max_scaled(f[name], d["rows"][0]["fields"][name]) <= 1e-6, name
+
+
+def test_symmetry_patch_matches_reference():
+    d = load("ref_channel_symmetry.json")
+    prob = symmetry_channel()
+    res = _batch(prob).evaluate(np.asarray([d["theta"]]), fields=True)
+    _check_row(res, 0, d["row"])
+    f = _fields(res, 0, prob.mesh.n_cells)
+    for name in FIELD_NAMES + ["flux"]:
+        assert max_scaled(f[name], d["row"]["fields"][name]) <= FIELD_TOL, name
 
 
 def test_batch_results_independent_of_batch_composition():

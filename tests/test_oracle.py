@@ -28,7 +28,8 @@ import oracle_py
 from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
-from fixtures import FIELD_NAMES, inout_channel, load, max_rel, small_channel, small_hill
+from fixtures import (FIELD_NAMES, inout_channel, load, max_rel, small_channel, small_hill,
+                      symmetry_channel)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -335,6 +336,16 @@ def test_inlet_outlet_channel(scheme):
         if i == 0:
             for name in FIELD_NAMES + ["flux"]:
                 assert np.array_equal(r[name], np.asarray(row["fields"][name])), name
+
+
+def test_symmetry_patch():
+    """A symmetry plane (BoundaryCondition.hpp:279-384 symmetry branches)."""
+    d = load("ref_channel_symmetry.json")
+    prob = symmetry_channel()
+    r = _oracle(prob).evaluate(prob.coefficients(d["theta"]), fields=True)
+    _check_row(r, d["row"])
+    for name in FIELD_NAMES + ["flux"]:
+        assert np.array_equal(r[name], np.asarray(d["row"]["fields"][name])), name
 
 
 def test_channel_draws_fixture():
