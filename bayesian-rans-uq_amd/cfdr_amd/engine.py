@@ -12,9 +12,10 @@ on the GPU in one persistent kernel launch. The engine library must be built
 from __future__ import annotations
 
 import ctypes as C
+import enum
 import math
 import os
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -198,16 +199,32 @@ def obs_struct(P, keep: _Keep):
     return r
 
 
+class EvaluationStatus(enum.IntEnum):
+    """EvaluationStatus (core/include/EvaluationTypes.hpp:6) with the names
+    of the reference's Python binding (rans-uq/python/bindings.cpp:464-469)."""
+
+    Converged = 0
+    Unconverged = 1
+    DivergenceDetected = 2
+    InvalidParameters = 3
+    Unknown = 4
+
+
+assert [s.name for s in EvaluationStatus] == STATUS_NAMES
+
+
 @dataclass
 class EvaluationResult:
-    status: int
+    """EvaluationResult (core/include/EvaluationTypes.hpp:24-29)."""
+
+    status: EvaluationStatus
     log_lik: float
     predictions: np.ndarray
     simple_iters: int
 
     @property
     def status_name(self) -> str:
-        return STATUS_NAMES[self.status]
+        return self.status.name
 
 
 @dataclass
@@ -224,13 +241,15 @@ class ForwardBatchRow:
 @dataclass
 class ForwardBatchResult:
     """ForwardBatchResult with the array views of the reference's binding
-    (rans-uq/python/bindings.cpp:830-887); every array is owned."""
+    (rans-uq/python/bindings.cpp:830-887); every array is owned. `statuses`
+    is a list of EvaluationStatus as in the binding; `status_codes` holds the
+    same values as an int array."""
 
     n_observations: int
     rows: list
     predictions: np.ndarray
     log_likelihoods: np.ndarray
-    statuses: np.ndarray
+    statuses: list
     iteration_counts: np.ndarray
     errors: list
     convergence_mask: np.ndarray
@@ -245,6 +264,10 @@ class ForwardBatchResult:
     @property
     def converged(self) -> np.ndarray:
         return self.convergence_mask
+
+    @property
+    def status_codes(self) -> np.ndarray:
+        return np.asarray([int(s) for s in self.statuses], dtype=np.int32)
 
 
 class ForwardBatch:
@@ -318,12 +341,10 @@ class ForwardBatch:
         theta = np.asarray(theta, dtype=np.float64)
         if theta.ndim == 1 and theta.size == 0:
             theta = theta.reshape(0, self.n_parameters())
-        if theta.ndim != 2:
-            raise ValueError("theta must be a two-dimensional array")
+        if theta.ndim != 2 or theta.shape[1] != self.n_parameters():
+            raise ValueError("theta must have shape (n_rows, n_parameters)")
         if workers < 1:
             raise ValueError("workers must be positive")
-        if theta.shape[1] != self.n_parameters():
-            raise ValueError("batch rows must all have n_parameters entries")
         theta = np.ascontiguousarray(theta)
         n, nobs = theta.shape[0], self.n_obs
         status = np.zeros(n, np.int32)
@@ -365,12 +386,13 @@ class ForwardBatch:
         raw = errs.raw
         errors = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(n)]
         rows = []
+        statuses = [EvaluationStatus(int(v)) for v in status]
         for i in range(n):
-            res = EvaluationResult(int(status[i]), float(loglik[i]), pred[i].copy(), int(iters[i]))
+            res = EvaluationResult(statuses[i], float(loglik[i]), pred[i].copy(), int(iters[i]))
             rows.append(ForwardBatchRow(res, float(worst[i]), float(rate[i]), bool(stalled[i]),
                                         errors[i]))
         timing = self.last_timing()
-        return ForwardBatchResult(nobs, rows, pred, loglik, status, iters, errors,
+        return ForwardBatchResult(nobs, rows, pred, loglik, statuses, iters, errors,
                                   conv.astype(bool), worst, rate, stalled.astype(bool), kry,
                                   fld, timing["kernel_ms"], cyc)
 

@@ -231,7 +231,7 @@ def main():
     launch_bytes = sum(solve_bytes(N, F, B, int(last.iteration_counts[i]),
                                    last.krylov_iterations[i], prob.settings.turb_start_iter,
                                    prob.settings)
-                       for i in range(len(theta)) if last.statuses[i] in (0, 1))
+                       for i in range(len(theta)) if last.status_codes[i] in (0, 1))
     kernel_avg_s = statistics.mean(kernel_ms) / 1e3
     achieved = launch_bytes / kernel_avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
@@ -261,8 +261,8 @@ def main():
         rows, cpu = cpu_baseline(prob, theta, _cores())
         errs = []
         for i, o in enumerate(rows):
-            if o["status"] != last.statuses[i]:
-                raise SystemExit(f"bench.py: draw {i} status {last.statuses[i]} vs oracle "
+            if o["status"] != last.status_codes[i]:
+                raise SystemExit(f"bench.py: draw {i} status {last.status_codes[i]} vs oracle "
                                  f"{o['status']}")
             if o["status"] in (0, 1):
                 p, q = last.predictions[i], o["predictions"]
@@ -287,7 +287,8 @@ def main():
                                "tol 1e-8, PCG pressure / BiCGSTAB transport, 30 probes",
                    "draws": int(len(theta)), "cells": N, "path": "batched persistent kernel"},
         "converged": converged,
-        "statuses": {str(s): int(np.sum(last.statuses == s)) for s in np.unique(last.statuses)},
+        "statuses": {str(s): int(np.sum(last.status_codes == s))
+                     for s in np.unique(last.status_codes)},
         "iterations_mean": float(np.mean(last.iteration_counts)),
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 1),
                        "p99": round(float(np.percentile(step_ms, 99)), 1),

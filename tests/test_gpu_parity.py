@@ -223,7 +223,7 @@ def test_block_sizes_agree():
     base = _batch(prob, block_size=64).evaluate(theta)
     for bs in (128, 256):
         r = _batch(prob, block_size=bs).evaluate(theta)
-        assert np.array_equal(r.statuses, base.statuses)
+        assert np.array_equal(r.status_codes, base.status_codes)
         assert max_rel(r.predictions, base.predictions) <= REL_OBS
 
 
