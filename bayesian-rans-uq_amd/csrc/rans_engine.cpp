@@ -141,7 +141,8 @@ void validate_settings(const rq_settings &s) {
         throw Fail{RQ_EINVAL, "convergence window exceeds the engine limit of 256"};
     if (s.mixed_precision != 0 && s.mixed_precision != 1)
         throw Fail{RQ_EINVAL, "mixed_precision must be 0 or 1"};
-    if (s.mixed_precision) throw Fail{RQ_EINVAL, "mixed precision is not available in this engine build"};
+    if (s.mixed_precision && s.block_size == 512)
+        throw Fail{RQ_EINVAL, "mixed precision runs with block sizes up to 256"};
     if (s.profile != 0 && s.profile != 1) throw Fail{RQ_EINVAL, "profile must be 0 or 1"};
     if (s.block_size != 0 && s.block_size != 64 && s.block_size != 128 && s.block_size != 256 &&
         s.block_size != 512)
@@ -573,6 +574,8 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.init_w = init[4];
         S.mixed_precision = s.mixed_precision;
         S.profile = s.profile;
+        if (s.mixed_precision && nc > 256)
+            throw Fail{RQ_EINVAL, "mixed precision is available for meshes of up to 256 cells"};
         if (s.block_size) E->block = s.block_size;
         else E->block = nc <= 64 ? 64 : nc <= 128 ? 128 : nc <= 256 ? 256 : 512;
         int maxLds = 0;

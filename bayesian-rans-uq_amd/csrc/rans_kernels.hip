@@ -64,6 +64,21 @@ DEV double wave_max(double v) {
     return lane63(v);
 }
 
+template <int CTRL, int ROW_MASK = 0xf>
+DEV float dpp_movef(float v, float old) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL,
+                                                      ROW_MASK, 0xf, false));
+}
+DEV float wave_sumf(float v) {
+    v += dpp_movef<0xB1>(v, 0.0f);
+    v += dpp_movef<0x4E>(v, 0.0f);
+    v += dpp_movef<0x141>(v, 0.0f);
+    v += dpp_movef<0x140>(v, 0.0f);
+    v += dpp_movef<0x142, 0xa>(v, 0.0f);
+    v += dpp_movef<0x143, 0xc>(v, 0.0f);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 template <int BS>
 struct Reducer {
     static constexpr int NW = BS / 64;
@@ -1097,8 +1112,233 @@ struct Draw {
         return res;
     }
 
+    // -------------------------------------- mixed precision (wave-resident)
+    // FP32 Krylov on the defect equation A d = r0 / |r0| with the FP64
+    // residual r0 = b - A x0 already in kr; x = x0 + |r0| d is formed in FP64.
+    // The matrix, the vectors and the dot products are FP32; the stopping
+    // rule is the solver's own (|r| / |r0| < tol). The breakdown guards act on
+    // the normalised quantities.
+    template <int CPL>
+    struct Rows32 {
+        float dg[CPL], of[CPL][kSlots];
+        int nbr[CPL][kSlots];
+        bool act[CPL];
+    };
+
+    template <int CPL>
+    DEV void load_rows32(Rows32<CPL> &R) const {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int ci = tid + 64 * j;
+            R.act[j] = ci < nc;
+            const v4i nb4 = R.act[j] ? slot4[2 * ci + 1] : v4i{-1, -1, -1, -1};
+            R.nbr[j][0] = nb4.x;
+            R.nbr[j][1] = nb4.y;
+            R.nbr[j][2] = nb4.z;
+            R.nbr[j][3] = nb4.w;
+            R.dg[j] = R.act[j] ? (float)C(ci, V_diag) : 0.0f;
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s)
+                R.of[j][s] = R.nbr[j][s] >= 0 ? (float)off(ci, s) : 0.0f;
+        }
+    }
+
+    template <int CPL>
+    DEV void spmv_wave32(const Rows32<CPL> &R, const float (&xs)[CPL], int xv, float (&y)[CPL]) const {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            float a = R.dg[j] * xs[j];
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s)
+                if (R.nbr[j][s] >= 0) a += R.of[j][s] * (float)C(R.nbr[j][s], xv);
+            y[j] = a;
+        }
+    }
+
+    template <int CPL>
+    DEV void publish32(const Rows32<CPL> &R, const float (&xs)[CPL], int xv) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            if (R.act[j]) C(tid + 64 * j, xv) = (double)xs[j];
+        wave_sync();
+    }
+
+    DEV static float jacf(float r, float d) { return (fabsf(d) > 1e-30f) ? r / d : r; }
+
+    template <int CPL>
+    DEV void apply_defect(const Rows32<CPL> &R, double scale, const float (&d)[CPL]) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+            if (R.act[j]) {
+                const int ci = tid + 64 * j;
+                C(ci, V_kx) = C(ci, V_kx) + scale * (double)d[j];
+            }
+    }
+
+    template <int CPL>
+    DEV Krylov bicgstab_wave32(double rr, int maxIter, double tol64) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        const float tol = (float)tol64;
+        const double inv = 1.0 / r0;
+        Rows32<CPL> R;
+        load_rows32(R);
+        float r[CPL], rh[CPL], p[CPL], v[CPL], x[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int ci = tid + 64 * j;
+            r[j] = R.act[j] ? (float)(C(ci, V_kr) * inv) : 0.0f;
+            x[j] = 0.0f;
+            rh[j] = r[j];
+            p[j] = 0.0f;
+            v[j] = 0.0f;
+        }
+        float rho = 1, alpha = 1, omega = 1;
+        float rhoN = 0.0f;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) rhoN += r[j] * r[j];
+        rhoN = wave_sumf(rhoN);
+        for (int it = 0; it < maxIter; ++it) {
+            if (fabsf(rhoN) < 1e-30f) break;
+            const float beta = (rhoN / (rho + 1e-30f)) * (alpha / (omega + 1e-30f));
+            float ph[CPL];
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                p[j] = r[j] + beta * (p[j] - omega * v[j]);
+                ph[j] = jacf(p[j], R.dg[j]);
+            }
+            publish32(R, ph, V_kph);
+            spmv_wave32(R, ph, V_kph, v);
+            float a1 = 0.0f;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) a1 += rh[j] * v[j];
+            const float rv = wave_sumf(a1);
+            if (fabsf(rv) < 1e-30f) break;
+            alpha = rhoN / rv;
+            float s[CPL], sh[CPL], a2 = 0.0f;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                s[j] = r[j] - alpha * v[j];
+                sh[j] = jacf(s[j], R.dg[j]);
+                a2 += s[j] * s[j];
+            }
+            const float sn = sqrtf(wave_sumf(a2));
+            if (sn < tol) {
+#pragma unroll
+                for (int j = 0; j < CPL; ++j) x[j] += alpha * ph[j];
+                apply_defect(R, r0, x);
+                res.iterations = it + 1;
+                res.converged = 1;
+                return res;
+            }
+            publish32(R, sh, V_ksh);
+            float t[CPL];
+            spmv_wave32(R, sh, V_ksh, t);
+            float tt = 0.0f, ts = 0.0f;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                tt += t[j] * t[j];
+                ts += t[j] * s[j];
+            }
+            tt = wave_sumf(tt);
+            ts = wave_sumf(ts);
+            omega = (tt > 1e-30f) ? ts / tt : 0.0f;
+            float rr2 = 0.0f, rhr = 0.0f;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                x[j] += alpha * ph[j] + omega * sh[j];
+                r[j] = s[j] - omega * t[j];
+                rr2 += r[j] * r[j];
+                rhr += rh[j] * r[j];
+            }
+            const float rn = sqrtf(wave_sumf(rr2));
+            rhr = wave_sumf(rhr);
+            res.iterations = it + 1;
+            if (rn < tol) {
+                apply_defect(R, r0, x);
+                res.converged = 1;
+                return res;
+            }
+            if (fabsf(omega) < 1e-30f) break;
+            rho = rhoN;
+            rhoN = rhr;
+        }
+        apply_defect(R, r0, x);
+        return res;
+    }
+
+    template <int CPL>
+    DEV Krylov pcg_wave32(double rr, int maxIter, double tol64) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        const float tol = (float)tol64;
+        const double inv = 1.0 / r0;
+        Rows32<CPL> R;
+        load_rows32(R);
+        float r[CPL], z[CPL], p[CPL], x[CPL], a0 = 0.0f;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const int ci = tid + 64 * j;
+            r[j] = R.act[j] ? (float)(C(ci, V_kr) * inv) : 0.0f;
+            x[j] = 0.0f;
+            z[j] = jacf(r[j], R.dg[j]);
+            p[j] = z[j];
+            a0 += r[j] * z[j];
+        }
+        float rz = wave_sumf(a0);
+        for (int it = 0; it < maxIter; ++it) {
+            publish32(R, p, V_kp);
+            float ap[CPL];
+            spmv_wave32(R, p, V_kp, ap);
+            float a1 = 0.0f;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) a1 += p[j] * ap[j];
+            const float pAp = wave_sumf(a1);
+            if (fabsf(pAp) < 1e-30f) break;
+            const float alpha = rz / pAp;
+            float rr2 = 0.0f, rzn = 0.0f;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                x[j] += alpha * p[j];
+                r[j] = r[j] - alpha * ap[j];
+                z[j] = jacf(r[j], R.dg[j]);
+                rr2 += r[j] * r[j];
+                rzn += r[j] * z[j];
+            }
+            const float rn = sqrtf(wave_sumf(rr2));
+            const float rzNew = wave_sumf(rzn);
+            res.iterations = it + 1;
+            if (rn < tol) {
+                apply_defect(R, r0, x);
+                res.converged = 1;
+                return res;
+            }
+            const float beta = rzNew / (rz + 1e-30f);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) p[j] = z[j] + beta * p[j];
+            rz = rzNew;
+        }
+        apply_defect(R, r0, x);
+        return res;
+    }
+
     template <int CPL>
     DEV Krylov solve_wave(int kind, double rr, bool zeroStart) {
+        if (S.mixed_precision) {
+            if (kind == SOLVER_PCG && zeroStart)
+                return pcg_wave32<CPL>(rr, S.inner_iterations, S.inner_tolerance);
+            return bicgstab_wave32<CPL>(rr, S.inner_iterations, S.inner_tolerance);
+        }
         if (kind == SOLVER_PCG && zeroStart)
             return pcg_wave<CPL>(rr, S.inner_iterations, S.inner_tolerance);
         return bicgstab_wave<CPL>(rr, S.inner_iterations, S.inner_tolerance);

@@ -175,6 +175,7 @@ def main():
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
@@ -240,6 +241,26 @@ def main():
                 "kernel": "simple_solve_kernel", "kernel_ms": round(1e3 * kernel_avg_s, 3),
                 "algorithmic_bytes_per_launch": int(launch_bytes)}
 
+    # the same sweep with FP32 Krylov iterations and FP64 defect correction
+    mixed = None
+    if not args.no_mixed:
+        mb = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
+                                 prob.settings, prob.u_init, prob.p_init, prob.k_init,
+                                 prob.omega_init, block_size=args.block_size,
+                                 mixed_precision=True)
+        t0 = time.perf_counter()
+        rm = mb.evaluate(theta)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(rm.status_codes, last.status_codes))
+        ok = (last.status_codes == 0) & (rm.status_codes == 0)
+        err_fp64 = float(np.max(np.abs(rm.predictions[ok] - last.predictions[ok]) /
+                                np.abs(last.predictions[ok]))) if ok.any() else None
+        mixed = {"value": round(int(np.sum(rm.convergence_mask)) / dt, 4),
+                 "unit": "converged solves/s", "dtype": "f64 assembly/norms, f32 Krylov",
+                 "kernel_ms": round(rm.kernel_ms, 3), "statuses_match_fp64": same,
+                 "max_rel_err_vs_fp64_gpu": err_fp64}
+        print(f"# mixed precision: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
     plain = None
     if not args.no_plain and args.plain_sample > 0:
         lat = []
@@ -298,6 +319,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "max_rel_err_vs_oracle": err,
+        "mixed": mixed,
         "block_size": batch.last_timing()["block_size"],
         "arena_in_lds": batch.last_timing()["arena_in_lds"],
     }

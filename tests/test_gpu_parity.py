@@ -199,6 +199,28 @@ def test_symmetry_patch_matches_reference():
         assert max_scaled(f[name], d["row"]["fields"][name]) <= FIELD_TOL, name
 
 
+def test_mixed_precision_matches_fp64_reference():
+    """FP32 Krylov iterations with FP64 defect correction: same converged
+    flags as the FP64 reference, probes within REL_OBS of it."""
+    for prob, fixture in ((small_channel(), "ref_channel_small.json"),
+                          (cases.channel_problem(), "ref_channel_c1.json")):
+        d = load(fixture)
+        res = _batch(prob, mixed_precision=True).evaluate(np.asarray(d["thetas"]))
+        for i, row in enumerate(d["rows"]):
+            assert res.statuses[i] == row["status"], (fixture, i)
+            assert _iters_close(res.iteration_counts[i], row["iters"])
+            assert max_rel(res.predictions[i], row["predictions"]) <= REL_OBS
+
+
+def test_mixed_precision_limits():
+    prob = small_channel()
+    with pytest.raises(ValueError, match="block sizes up to 256"):
+        _batch(prob, mixed_precision=True, block_size=512)
+    prob = inout_channel("upwind", "upwind")
+    with pytest.raises(ValueError, match="up to 256 cells"):
+        _batch(prob, mixed_precision=True)
+
+
 def test_batch_results_independent_of_batch_composition():
     """Each row is bitwise the same alone, inside a batch and in another order
     (test_batch_determinism.cpp:49-63)."""
