@@ -86,7 +86,9 @@ class RqBatchOut(C.Structure):
 
 EXPORTS = ["rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
            "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
-           "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info"]
+           "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info",
+           "rq_server_create", "rq_server_add_problem", "rq_server_submit", "rq_server_poll",
+           "rq_server_stats", "rq_server_destroy"]
 
 _lib = None
 

@@ -114,4 +114,38 @@ struct DevResult {
     long long cycles[kPhases];
 };
 
+// ---------------------------------------------------------------- serving
+constexpr int kServeBlock = 256;      // threads per serving workgroup
+constexpr int kMaxServeObs = 64;      // observations per served problem
+
+// A registered problem: its mesh tables, settings and arena placement.
+struct DevProblem {
+    DevMesh mesh;
+    DevSettings settings;
+    int lds;  // 1: the arena fits the workgroup's LDS
+};
+
+// Host-written control block (pinned host memory).
+struct ServeControl {
+    long long tail;  // number of requests published
+    int stop;        // 1: workgroups return once the published requests are claimed
+    int pad;
+};
+
+// One request of the ring (pinned host memory).
+struct ServeRequest {
+    long long ticket;
+    int problem;
+    int pad;
+    double coeffs[11];
+};
+
+// One result of the ring (pinned host memory); done = ticket + 1 is written
+// last, after the result.
+struct ServeOutput {
+    DevResult res;
+    double predictions[kMaxServeObs];
+    long long done;
+};
+
 }  // namespace rq

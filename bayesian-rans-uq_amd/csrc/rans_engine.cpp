@@ -22,6 +22,10 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
                                const DevSettings &s, DevResult *results, double *pred,
                                int pred_stride, double *fields, int64_t field_stride,
                                size_t lds_bytes, hipStream_t stream);
+hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
+                        const ServeRequest *ring, ServeOutput *outs, int ring_size,
+                        unsigned long long *head, double *arenas, int64_t arena_stride,
+                        long long spin_limit, size_t lds_bytes, hipStream_t stream);
 }
 
 using namespace rq;
@@ -76,6 +80,7 @@ struct rq_engine {
     bool lds = true;
     size_t lds_bytes = 0;
     ArenaLayout lay{};
+    DevMesh mesh_host{};  // device pointers, host copy of the table
     // device
     int device = 0;
     hipStream_t stream = nullptr;
@@ -536,6 +541,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         D.nobs = no;
         D.lay = L;
         E->lay = L;
+        E->mesh_host = D;
         HIPCHECK(hipMalloc(&E->d_mesh, sizeof(DevMesh)));
         HIPCHECK(hipMemcpy(E->d_mesh, &D, sizeof(DevMesh), hipMemcpyHostToDevice));
 
@@ -611,6 +617,68 @@ void write_row_error(const rq_batch_out *out, int64_t row, const std::string &ms
                       msg.c_str());
 }
 
+// Status, log-likelihood and error text of one solved row
+// (ForwardModel.cpp:92-135, ObservationOperator.hpp:592-607).
+struct RowFinal {
+    int status;
+    double loglik;
+    bool keep_predictions;
+    std::string msg;
+};
+
+RowFinal finalize_row(const rq_engine *E, const DevResult &d, const double *pred) {
+    RowFinal f{d.status, -1e6, false, std::string()};
+    if (f.status == ST_UNKNOWN)
+        f.msg = d.error == ERR_NONFINITE_MONITOR
+                    ? "convergence monitor returned a nonfinite observation"
+                    : "linear system coefficients and source must be finite";
+    if (f.status == ST_CONVERGED || f.status == ST_UNCONVERGED) {
+        double acc = 0.0;
+        for (int j = 0; j < E->nobs; ++j) {
+            const double pj = pred[j];
+            if (!std::isfinite(pj)) {
+                f.msg = "Likelihood predictions must be finite";
+                break;
+            }
+            const double diff = pj - E->observed[j];
+            acc += -0.5 * diff * diff / E->variance[j];
+            if (!std::isfinite(acc)) {
+                f.msg = "Gaussian log-likelihood exceeds finite range";
+                break;
+            }
+        }
+        if (f.msg.empty()) {
+            f.loglik = acc;
+            f.keep_predictions = true;
+        } else {
+            f.status = ST_UNKNOWN;
+        }
+    }
+    return f;
+}
+
+// InferenceParameterSet::inBounds and SSTCoefficients::validate for one row;
+// fills the 11 coefficients. Returns 0, or ST_INVALID / ST_UNKNOWN with text.
+int admit_row(const rq_engine *E, const double *row, double *coeffs, std::string &msg) {
+    for (int i = 0; i < E->n_active; ++i) {
+        const double v = row[i];
+        if (v < E->lower[i] || v > E->upper[i] || std::isnan(v) || std::isinf(v)) {
+            msg = "coefficient row is nonfinite or out of bounds";
+            return ST_INVALID;
+        }
+    }
+    for (int i = 0; i < 11; ++i) coeffs[i] = E->base[i];
+    for (int i = 0; i < E->n_active; ++i) coeffs[E->active[i]] = row[i];
+    bool ok = true;
+    for (int i = 0; i < 11; ++i)
+        if (!std::isfinite(coeffs[i]) || coeffs[i] < 0) ok = false;
+    if (!ok || coeffs[2] == 0. || coeffs[8] == 0.) {
+        msg = "SST coefficients must be finite and nonnegative, with positive beta1 and betaStar";
+        return ST_UNKNOWN;
+    }
+    return 0;
+}
+
 int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
                   int32_t workers, rq_batch_out *out) {
     if (!E || !out || !out->status) throw Fail{RQ_EINVAL, "null argument"};
@@ -638,30 +706,15 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         if (out->phase_cycles)
             for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = 0;
         // InferenceParameterSet::inBounds (core/include/InferenceParameters.hpp:127-140)
-        bool ok = true;
-        for (int i = 0; i < n_params; ++i) {
-            const double v = row[i];
-            if (v < E->lower[i] || v > E->upper[i] || std::isnan(v) || std::isinf(v)) ok = false;
-        }
-        if (!ok) {
-            out->status[r] = ST_INVALID;
-            if (out->loglik) out->loglik[r] = -1e30;
-            write_row_error(out, r, "coefficient row is nonfinite or out of bounds");
-            continue;
-        }
+        // and SSTCoefficients::validate (core/include/SSTModel.hpp:29-45)
         DevSlot sl{};
         sl.mesh = E->d_mesh;
-        for (int i = 0; i < 11; ++i) sl.coeffs[i] = E->base[i];
-        for (int i = 0; i < n_params; ++i) sl.coeffs[E->active[i]] = row[i];
-        // SSTCoefficients::validate (core/include/SSTModel.hpp:29-45)
-        bool cok = true;
-        for (int i = 0; i < 11; ++i)
-            if (!std::isfinite(sl.coeffs[i]) || sl.coeffs[i] < 0) cok = false;
-        if (!cok || sl.coeffs[2] == 0. || sl.coeffs[8] == 0.) {
-            out->status[r] = ST_UNKNOWN;
-            if (out->loglik) out->loglik[r] = -1e6;
-            write_row_error(out, r, "SST coefficients must be finite and nonnegative, with "
-                                    "positive beta1 and betaStar");
+        std::string msg;
+        const int bad = admit_row(E, row, sl.coeffs, msg);
+        if (bad) {
+            out->status[r] = bad;
+            if (out->loglik) out->loglik[r] = bad == ST_INVALID ? -1e30 : -1e6;
+            write_row_error(out, r, msg);
             continue;
         }
         sl.row = (int)slots.size();
@@ -706,40 +759,14 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
     for (int i = 0; i < ns; ++i) {
         const int64_t r = valid[i];
         const DevResult &d = res[i];
-        int status = d.status;
-        double ll = -1e6;
-        std::string msg;
-        if (status == ST_UNKNOWN)
-            msg = d.error == ERR_NONFINITE_MONITOR
-                      ? "convergence monitor returned a nonfinite observation"
-                      : "linear system coefficients and source must be finite";
-        if (status == ST_CONVERGED || status == ST_UNCONVERGED) {
-            // ObservationOperator::logLikelihood (ObservationOperator.hpp:592-607)
-            double acc = 0.0;
-            for (int j = 0; j < nobs; ++j) {
-                const double pj = pred[(size_t)i * nobs + j];
-                if (!std::isfinite(pj)) {
-                    msg = "Likelihood predictions must be finite";
-                    break;
-                }
-                const double diff = pj - E->observed[j];
-                acc += -0.5 * diff * diff / E->variance[j];
-                if (!std::isfinite(acc)) {
-                    msg = "Gaussian log-likelihood exceeds finite range";
-                    break;
-                }
-            }
-            if (msg.empty()) {
-                ll = acc;
-                if (out->predictions)
-                    for (int j = 0; j < nobs; ++j)
-                        out->predictions[r * nobs + j] = pred[(size_t)i * nobs + j];
-            } else {
-                status = ST_UNKNOWN;
-            }
-        }
+        const RowFinal fin = finalize_row(E, d, pred.data() + (size_t)i * std::max(nobs, 1));
+        const int status = fin.status;
+        const std::string &msg = fin.msg;
+        if (fin.keep_predictions && out->predictions)
+            for (int j = 0; j < nobs; ++j)
+                out->predictions[r * nobs + j] = pred[(size_t)i * nobs + j];
         out->status[r] = status;
-        if (out->loglik) out->loglik[r] = ll;
+        if (out->loglik) out->loglik[r] = fin.loglik;
         if (out->iterations) out->iterations[r] = d.iters;
         if (out->final_worst_measure) out->final_worst_measure[r] = d.worst;
         if (out->contraction_rate_u) out->contraction_rate_u[r] = d.rate;
@@ -755,6 +782,152 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
                         fstride * sizeof(double));
     }
     return RQ_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Serving: registered problems, a persistent grid of workgroups (one slot
+// each, as many as the HBM budget and the device's co-resident capacity
+// allow), a request ring and a result ring in pinned host memory.
+// ----------------------------------------------------------------------------
+struct rq_server {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<rq_engine *> problems;  // owned: validated problem copies
+    DevProblem *d_problems = nullptr;
+    ServeControl *ctl = nullptr;        // pinned, coherent
+    ServeRequest *ring = nullptr;
+    ServeOutput *outs = nullptr;
+    int ring_size = 0;
+    unsigned long long *d_head = nullptr;
+    double *d_arenas = nullptr;
+    int64_t arena_stride = 0;           // doubles per slot arena
+    int slots = 0, max_slots = 0;
+    int64_t budget = 0;
+    size_t lds_bytes = 0;
+    long long spin_limit = 0;
+    bool prepared = false;
+    // tickets: host-visible ids; admitted ones map to a device sequence number
+    struct Ticket {
+        int problem = -1;
+        long long seq = -1;          // device sequence, -1 when settled on the host
+        bool done = false;
+        DevResult res{};
+        std::vector<double> pred;
+        int status = 0;
+        double loglik = 0.0;
+        std::string msg;
+    };
+    std::vector<Ticket> tickets;
+    std::vector<long long> slot_seq;   // device sequence occupying each ring slot, -1 free
+    std::vector<int64_t> seq_ticket;    // ring slot -> ticket id
+    long long next_seq = 0;
+    int64_t completed = 0;
+
+    ~rq_server() {
+        if (ctl) {
+            __atomic_store_n(&ctl->stop, 1, __ATOMIC_RELEASE);
+            if (stream) (void)hipStreamSynchronize(stream);
+        }
+        for (rq_engine *p : problems) delete p;
+        if (d_problems) (void)hipFree(d_problems);
+        if (d_head) (void)hipFree(d_head);
+        if (d_arenas) (void)hipFree(d_arenas);
+        if (ctl) (void)hipHostFree(ctl);
+        if (ring) (void)hipHostFree(ring);
+        if (outs) (void)hipHostFree(outs);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+bool server_running(rq_server *S) {
+    const hipError_t q = hipStreamQuery(S->stream);
+    return q == hipErrorNotReady;
+}
+
+// Sizes the slot pool from the HBM budget once the problem set is known.
+void server_prepare(rq_server *S) {
+    if (S->prepared) return;
+    if (S->problems.empty()) throw Fail{RQ_EINVAL, "no problem registered"};
+    HIPCHECK(hipSetDevice(S->device));
+    int maxLds = 0, cus = 0;
+    HIPCHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, S->device));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->device));
+    std::vector<DevProblem> table(S->problems.size());
+    int64_t stride = 2;
+    size_t lds = 0;
+    for (size_t i = 0; i < S->problems.size(); ++i) {
+        const rq_engine *E = S->problems[i];
+        const size_t bytes = (size_t)E->lay.stride * sizeof(double);
+        table[i].mesh = E->mesh_host;
+        table[i].settings = E->settings;
+        table[i].lds = bytes + 8192 <= (size_t)maxLds ? 1 : 0;
+        if (table[i].lds) lds = std::max(lds, bytes);
+        else stride = std::max<int64_t>(stride, E->lay.stride);
+    }
+    stride = (stride + 1) & ~int64_t(1);
+    const int64_t slot_bytes = stride * 8;
+    int64_t by_budget = S->budget > 0 ? S->budget / slot_bytes : S->max_slots;
+    int slots = (int)std::min<int64_t>(S->max_slots, by_budget);
+    slots = std::min(slots, cus);  // one co-resident workgroup per CU
+    if (slots < 1) throw Fail{RQ_ENOMEM, "HBM budget does not hold one solve slot"};
+    HIPCHECK(hipMalloc(&S->d_problems, table.size() * sizeof(DevProblem)));
+    HIPCHECK(hipMemcpy(S->d_problems, table.data(), table.size() * sizeof(DevProblem),
+                       hipMemcpyHostToDevice));
+    HIPCHECK(hipMalloc(&S->d_arenas, (size_t)slots * slot_bytes));
+    S->arena_stride = stride;
+    S->slots = slots;
+    S->lds_bytes = lds;
+    S->prepared = true;
+}
+
+void server_ensure_running(rq_server *S) {
+    if (server_running(S)) return;
+    HIPCHECK(hipSetDevice(S->device));
+    HIPCHECK(launch_serve(S->slots, S->d_problems, S->ctl, S->ring, S->outs, S->ring_size,
+                          S->d_head, S->d_arenas, S->arena_stride, S->spin_limit, S->lds_bytes,
+                          S->stream));
+}
+
+// Moves finished device results of the ring into their tickets.
+void server_collect(rq_server *S) {
+    for (int slot = 0; slot < S->ring_size; ++slot) {
+        const long long seq = S->slot_seq[slot];
+        if (seq < 0) continue;
+        const long long done = __atomic_load_n(&S->outs[slot].done, __ATOMIC_ACQUIRE);
+        if (done != seq + 1) continue;
+        rq_server::Ticket &t = S->tickets[S->seq_ticket[slot]];
+        const rq_engine *E = S->problems[t.problem];
+        t.res = S->outs[slot].res;
+        t.pred.assign(S->outs[slot].predictions, S->outs[slot].predictions + E->nobs);
+        const RowFinal fin = finalize_row(E, t.res, t.pred.data());
+        t.status = fin.status;
+        t.loglik = fin.loglik;
+        t.msg = fin.msg;
+        if (!fin.keep_predictions)
+            std::fill(t.pred.begin(), t.pred.end(), std::numeric_limits<double>::quiet_NaN());
+        t.done = true;
+        S->slot_seq[slot] = -1;
+        ++S->completed;
+    }
+}
+
+void fill_row(const rq_server::Ticket &t, const rq_engine *E, rq_serve_row *row) {
+    row->status = t.status;
+    row->loglik = t.loglik;
+    if (row->predictions)
+        for (int j = 0; j < E->nobs; ++j) row->predictions[j] = t.pred[j];
+    const bool solved = t.seq >= 0;
+    row->iterations = solved ? t.res.iters : 0;
+    row->final_worst_measure = solved ? t.res.worst : 0.0;
+    row->contraction_rate_u = solved ? t.res.rate : 0.0;
+    row->stalled = solved ? (uint8_t)t.res.stalled : 0;
+    row->converged = t.status == ST_CONVERGED;
+    for (int e = 0; e < 5; ++e) row->krylov_iterations[e] = solved ? t.res.krylov[e] : 0;
+    std::snprintf(row->error, sizeof row->error, "%s", t.msg.c_str());
 }
 
 }  // namespace
@@ -822,4 +995,163 @@ const char *rq_build_info(void) {
         ;
 }
 
+
+rq_server *rq_server_create(const rq_serve_options *opt, char *err, size_t errcap) {
+    rq_server *S = nullptr;
+    try {
+        if (!opt) throw Fail{RQ_EINVAL, "null argument"};
+        if (opt->max_slots < 1) throw Fail{RQ_EINVAL, "max_slots must be positive"};
+        if (opt->ring_size < 1 || opt->ring_size > (1 << 20))
+            throw Fail{RQ_EINVAL, "ring_size must be in [1, 2^20]"};
+        if (opt->hbm_budget_bytes < 0) throw Fail{RQ_EINVAL, "HBM budget must be nonnegative"};
+        if (!(opt->idle_exit_seconds > 0.0) || opt->idle_exit_seconds > 3600.0)
+            throw Fail{RQ_EINVAL, "idle_exit_seconds must be in (0, 3600]"};
+        S = new rq_server();
+        HIPCHECK(hipGetDevice(&S->device));
+        S->max_slots = opt->max_slots;
+        S->budget = opt->hbm_budget_bytes;
+        S->ring_size = opt->ring_size;
+        // one idle poll sleeps about 100 x 64 shader clocks (~2.7 us at 2.4 GHz)
+        S->spin_limit = (long long)(opt->idle_exit_seconds / 2.7e-6) + 1;
+        HIPCHECK(hipHostMalloc((void **)&S->ctl, sizeof(ServeControl),
+                               hipHostMallocCoherent | hipHostMallocMapped));
+        HIPCHECK(hipHostMalloc((void **)&S->ring, sizeof(ServeRequest) * S->ring_size,
+                               hipHostMallocCoherent | hipHostMallocMapped));
+        HIPCHECK(hipHostMalloc((void **)&S->outs, sizeof(ServeOutput) * S->ring_size,
+                               hipHostMallocCoherent | hipHostMallocMapped));
+        std::memset(S->ctl, 0, sizeof(ServeControl));
+        std::memset(S->ring, 0, sizeof(ServeRequest) * S->ring_size);
+        std::memset(S->outs, 0, sizeof(ServeOutput) * S->ring_size);
+        HIPCHECK(hipMalloc(&S->d_head, sizeof(unsigned long long)));
+        HIPCHECK(hipMemset(S->d_head, 0, sizeof(unsigned long long)));
+        HIPCHECK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+        S->slot_seq.assign(S->ring_size, -1);
+        S->seq_ticket.assign(S->ring_size, -1);
+        set_err(err, errcap, "");
+        return S;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+    } catch (const std::exception &ex) {
+        set_err(err, errcap, ex.what());
+    }
+    delete S;
+    return nullptr;
+}
+
+int32_t rq_server_add_problem(rq_server *S, const rq_mesh_desc *mesh, const rq_param_desc *params,
+                              const rq_obs_desc *obs, const rq_bc_desc *bcs, double nu,
+                              const rq_settings *settings, const double init[5], char *err,
+                              size_t errcap) {
+    try {
+        if (!S) throw Fail{RQ_EINVAL, "null server"};
+        if (S->prepared)
+            throw Fail{RQ_EINVAL, "problems are registered before the first submit"};
+        if (settings && settings->mixed_precision)
+            throw Fail{RQ_EINVAL, "served problems run in FP64"};
+        if (obs && obs->n_obs > kMaxServeObs)
+            throw Fail{RQ_EINVAL, "a served problem has at most 64 observations"};
+        rq_engine *E = create_impl(mesh, params, obs, bcs, nu, settings, init);
+        S->problems.push_back(E);
+        set_err(err, errcap, "");
+        return (int32_t)S->problems.size() - 1;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+    } catch (const std::bad_alloc &) {
+        set_err(err, errcap, "host allocation failed");
+    } catch (const std::exception &ex) {
+        set_err(err, errcap, ex.what());
+    }
+    return -1;
+}
+
+int64_t rq_server_submit(rq_server *S, int32_t problem, const double *theta, int32_t n_params,
+                         char *err, size_t errcap) {
+    try {
+        if (!S || !theta) throw Fail{RQ_EINVAL, "null argument"};
+        if (problem < 0 || problem >= (int32_t)S->problems.size())
+            throw Fail{RQ_EINVAL, "unknown problem"};
+        const rq_engine *E = S->problems[problem];
+        if (n_params != E->n_active)
+            throw Fail{RQ_EINVAL, "theta must have n_parameters entries"};
+        server_prepare(S);
+        rq_server::Ticket t;
+        t.problem = problem;
+        double coeffs[11];
+        std::string msg;
+        const int bad = admit_row(E, theta, coeffs, msg);
+        if (bad) {  // settled on the host, like a batch row that never reaches the device
+            t.done = true;
+            t.status = bad;
+            t.loglik = bad == ST_INVALID ? -1e30 : -1e6;
+            t.msg = msg;
+            t.pred.assign(E->nobs, std::numeric_limits<double>::quiet_NaN());
+            S->tickets.push_back(std::move(t));
+            set_err(err, errcap, "");
+            return (int64_t)S->tickets.size() - 1;
+        }
+        server_collect(S);
+        const long long seq = S->next_seq;
+        const int slot = (int)(seq % S->ring_size);
+        if (S->slot_seq[slot] >= 0) throw Fail{RQ_EAGAIN, "request ring is full"};
+        ServeRequest &rq = S->ring[slot];
+        rq.ticket = seq;
+        rq.problem = problem;
+        for (int i = 0; i < 11; ++i) rq.coeffs[i] = coeffs[i];
+        S->outs[slot].done = 0;
+        t.seq = seq;
+        S->tickets.push_back(std::move(t));
+        S->slot_seq[slot] = seq;
+        S->seq_ticket[slot] = (int64_t)S->tickets.size() - 1;
+        S->next_seq = seq + 1;
+        __atomic_store_n(&S->ctl->tail, seq + 1, __ATOMIC_RELEASE);
+        server_ensure_running(S);
+        set_err(err, errcap, "");
+        return (int64_t)S->tickets.size() - 1;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+        return f.code == RQ_EAGAIN ? -2 : -1;
+    } catch (const std::exception &ex) {
+        set_err(err, errcap, ex.what());
+    }
+    return -1;
+}
+
+int rq_server_poll(rq_server *S, int64_t ticket, rq_serve_row *row, int32_t *done, char *err,
+                   size_t errcap) {
+    try {
+        if (!S || !done) throw Fail{RQ_EINVAL, "null argument"};
+        if (ticket < 0 || ticket >= (int64_t)S->tickets.size())
+            throw Fail{RQ_EINVAL, "unknown ticket"};
+        *done = 0;
+        rq_server::Ticket &t = S->tickets[ticket];
+        if (!t.done) {
+            server_collect(S);
+            if (!t.done) server_ensure_running(S);
+        }
+        if (t.done) {
+            *done = 1;
+            if (row) fill_row(t, S->problems[t.problem], row);
+        }
+        set_err(err, errcap, "");
+        return RQ_OK;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+        return f.code;
+    } catch (const std::exception &ex) {
+        set_err(err, errcap, ex.what());
+    }
+    return RQ_EDEVICE;
+}
+
+int rq_server_stats(const rq_server *S, int32_t *slots, int64_t *slot_arena_bytes,
+                    int64_t *submitted, int64_t *completed) {
+    if (!S) return RQ_EINVAL;
+    if (slots) *slots = S->slots;
+    if (slot_arena_bytes) *slot_arena_bytes = S->arena_stride * 8;
+    if (submitted) *submitted = S->next_seq;
+    if (completed) *completed = S->completed;
+    return RQ_OK;
+}
+
+void rq_server_destroy(rq_server *S) { delete S; }
 }  // extern "C"

@@ -2014,28 +2014,26 @@ struct Draw {
 };
 
 // ----------------------------------------------------------------------------
-// The persistent solve kernel: blockIdx.x -> one slot (draw). The draw's
-// arena starts as a copy of the mesh's pre-filled image (geometry and int
-// tables); in LDS mode it lives in dynamic LDS.
+// One draw, start to finish, by one workgroup: the arena starts as a copy of
+// the mesh's pre-filled image (geometry and int tables), the solve runs, the
+// observations, fields and result record are written. Shared by the batch
+// kernel and the serving kernel.
 // ----------------------------------------------------------------------------
+template <int BS>
+struct BlockShared {
+    Book book;
+    double redbuf[2 * (BS / 64) * 4];
+    double coeffs[11];
+    DevSettings sset;
+    Krylov kslot;
+};
+
 template <int BS, bool kLDS>
-__global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restrict__ slots,
-                                                          DevSettings settings,
-                                                          DevResult *__restrict__ results,
-                                                          double *__restrict__ predictions,
-                                                          int pred_stride, double *__restrict__ fields,
-                                                          int64_t field_stride) {
-    extern __shared__ double smem[];
-    __shared__ Book book;
-    __shared__ double redbuf[2 * (BS / 64) * 4];
-    __shared__ double coeffs[11];
-    __shared__ DevSettings sset;
-    __shared__ Krylov kslot;
-    const DevSlot &slot = slots[blockIdx.x];
-    const DevMesh &M = *slot.mesh;
-    if (threadIdx.x < 11) coeffs[threadIdx.x] = slot.coeffs[threadIdx.x];
-    if (threadIdx.x == 0) sset = settings;
-    double *arena = kLDS ? smem : slot.arena;
+DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &settings,
+                    const double *coeffSrc, double *arena, DevResult *res, double *pred,
+                    double *fields) {
+    if (threadIdx.x < 11) sh.coeffs[threadIdx.x] = coeffSrc[threadIdx.x];
+    if (threadIdx.x == 0) sh.sset = settings;
     {
         const int64_t n16 = M.lay.image_bytes / 16;
         const uint4 *gsrc = reinterpret_cast<const uint4 *>(M.img);
@@ -2043,24 +2041,22 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
         for (int64_t i = threadIdx.x; i < n16; i += BS) dst[i] = gsrc[i];
     }
     __syncthreads();
-    Draw<BS, kLDS> d(M, sset, coeffs, arena, &book, redbuf, &kslot);
+    Draw<BS, kLDS> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot);
     d.run();
-    const int row = slot.row;
+    const Book &book = sh.book;
     const bool observe = book.status == ST_CONVERGED || book.status == ST_UNCONVERGED;
-    for (int i = threadIdx.x; i < M.nobs; i += BS)
-        predictions[(int64_t)row * pred_stride + i] = observe ? d.observe(i) : NAN;
+    for (int i = threadIdx.x; i < M.nobs; i += BS) pred[i] = observe ? d.observe(i) : NAN;
     if (fields) {
-        double *out = fields + row * field_stride;
         const int nc = M.nc;
         const int vars[10] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
         for (int a = 0; a < 10; ++a)
             for (int ci = threadIdx.x; ci < nc; ci += BS)
-                out[a * nc + ci] = arena[ci * kNA + vars[a]];
+                fields[a * nc + ci] = arena[ci * kNA + vars[a]];
         for (int f = threadIdx.x; f < M.nf; f += BS)
-            out[10 * nc + f] = arena[M.lay.face + f * kNF + W_phi];
+            fields[10 * nc + f] = arena[M.lay.face + f * kNF + W_phi];
     }
     if (threadIdx.x == 0) {
-        DevResult &r = results[row];
+        DevResult &r = *res;
         r.status = book.status;
         r.error = book.error;
         r.iters = book.status == ST_UNKNOWN ? 0 : book.finalIter;
@@ -2071,6 +2067,93 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
         r.diverged = book.status == ST_DIVERGED;
         for (int e = 0; e < 5; ++e) r.krylov[e] = book.kry[e];
         for (int e = 0; e < kPhases; ++e) r.cycles[e] = book.cyc[e];
+    }
+}
+
+// The batch kernel: blockIdx.x -> one slot (draw); in LDS mode the arena is
+// the workgroup's dynamic LDS.
+template <int BS, bool kLDS>
+__global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restrict__ slots,
+                                                          DevSettings settings,
+                                                          DevResult *__restrict__ results,
+                                                          double *__restrict__ predictions,
+                                                          int pred_stride, double *__restrict__ fields,
+                                                          int64_t field_stride) {
+    extern __shared__ double smem[];
+    __shared__ BlockShared<BS> sh;
+    const DevSlot &slot = slots[blockIdx.x];
+    const int row = slot.row;
+    solve_draw<BS, kLDS>(sh, *slot.mesh, settings, slot.coeffs, kLDS ? smem : slot.arena,
+                         results + row, predictions + (int64_t)row * pred_stride,
+                         fields ? fields + row * field_stride : nullptr);
+}
+
+// ----------------------------------------------------------------------------
+// The serving kernel: a persistent grid whose workgroups claim requests one
+// at a time from a ring in host memory (published by the host through the
+// ring's tail), solve each in their own arena and publish the result with a
+// completion sequence number. Every wait is bounded: a workgroup returns when
+// the host raises stop, or after spin_limit idle polls, so the grid always
+// drains; the host relaunches it when new requests arrive.
+// ----------------------------------------------------------------------------
+template <int BS>
+__global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict__ problems,
+                                                   ServeControl *ctl, const ServeRequest *ring,
+                                                   ServeOutput *outs, int ring_size,
+                                                   unsigned long long *head,
+                                                   double *__restrict__ arenas,
+                                                   int64_t arena_stride, long long spin_limit) {
+    extern __shared__ double smem[];
+    __shared__ BlockShared<BS> sh;
+    __shared__ long long claimed;
+    __shared__ double coeffs[11];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            long long got = -1, idle = 0;
+            for (;;) {
+                const long long tail =
+                    __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                unsigned long long h =
+                    __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((long long)h < tail) {
+                    if (__hip_atomic_compare_exchange_strong(head, &h, h + 1, __ATOMIC_ACQ_REL,
+                                                             __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        got = (long long)h;
+                        break;
+                    }
+                    continue;
+                }
+                if (__hip_atomic_load(&ctl->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM))
+                    break;
+                if (++idle > spin_limit) break;
+                __builtin_amdgcn_s_sleep(100);
+            }
+            claimed = got;
+        }
+        __syncthreads();
+        const long long ticket = claimed;
+        if (ticket < 0) return;
+        const int slot = (int)(ticket % ring_size);
+        const ServeRequest &rq = ring[slot];
+        const DevProblem &P = problems[rq.problem];
+        if (threadIdx.x < 11) coeffs[threadIdx.x] = rq.coeffs[threadIdx.x];
+        __syncthreads();
+        ServeOutput &o = outs[slot];
+        double *arena = arenas + (int64_t)blockIdx.x * arena_stride;
+        if (P.lds)
+            solve_draw<BS, true>(sh, P.mesh, P.settings, coeffs, smem, &o.res, o.predictions,
+                                 nullptr);
+        else
+            solve_draw<BS, false>(sh, P.mesh, P.settings, coeffs, arena, &o.res, o.predictions,
+                                  nullptr);
+        // every thread's result stores reach host memory before the
+        // completion number is published
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&o.done, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();
     }
 }
 
@@ -2116,6 +2199,20 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
         return hipErrorInvalidValue;
     }
 #undef RQ_CASE
+}
+
+
+hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
+                        const ServeRequest *ring, ServeOutput *outs, int ring_size,
+                        unsigned long long *head, double *arenas, int64_t arena_stride,
+                        long long spin_limit, size_t lds_bytes, hipStream_t stream) {
+    auto kern = serve_kernel<kServeBlock>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kServeBlock), lds_bytes, stream, problems, ctl, ring,
+                       outs, ring_size, head, arenas, arena_stride, spin_limit);
+    return hipGetLastError();
 }
 
 }  // namespace rq

@@ -147,6 +147,58 @@ void rq_engine_destroy(rq_engine *engine);
 /* Library identity: compiler, target and build flags. */
 const char *rq_build_info(void);
 
+/* ---------------------------------------------------------------------------
+ * Serving. No counterpart in the reference (its callers build a ForwardBatch
+ * per batch); these entry points are the continuous-batching front of the
+ * same solve: single-row requests on registered problems are admitted into
+ * GPU slots as slots free up, and overlap on the device.
+ * ------------------------------------------------------------------------- */
+#define RQ_EAGAIN 4   /* request ring full: poll finished tickets, then resubmit */
+
+typedef struct rq_server rq_server;
+
+typedef struct rq_serve_options {
+    int32_t max_slots;          /* concurrent solves at most (also capped by CU count) */
+    int64_t hbm_budget_bytes;   /* arena memory for the slots; 0 = no limit */
+    int32_t ring_size;          /* requests in flight at most */
+    double idle_exit_seconds;   /* the device loop returns after this long without work */
+} rq_serve_options;
+
+/* One finished request (the ForwardBatchRow fields of one row). */
+typedef struct rq_serve_row {
+    int32_t status;
+    double loglik;
+    double *predictions;        /* caller-owned, n_observations of the problem; may be NULL */
+    int32_t iterations;
+    double final_worst_measure, contraction_rate_u;
+    uint8_t stalled, converged;
+    int64_t krylov_iterations[5];
+    char error[256];
+} rq_serve_row;
+
+rq_server *rq_server_create(const rq_serve_options *options, char *err, size_t errcap);
+
+/* Registers a problem (the ForwardBatch constructor's inputs); returns its id
+ * or -1. All problems are registered before the first submit. */
+int32_t rq_server_add_problem(rq_server *server, const rq_mesh_desc *mesh,
+                              const rq_param_desc *params, const rq_obs_desc *obs,
+                              const rq_bc_desc *bcs, double nu, const rq_settings *settings,
+                              const double init[5], char *err, size_t errcap);
+
+/* Submits one coefficient row; returns a ticket >= 0, -2 when the ring is
+ * full, -1 on error. Rows out of bounds settle at once as InvalidParameters. */
+int64_t rq_server_submit(rq_server *server, int32_t problem, const double *theta,
+                         int32_t n_params, char *err, size_t errcap);
+
+/* Non-blocking: *done = 1 and row filled once the ticket's solve finished. */
+int rq_server_poll(rq_server *server, int64_t ticket, rq_serve_row *row, int32_t *done,
+                   char *err, size_t errcap);
+
+int rq_server_stats(const rq_server *server, int32_t *slots, int64_t *slot_arena_bytes,
+                    int64_t *submitted, int64_t *completed);
+
+void rq_server_destroy(rq_server *server);
+
 #ifdef __cplusplus
 }
 #endif
