@@ -69,6 +69,28 @@ struct ArenaLayout {
     int64_t image_bytes;             // bytes of the pre-filled image copied at launch
 };
 
+// Algebraic multigrid hierarchy for the pressure correction: patterns and
+// transfer maps shared by every draw on the mesh (built on the host once);
+// each draw's values live in its workspace at the given offsets (doubles).
+constexpr int kMaxAmgLevels = 8;
+constexpr int kMaxAmgCoarse = 128;  // rows of the coarsest level (dense inverse)
+struct AmgLevel {
+    int n, nnz;
+    const int *rowptr, *col, *diagpos;    // CSR pattern (levels >= 1)
+    const int *agg;                       // [n] row -> aggregate on the next level
+    const int *mptr, *midx;               // next level's rows -> members here, ascending
+    const int *gptr, *gidx;               // next level's entries -> entries here (Galerkin)
+    int64_t val, x, b, r, dinv;           // workspace offsets (levels >= 1)
+};
+struct AmgMesh {
+    int levels;                           // 0: no AMG
+    int coarse_n;                         // rows of the last level
+    int64_t inv;                          // workspace offset of its dense inverse
+    int64_t workspace;                    // doubles per draw
+    const int *src0;                      // level-0 CSR entry -> cell*8 + (0 diag | 1+slot)
+    AmgLevel lev[kMaxAmgLevels];
+};
+
 // Mesh and problem tables in device memory, shared by every draw on the mesh.
 struct DevMesh {
     const double *img;  // pre-filled arena image (geometry, zero state, int tables)
@@ -77,6 +99,7 @@ struct DevMesh {
     const double *wall_y;        // [nc] wall-face distance of the constrained cell, NaN otherwise
     int nobs;                    // prepared observations (their stencils are in the image)
     ArenaLayout lay;  // per-draw arena layout for this mesh
+    AmgMesh amg;      // pressure-correction multigrid (levels == 0 when unused)
 };
 
 struct DevSettings {
@@ -102,6 +125,7 @@ struct DevSettings {
 struct DevSlot {
     const DevMesh *mesh;
     double *arena;          // used when the arena lives in global memory
+    double *amg;            // multigrid workspace (mesh.amg.workspace doubles) or null
     double coeffs[11];
     int row;                // output row index
 };

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -25,7 +26,8 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
 hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                         const ServeRequest *ring, ServeOutput *outs, int ring_size,
                         unsigned long long *head, double *arenas, int64_t arena_stride,
-                        long long spin_limit, size_t lds_bytes, hipStream_t stream);
+                        double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
+                        hipStream_t stream);
 }
 
 using namespace rq;
@@ -89,8 +91,9 @@ struct rq_engine {
     DevMesh *d_mesh = nullptr;
     DevSlot *d_slots = nullptr;
     DevResult *d_results = nullptr;
-    double *d_pred = nullptr, *d_fields = nullptr, *d_arena = nullptr;
+    double *d_pred = nullptr, *d_fields = nullptr, *d_arena = nullptr, *d_amg = nullptr;
     size_t cap_slots = 0, cap_results = 0, cap_pred = 0, cap_fields = 0, cap_arena = 0;
+    size_t cap_amg = 0;
     double last_ms = 0.0;
 
     ~rq_engine() {
@@ -101,6 +104,7 @@ struct rq_engine {
         if (d_pred) (void)hipFree(d_pred);
         if (d_fields) (void)hipFree(d_fields);
         if (d_arena) (void)hipFree(d_arena);
+        if (d_amg) (void)hipFree(d_amg);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -140,8 +144,6 @@ void validate_settings(const rq_settings &s) {
         if (v < 0 || v > 2) throw Fail{RQ_EINVAL, "unknown linear solver"};
     if (s.momentum_solver == SOLVER_AMG || s.turbulence_solver == SOLVER_AMG)
         throw Fail{RQ_EINVAL, "AMG is supported for the pressure correction only"};
-    if (s.pressure_solver == SOLVER_AMG)
-        throw Fail{RQ_EINVAL, "the AMG pressure solver is not available in this engine build"};
     if (s.convergence_window > kMaxWindow)
         throw Fail{RQ_EINVAL, "convergence window exceeds the engine limit of 256"};
     if (s.mixed_precision != 0 && s.mixed_precision != 1)
@@ -241,6 +243,169 @@ int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, 
     L.stride = o;
     L.image_bytes = o * 8;
     return o;
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid hierarchy of the pressure correction, built once per mesh.
+// Level 0 is the LDU system in CSR form with each row sorted by column
+// (lduToCSR, core/include/LinearSolver.hpp:472-532); its values for the
+// setup are the geometric Laplacian A_f / delta_f (the pressure-correction
+// coefficients without the per-draw (V/aP)_f factor, plus the outlet faces
+// and the cell-0 pin). Aggregates follow buildAggregates
+// (LinearSolver.hpp:535-575): greedy seeds with strength
+// |a_ij| / sqrt(|a_ii a_jj|) > 0.25, orphans joining their strongest
+// neighbour. Coarse patterns follow galerkinProduct (LinearSolver.hpp:578-618)
+// and coarsening stops at 50 rows or when it stalls (LinearSolver.hpp:797-812).
+// Only the finest level uses the strength filter: on coarser levels every
+// unaggregated neighbour joins the seed, because the filtered coarsening
+// stalls on anisotropic near-wall meshes (a 120x64 hill keeps 470 of 7680
+// rows after 15 levels) while a V-cycle in one workgroup needs few levels
+// and a small dense coarsest solve.
+// ---------------------------------------------------------------------------
+struct AmgHostLevel {
+    int n = 0;
+    std::vector<int> rowptr, col, diagpos, agg, mptr, midx, gptr, gidx;
+    std::vector<double> val;
+};
+
+struct AmgHost {
+    std::vector<AmgHostLevel> lev;
+    std::vector<int> src0;
+};
+
+void amg_aggregate(const AmgHostLevel &A, double threshold, std::vector<int> &map,
+                   int &ncoarse) {
+    const int n = A.n;
+    map.assign(n, -1);
+    ncoarse = 0;
+    std::vector<double> diag(n, 0.0);
+    for (int i = 0; i < n; ++i) diag[i] = A.val[A.diagpos[i]];
+    for (int i = 0; i < n; ++i) {
+        if (map[i] >= 0) continue;
+        const int agg = ncoarse++;
+        map[i] = agg;
+        const double aii = std::abs(diag[i]);
+        for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; ++j) {
+            const int c = A.col[j];
+            if (c == i || map[c] >= 0) continue;
+            const double ajj = std::abs(diag[c]);
+            const double strength = std::abs(A.val[j]) / std::sqrt(aii * ajj + 1e-30);
+            if (strength > threshold) map[c] = agg;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (map[i] >= 0) continue;
+        double best = -1.0;
+        int bestAgg = 0;
+        for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; ++j) {
+            const int c = A.col[j];
+            if (c == i && map[c] < 0) continue;
+            if (map[c] >= 0 && std::abs(A.val[j]) > best) {
+                best = std::abs(A.val[j]);
+                bestAgg = map[c];
+            }
+        }
+        map[i] = bestAgg;
+    }
+}
+
+void amg_build(const rq_mesh_desc &m, const int *ISL, const int *IBF, bool has_outlet,
+               AmgHost &H) {
+    const int nc = m.n_cells, nif = m.n_internal;
+    AmgHostLevel L0;
+    L0.n = nc;
+    H.src0.clear();
+    L0.rowptr.assign(nc + 1, 0);
+    for (int c = 0; c < nc; ++c) {
+        struct E {
+            int col, src;
+            double v;
+        };
+        std::vector<E> row;
+        double d = 0.0;
+        for (int s = 0; s < kSlots; ++s) {
+            const int code = ISL[8 * c + s];
+            if (code < 0) continue;
+            const int f = code & 0x3fffffff;
+            const double g = m.face_area[f] / std::max(m.face_delta[f], 1e-20);
+            if (f < nif) {
+                const int nbr = ISL[8 * c + 4 + s];
+                if (nbr == c) throw Fail{RQ_EINVAL, "the AMG pressure solver needs a mesh "
+                                                    "without self-periodic faces"};
+                row.push_back({nbr, 1 + s, -g});
+                d += g;
+            } else if (IBF[f - nif] & kFlagOutlet) {
+                d += g;
+            }
+        }
+        if (!has_outlet && c == 0) d += std::abs(d) > 1e-30 ? d : 1.0;
+        row.push_back({c, 0, d});
+        std::sort(row.begin(), row.end(), [](const E &a, const E &b) { return a.col < b.col; });
+        for (size_t k = 1; k < row.size(); ++k)
+            if (row[k].col == row[k - 1].col)
+                throw Fail{RQ_EINVAL, "the AMG pressure solver needs rows without repeated "
+                                      "columns"};
+        for (const E &x : row) {
+            if (x.col == c) L0.diagpos.push_back((int)L0.col.size());
+            L0.col.push_back(x.col);
+            L0.val.push_back(x.v);
+            H.src0.push_back(c * 8 + x.src);
+        }
+        L0.rowptr[c + 1] = (int)L0.col.size();
+    }
+    H.lev.clear();
+    H.lev.push_back(std::move(L0));
+    for (int l = 0; l + 1 < kMaxAmgLevels; ++l) {
+        AmgHostLevel &F = H.lev[l];
+        if (F.n <= 50) break;
+        std::vector<int> map;
+        int ncoarse = 0;
+        amg_aggregate(F, l == 0 ? 0.25 : 0.0, map, ncoarse);
+        if (ncoarse == 0 || ncoarse >= F.n) break;
+        // Galerkin pattern, each coarse row sorted by column
+        std::vector<std::map<int, double>> rows(ncoarse);
+        for (int i = 0; i < F.n; ++i)
+            for (int j = F.rowptr[i]; j < F.rowptr[i + 1]; ++j) rows[map[i]][map[F.col[j]]] += F.val[j];
+        AmgHostLevel Cl;
+        Cl.n = ncoarse;
+        Cl.rowptr.assign(ncoarse + 1, 0);
+        Cl.diagpos.assign(ncoarse, -1);
+        for (int I = 0; I < ncoarse; ++I) {
+            for (const auto &kv : rows[I]) {
+                if (kv.first == I) Cl.diagpos[I] = (int)Cl.col.size();
+                Cl.col.push_back(kv.first);
+                Cl.val.push_back(kv.second);
+            }
+            Cl.rowptr[I + 1] = (int)Cl.col.size();
+        }
+        for (int I = 0; I < ncoarse; ++I)
+            if (Cl.diagpos[I] < 0) throw Fail{RQ_EINVAL, "AMG coarse level without a diagonal"};
+        // fine entry -> coarse entry, gathered per coarse entry in fine order
+        std::vector<std::vector<int>> contrib(Cl.col.size());
+        for (int i = 0; i < F.n; ++i)
+            for (int j = F.rowptr[i]; j < F.rowptr[i + 1]; ++j) {
+                const int I = map[i], J = map[F.col[j]];
+                const int *b = Cl.col.data() + Cl.rowptr[I], *e2 = Cl.col.data() + Cl.rowptr[I + 1];
+                const int pos = (int)(std::lower_bound(b, e2, J) - Cl.col.data());
+                contrib[pos].push_back(j);
+            }
+        F.gptr.assign(Cl.col.size() + 1, 0);
+        for (size_t k = 0; k < contrib.size(); ++k) {
+            F.gidx.insert(F.gidx.end(), contrib[k].begin(), contrib[k].end());
+            F.gptr[k + 1] = (int)F.gidx.size();
+        }
+        F.agg = map;
+        std::vector<std::vector<int>> members(ncoarse);
+        for (int i = 0; i < F.n; ++i) members[map[i]].push_back(i);
+        F.mptr.assign(ncoarse + 1, 0);
+        for (int I = 0; I < ncoarse; ++I) {
+            F.midx.insert(F.midx.end(), members[I].begin(), members[I].end());
+            F.mptr[I + 1] = (int)F.midx.size();
+        }
+        H.lev.push_back(std::move(Cl));
+    }
+    if (H.lev.back().n > kMaxAmgCoarse)
+        throw Fail{RQ_EINVAL, "AMG coarsening stalled above 128 rows"};
 }
 
 rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq_obs_desc *op,
@@ -523,11 +688,29 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
             if (m.patch_type[pi] == PATCH_OUTLET && m.patch_face_ptr[pi + 1] > m.patch_face_ptr[pi])
                 has_outlet = true;
 
-        // ---- device copy: the arena image and the initial wall-omega table
+        AmgHost H;
+        const bool use_amg = s.pressure_solver == SOLVER_AMG;
+        if (use_amg) amg_build(m, ISL, IBF, has_outlet, H);
+
+        // ---- device copy: the arena image, the initial wall-omega table and
+        //      the multigrid patterns
         Blob B;
         DevMesh D{};
         const size_t o_img = B.add(image);
         size_t o_wy = B.add(wall_y);
+        struct LevOff {
+            size_t rowptr, col, diagpos, agg, mptr, midx, gptr, gidx;
+        };
+        std::vector<LevOff> lo(H.lev.size());
+        size_t o_src0 = 0;
+        if (use_amg) {
+            o_src0 = B.add(H.src0);
+            for (size_t l = 0; l < H.lev.size(); ++l) {
+                const AmgHostLevel &A = H.lev[l];
+                lo[l] = {B.add(A.rowptr), B.add(A.col), B.add(A.diagpos), B.add(A.agg),
+                         B.add(A.mptr),   B.add(A.midx), B.add(A.gptr),  B.add(A.gidx)};
+            }
+        }
 
         HIPCHECK(hipGetDevice(&E->device));
         HIPCHECK(hipMalloc(&E->d_blob, B.host.size()));
@@ -539,6 +722,45 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         D.has_outlet = has_outlet; D.planar = planar;
         D.wall_y = Dd(o_wy);
         D.nobs = no;
+        if (use_amg) {
+            auto I = [&](size_t o) { return reinterpret_cast<const int *>(b0 + o); };
+            AmgMesh &G = D.amg;
+            G.levels = (int)H.lev.size();
+            G.src0 = I(o_src0);
+            int64_t w = 0;
+            for (int l = 0; l < G.levels; ++l) {
+                const AmgHostLevel &A = H.lev[l];
+                AmgLevel &g = G.lev[l];
+                g.n = A.n;
+                g.nnz = (int)A.col.size();
+                g.rowptr = I(lo[l].rowptr);
+                g.col = I(lo[l].col);
+                g.diagpos = I(lo[l].diagpos);
+                g.agg = I(lo[l].agg);
+                g.mptr = I(lo[l].mptr);
+                g.midx = I(lo[l].midx);
+                g.gptr = I(lo[l].gptr);
+                g.gidx = I(lo[l].gidx);
+                if (l == 0) {
+                    g.val = g.x = g.b = g.r = g.dinv = -1;
+                    continue;
+                }
+                g.val = w;
+                w += g.nnz;
+                g.x = w;
+                w += g.n;
+                g.b = w;
+                w += g.n;
+                g.r = w;
+                w += g.n;
+                g.dinv = w;
+                w += g.n;
+            }
+            G.coarse_n = H.lev.back().n;
+            G.inv = w;
+            w += 2 * (int64_t)G.coarse_n * G.coarse_n + G.coarse_n;
+            G.workspace = (w + 1) & ~int64_t(1);
+        }
         D.lay = L;
         E->lay = L;
         E->mesh_host = D;
@@ -733,6 +955,11 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         grow(E->d_arena, E->cap_arena, (size_t)ns * E->lay.stride);
         for (int i = 0; i < ns; ++i) slots[i].arena = E->d_arena + (size_t)i * E->lay.stride;
     }
+    const int64_t amgw = E->mesh_host.amg.workspace;
+    if (amgw > 0) {
+        grow(E->d_amg, E->cap_amg, (size_t)ns * amgw);
+        for (int i = 0; i < ns; ++i) slots[i].amg = E->d_amg + (size_t)i * amgw;
+    }
     HIPCHECK(hipMemcpyAsync(E->d_slots, slots.data(), ns * sizeof(DevSlot), hipMemcpyHostToDevice,
                             E->stream));
     HIPCHECK(hipEventRecord(E->ev0, E->stream));
@@ -803,6 +1030,8 @@ struct rq_server {
     unsigned long long *d_head = nullptr;
     double *d_arenas = nullptr;
     int64_t arena_stride = 0;           // doubles per slot arena
+    double *d_amg = nullptr;
+    int64_t amg_stride = 0;             // doubles per slot multigrid workspace
     int slots = 0, max_slots = 0;
     int64_t budget = 0;
     size_t lds_bytes = 0;
@@ -834,6 +1063,7 @@ struct rq_server {
         if (d_problems) (void)hipFree(d_problems);
         if (d_head) (void)hipFree(d_head);
         if (d_arenas) (void)hipFree(d_arenas);
+        if (d_amg) (void)hipFree(d_amg);
         if (ctl) (void)hipHostFree(ctl);
         if (ring) (void)hipHostFree(ring);
         if (outs) (void)hipHostFree(outs);
@@ -857,7 +1087,7 @@ void server_prepare(rq_server *S) {
     HIPCHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, S->device));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->device));
     std::vector<DevProblem> table(S->problems.size());
-    int64_t stride = 2;
+    int64_t stride = 2, amgw = 0;
     size_t lds = 0;
     for (size_t i = 0; i < S->problems.size(); ++i) {
         const rq_engine *E = S->problems[i];
@@ -867,9 +1097,10 @@ void server_prepare(rq_server *S) {
         table[i].lds = bytes + 8192 <= (size_t)maxLds ? 1 : 0;
         if (table[i].lds) lds = std::max(lds, bytes);
         else stride = std::max<int64_t>(stride, E->lay.stride);
+        amgw = std::max<int64_t>(amgw, E->mesh_host.amg.workspace);
     }
     stride = (stride + 1) & ~int64_t(1);
-    const int64_t slot_bytes = stride * 8;
+    const int64_t slot_bytes = (stride + amgw) * 8;
     int64_t by_budget = S->budget > 0 ? S->budget / slot_bytes : S->max_slots;
     int slots = (int)std::min<int64_t>(S->max_slots, by_budget);
     slots = std::min(slots, cus);  // one co-resident workgroup per CU
@@ -877,8 +1108,10 @@ void server_prepare(rq_server *S) {
     HIPCHECK(hipMalloc(&S->d_problems, table.size() * sizeof(DevProblem)));
     HIPCHECK(hipMemcpy(S->d_problems, table.data(), table.size() * sizeof(DevProblem),
                        hipMemcpyHostToDevice));
-    HIPCHECK(hipMalloc(&S->d_arenas, (size_t)slots * slot_bytes));
+    HIPCHECK(hipMalloc(&S->d_arenas, (size_t)slots * stride * 8));
+    if (amgw > 0) HIPCHECK(hipMalloc(&S->d_amg, (size_t)slots * amgw * 8));
     S->arena_stride = stride;
+    S->amg_stride = amgw;
     S->slots = slots;
     S->lds_bytes = lds;
     S->prepared = true;
@@ -888,8 +1121,8 @@ void server_ensure_running(rq_server *S) {
     if (server_running(S)) return;
     HIPCHECK(hipSetDevice(S->device));
     HIPCHECK(launch_serve(S->slots, S->d_problems, S->ctl, S->ring, S->outs, S->ring_size,
-                          S->d_head, S->d_arenas, S->arena_stride, S->spin_limit, S->lds_bytes,
-                          S->stream));
+                          S->d_head, S->d_arenas, S->arena_stride, S->d_amg, S->amg_stride,
+                          S->spin_limit, S->lds_bytes, S->stream));
 }
 
 // Moves finished device results of the ring into their tickets.

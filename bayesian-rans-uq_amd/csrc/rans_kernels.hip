@@ -160,6 +160,7 @@ struct Draw {
     const double *c;  // 11 coefficients (LDS)
     Book *bk;
     Krylov *kres;  // LDS slot broadcasting a wave-resident solve's result
+    double *amgw;  // multigrid workspace of this draw (global memory) or null
     Reducer<BS> red;
     int nc, nif, nf, nb;
     bool planar, has_outlet;
@@ -174,8 +175,8 @@ struct Draw {
     int nobs;
 
     DEV Draw(const DevMesh &m, const DevSettings &s, const double *coeffs, double *A, Book *b,
-             double *redbuf, Krylov *kr)
-        : M(m), S(s), c(coeffs), bk(b), kres(kr) {
+             double *redbuf, Krylov *kr, double *amg)
+        : M(m), S(s), c(coeffs), bk(b), kres(kr), amgw(amg) {
         red.buf = redbuf;
         red.parity = 0;
         nc = M.nc;
@@ -1345,9 +1346,239 @@ struct Draw {
     }
 
     // Every thread calls; the result is the same in every thread.
+    // ------------------------------------------------------------ multigrid
+    // AMG-preconditioned CG for the pressure correction (AMGSolver::solve,
+    // LinearSolver.hpp:908-970) with a frozen aggregation hierarchy: every
+    // solve refreshes the Galerkin values of all levels from the current
+    // level-0 matrix (sumCoarse, LinearSolver.hpp:699-706, as fixed-order
+    // gathers) and inverts the coarsest level densely. The V-cycle smooths
+    // with damped Jacobi (2 sweeps before and after, omega = 2/3), so the
+    // preconditioner is symmetric and every step is parallel.
+    static constexpr double kJacobiOmega = 2.0 / 3.0;
+
+    DEV double lev0_value(int src) const {
+        const int ci = src >> 3, which = src & 7;
+        return which == 0 ? C(ci, V_diag) : off(ci, which - 1);
+    }
+
+    DEV void amg_refresh() {
+        const AmgMesh &G = M.amg;
+        for (int l = 1; l < G.levels; ++l) {
+            const AmgLevel &F = G.lev[l - 1], &L = G.lev[l];
+            double *val = amgw + L.val;
+            for (int e = tid; e < L.nnz; e += BS) {
+                double a = 0.0;
+                for (int q = F.gptr[e]; q < F.gptr[e + 1]; ++q) {
+                    const int fe = F.gidx[q];
+                    a += l == 1 ? lev0_value(G.src0[fe]) : amgw[F.val + fe];
+                }
+                val[e] = a;
+            }
+            __syncthreads();
+            for (int i = tid; i < L.n; i += BS) {
+                const double d = val[L.diagpos[i]];
+                amgw[L.dinv + i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+            }
+        }
+        __syncthreads();
+        // dense Gauss-Jordan inverse of the coarsest level (symmetric positive
+        // definite: no pivoting); A in inv[0, n^2), its inverse after it
+        const int n = G.coarse_n;
+        const AmgLevel &L = G.lev[G.levels - 1];
+        double *A = amgw + G.inv, *X = A + n * n, *fac = X + n * n;
+        for (int e = tid; e < n * n; e += BS) {
+            A[e] = 0.0;
+            X[e] = (e / n == e % n) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += BS)
+            for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q)
+                A[i * n + L.col[q]] = G.levels == 1 ? lev0_value(G.src0[q]) : amgw[L.val + q];
+        __syncthreads();
+        for (int kk = 0; kk < n; ++kk) {
+            const double piv = A[kk * n + kk];
+            const double ip = fabs(piv) > 1e-300 ? 1.0 / piv : 1.0;
+            __syncthreads();
+            for (int j = tid; j < n; j += BS) {
+                A[kk * n + j] *= ip;
+                X[kk * n + j] *= ip;
+            }
+            for (int i = tid; i < n; i += BS) fac[i] = A[i * n + kk];
+            __syncthreads();
+            for (int e = tid; e < n * n; e += BS) {
+                const int i = e / n, j = e % n;
+                if (i == kk) continue;
+                A[e] -= fac[i] * A[kk * n + j];
+                X[e] -= fac[i] * X[kk * n + j];
+            }
+            __syncthreads();
+        }
+    }
+
+    // y = A x on a coarse level (CSR)
+    DEV double coarse_row(const AmgLevel &L, int i, const double *x) const {
+        double a = 0.0;
+        const double *val = amgw + L.val;
+        for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q) a += val[q] * x[L.col[q]];
+        return a;
+    }
+
+    // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks
+    DEV void amg_vcycle() {
+        const AmgMesh &G = M.amg;
+        const int nl = G.levels;
+        const double w = kJacobiOmega;
+        // down: pre-smooth, residual, restrict
+        for (int l = 0; l < nl - 1; ++l) {
+            const AmgLevel &L = G.lev[l];
+            if (l == 0) {
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const double d = C(ci, V_diag);
+                    C(ci, V_krh) = fabs(d) > 1e-300 ? w * C(ci, V_kr) / d : 0.0;
+                }
+                __syncthreads();
+                for (int ci = tid; ci < nc; ci += BS) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+                __syncthreads();
+                for (int ci = tid; ci < nc; ci += BS) {
+                    const double d = C(ci, V_diag);
+                    if (fabs(d) > 1e-300) C(ci, V_krh) += w * C(ci, V_ks) / d;
+                }
+                __syncthreads();
+                for (int ci = tid; ci < nc; ci += BS) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+            } else {
+                double *x = amgw + L.x, *b = amgw + L.b, *r = amgw + L.r, *di = amgw + L.dinv;
+                for (int i = tid; i < L.n; i += BS) x[i] = w * b[i] * di[i];
+                __syncthreads();
+                for (int i = tid; i < L.n; i += BS) r[i] = b[i] - coarse_row(L, i, x);
+                __syncthreads();
+                for (int i = tid; i < L.n; i += BS) x[i] += w * r[i] * di[i];
+                __syncthreads();
+                for (int i = tid; i < L.n; i += BS) r[i] = b[i] - coarse_row(L, i, x);
+            }
+            __syncthreads();
+            const AmgLevel &Cl = G.lev[l + 1];
+            double *bc = amgw + Cl.b;
+            for (int I = tid; I < Cl.n; I += BS) {
+                double a = 0.0;
+                for (int q = L.mptr[I]; q < L.mptr[I + 1]; ++q)
+                    a += l == 0 ? C(L.midx[q], V_ks) : amgw[L.r + L.midx[q]];
+                bc[I] = a;
+            }
+            __syncthreads();
+        }
+        // coarsest: x = A^-1 b
+        {
+            const int n = G.coarse_n;
+            const double *X = amgw + G.inv + n * n;
+            if (nl == 1) {
+                for (int i = tid; i < n; i += BS) {
+                    double a = 0.0;
+                    for (int j = 0; j < n; ++j) a += X[i * n + j] * C(j, V_kr);
+                    C(i, V_krh) = a;
+                }
+                __syncthreads();
+                return;
+            }
+            const AmgLevel &L = G.lev[nl - 1];
+            double *x = amgw + L.x, *b = amgw + L.b;
+            for (int i = tid; i < n; i += BS) {
+                double a = 0.0;
+                for (int j = 0; j < n; ++j) a += X[i * n + j] * b[j];
+                x[i] = a;
+            }
+            __syncthreads();
+        }
+        // up: prolong, post-smooth
+        for (int l = nl - 2; l >= 0; --l) {
+            const AmgLevel &L = G.lev[l];
+            const double *xc = amgw + G.lev[l + 1].x;
+            if (l == 0) {
+                for (int ci = tid; ci < nc; ci += BS) C(ci, V_krh) += xc[L.agg[ci]];
+                __syncthreads();
+                for (int sweep = 0; sweep < 2; ++sweep) {
+                    for (int ci = tid; ci < nc; ci += BS) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+                    __syncthreads();
+                    for (int ci = tid; ci < nc; ci += BS) {
+                        const double d = C(ci, V_diag);
+                        if (fabs(d) > 1e-300) C(ci, V_krh) += w * C(ci, V_ks) / d;
+                    }
+                    __syncthreads();
+                }
+            } else {
+                double *x = amgw + L.x, *b = amgw + L.b, *r = amgw + L.r, *di = amgw + L.dinv;
+                for (int i = tid; i < L.n; i += BS) x[i] += xc[L.agg[i]];
+                __syncthreads();
+                for (int sweep = 0; sweep < 2; ++sweep) {
+                    for (int i = tid; i < L.n; i += BS) r[i] = b[i] - coarse_row(L, i, x);
+                    __syncthreads();
+                    for (int i = tid; i < L.n; i += BS) x[i] += w * r[i] * di[i];
+                    __syncthreads();
+                }
+            }
+        }
+    }
+
+    // AMGSolver::solve from x = 0 (LinearSolver.hpp:908-970): kr holds b.
+    DEV Krylov amg_pcg(double rr, int maxIter, double tol) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        amg_refresh();
+        amg_vcycle();
+        double a0[1] = {0.0};
+        for (int ci = tid; ci < nc; ci += BS) {
+            C(ci, V_kp) = C(ci, V_krh);
+            a0[0] += C(ci, V_kr) * C(ci, V_krh);
+        }
+        red.sum(a0);
+        double rz = a0[0];
+        for (int it = 0; it < maxIter; ++it) {
+            double a1[1] = {0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                const double av = spmv(ci, V_kp);
+                C(ci, V_kv) = av;
+                a1[0] += C(ci, V_kp) * av;
+            }
+            red.sum(a1);
+            const double pAp = a1[0];
+            if (fabs(pAp) < 1e-30) break;
+            const double alpha = rz / pAp;
+            double a2[1] = {0.0};
+            for (int ci = tid; ci < nc; ci += BS) {
+                C(ci, V_kx) += alpha * C(ci, V_kp);
+                const double rv = C(ci, V_kr) - alpha * C(ci, V_kv);
+                C(ci, V_kr) = rv;
+                a2[0] += rv * rv;
+            }
+            red.sum(a2);
+            const double rn = sqrt(a2[0]);
+            res.iterations = it + 1;
+            if (rn / r0 < tol) {
+                res.converged = 1;
+                return res;
+            }
+            amg_vcycle();
+            double a3[1] = {0.0};
+            for (int ci = tid; ci < nc; ci += BS) a3[0] += C(ci, V_kr) * C(ci, V_krh);
+            red.sum(a3);
+            const double rzNew = a3[0];
+            const double beta = rzNew / (rz + 1e-30);
+            for (int ci = tid; ci < nc; ci += BS) C(ci, V_kp) = C(ci, V_krh) + beta * C(ci, V_kp);
+            __syncthreads();
+            rz = rzNew;
+        }
+        return res;
+    }
+
     // The wave-resident path is compiled into the launch shapes that small
     // meshes use (up to 256 threads).
     DEV Krylov solve(int kind, double rr, bool zeroStart) {
+        if (kind == SOLVER_AMG && zeroStart && M.amg.levels > 0)
+            return amg_pcg(rr, S.inner_iterations, S.inner_tolerance);
         if constexpr (BS <= 256) {
             if (nc <= 256) {
                 if (tid < 64) {
@@ -2030,8 +2261,8 @@ struct BlockShared {
 
 template <int BS, bool kLDS>
 DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &settings,
-                    const double *coeffSrc, double *arena, DevResult *res, double *pred,
-                    double *fields) {
+                    const double *coeffSrc, double *arena, double *amg, DevResult *res,
+                    double *pred, double *fields) {
     if (threadIdx.x < 11) sh.coeffs[threadIdx.x] = coeffSrc[threadIdx.x];
     if (threadIdx.x == 0) sh.sset = settings;
     {
@@ -2041,7 +2272,7 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
         for (int64_t i = threadIdx.x; i < n16; i += BS) dst[i] = gsrc[i];
     }
     __syncthreads();
-    Draw<BS, kLDS> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot);
+    Draw<BS, kLDS> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg);
     d.run();
     const Book &book = sh.book;
     const bool observe = book.status == ST_CONVERGED || book.status == ST_UNCONVERGED;
@@ -2084,7 +2315,7 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
     const DevSlot &slot = slots[blockIdx.x];
     const int row = slot.row;
     solve_draw<BS, kLDS>(sh, *slot.mesh, settings, slot.coeffs, kLDS ? smem : slot.arena,
-                         results + row, predictions + (int64_t)row * pred_stride,
+                         slot.amg, results + row, predictions + (int64_t)row * pred_stride,
                          fields ? fields + row * field_stride : nullptr);
 }
 
@@ -2102,7 +2333,8 @@ __global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict_
                                                    ServeOutput *outs, int ring_size,
                                                    unsigned long long *head,
                                                    double *__restrict__ arenas,
-                                                   int64_t arena_stride, long long spin_limit) {
+                                                   int64_t arena_stride, double *amgs,
+                                                   int64_t amg_stride, long long spin_limit) {
     extern __shared__ double smem[];
     __shared__ BlockShared<BS> sh;
     __shared__ long long claimed;
@@ -2141,12 +2373,13 @@ __global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict_
         __syncthreads();
         ServeOutput &o = outs[slot];
         double *arena = arenas + (int64_t)blockIdx.x * arena_stride;
+        double *amg = amgs ? amgs + (int64_t)blockIdx.x * amg_stride : nullptr;
         if (P.lds)
-            solve_draw<BS, true>(sh, P.mesh, P.settings, coeffs, smem, &o.res, o.predictions,
+            solve_draw<BS, true>(sh, P.mesh, P.settings, coeffs, smem, amg, &o.res, o.predictions,
                                  nullptr);
         else
-            solve_draw<BS, false>(sh, P.mesh, P.settings, coeffs, arena, &o.res, o.predictions,
-                                  nullptr);
+            solve_draw<BS, false>(sh, P.mesh, P.settings, coeffs, arena, amg, &o.res,
+                                  o.predictions, nullptr);
         // every thread's result stores reach host memory before the
         // completion number is published
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
@@ -2205,13 +2438,14 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
 hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                         const ServeRequest *ring, ServeOutput *outs, int ring_size,
                         unsigned long long *head, double *arenas, int64_t arena_stride,
-                        long long spin_limit, size_t lds_bytes, hipStream_t stream) {
+                        double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
+                        hipStream_t stream) {
     auto kern = serve_kernel<kServeBlock>;
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kServeBlock), lds_bytes, stream, problems, ctl, ring,
-                       outs, ring_size, head, arenas, arena_stride, spin_limit);
+                       outs, ring_size, head, arenas, arena_stride, amgs, amg_stride, spin_limit);
     return hipGetLastError();
 }
 

@@ -221,6 +221,34 @@ def test_mixed_precision_limits():
         _batch(prob, mixed_precision=True)
 
 
+def test_amg_matches_pcg_at_tight_tolerance():
+    """The device AMG-preconditioned CG solves the same pressure systems as
+    Jacobi-PCG: with tight inner tolerances the two give the same state."""
+    out = {}
+    for solver in ("PCG", "AMG"):
+        prob = cases.hill_problem(nx=30, ny=16, max_iterations=40, pressure_solver=solver)
+        prob.settings.inner_tolerance = 1e-11
+        prob.settings.inner_iterations = 3000
+        res = _batch(prob).evaluate(np.asarray([[0.31, 0.09, 0.85, 0.5]]), fields=True)
+        out[solver] = (res, _fields(res, 0, prob.mesh.n_cells))
+    for name in ("Ux", "Uy", "p", "k", "omega", "flux"):
+        assert max_scaled(out["AMG"][1][name], out["PCG"][1][name]) <= 1e-9, name
+    assert out["AMG"][0].krylov_iterations[0][2] * 5 < out["PCG"][0].krylov_iterations[0][2]
+
+
+def test_converged_hill_amg_matches_reference():
+    """Converged 30x16 hills: the reference's AMG (Gauss-Seidel smoothing,
+    value-based aggregates) against the device AMG (Jacobi smoothing,
+    geometric aggregates) reach the same fixed point to the solve tolerance."""
+    d = load("ref_hill_small_converged.json")
+    prob = cases.hill_problem(nx=30, ny=16)
+    res = _batch(prob).evaluate(np.asarray(d["thetas"]))
+    for i, row in enumerate(d["rows"]):
+        assert res.statuses[i] == row["status"] == 0
+        assert abs(int(res.iteration_counts[i]) - row["iters"]) <= max(50, int(0.02 * row["iters"]))
+        assert max_rel(res.predictions[i], row["predictions"]) <= 1e-4
+
+
 def test_batch_results_independent_of_batch_composition():
     """Each row is bitwise the same alone, inside a batch and in another order
     (test_batch_determinism.cpp:49-63)."""

@@ -348,6 +348,13 @@ def test_symmetry_patch():
         assert np.array_equal(r[name], np.asarray(d["row"]["fields"][name])), name
 
 
+def test_converged_small_hill_amg():
+    """Converged 30x16 hill with the reference's AMG pressure solver."""
+    d = load("ref_hill_small_converged.json")
+    prob = cases.hill_problem(nx=30, ny=16)
+    _check_row(_oracle(prob).evaluate(prob.coefficients(d["thetas"][0])), d["rows"][0])
+
+
 def test_channel_draws_fixture():
     d = load("channel_theta64.json")
     assert d["seed"] == cases.CHANNEL_SEED
