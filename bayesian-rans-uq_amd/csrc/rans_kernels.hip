@@ -165,6 +165,10 @@ struct Draw {
     int nc, nif, nf, nb;
     bool planar, has_outlet;
     const int tid = threadIdx.x;
+    // cells, faces and matrix rows are distributed over gs threads; a thread's
+    // first index is gt (in one-workgroup mode: the workgroup's threads)
+    int gt = threadIdx.x, gs = BS;
+    DEV void sync() { __syncthreads(); }
     using D = typename Space<kLDS>::D;
     D *Cb, *Fb, *Bb;                            // record bases
     typename Space<kLDS>::I4 *slot4;            // [nc][2]: slot codes, neighbour cells
@@ -267,7 +271,7 @@ struct Draw {
     // applyVelocityBC / applyPressureBC / applyKBC / applyOmegaBC
     // (core/include/BoundaryCondition.hpp:279-384)
     DEV void bc_velocity() {
-        for (int b = tid; b < nb; b += BS) {
+        for (int b = gt; b < nb; b += gs) {
             const int f = nif + b, o = fon[f].x, fl = bflag[b];
             switch (bc_type(fl, 0)) {
             case BC_WALL_NOSLIP:
@@ -294,19 +298,19 @@ struct Draw {
         }
     }
     DEV void bc_pressure() {
-        for (int b = tid; b < nb; b += BS) {
+        for (int b = gt; b < nb; b += gs) {
             const int f = nif + b, t = bc_type(bflag[b], 1);
             Bd(b, B_p) = (t == BC_OUTLET || t == BC_DIRICHLET) ? Bd(b, B_pv) : C(fon[f].x, V_p);
         }
     }
     DEV void bc_k() {
-        for (int b = tid; b < nb; b += BS) {
+        for (int b = gt; b < nb; b += gs) {
             const int f = nif + b, t = bc_type(bflag[b], 2);
             Bd(b, B_k) = t == BC_WALL_KOMEGA ? 0.0 : (t == BC_DIRICHLET ? Bd(b, B_kv) : C(fon[f].x, V_k));
         }
     }
     DEV void bc_omega(double beta1) {
-        for (int b = tid; b < nb; b += BS) {
+        for (int b = gt; b < nb; b += gs) {
             const int f = nif + b, t = bc_type(bflag[b], 3);
             double v;
             if (t == BC_WALL_KOMEGA) {
@@ -764,7 +768,7 @@ struct Draw {
             res.converged = 1;
             return res;
         }
-        for (int ci = tid; ci < nc; ci += BS) {
+        for (int ci = gt; ci < nc; ci += gs) {
             C(ci, V_krh) = C(ci, V_kr);
             C(ci, V_kp) = 0.0;
             C(ci, V_kv) = 0.0;
@@ -774,14 +778,14 @@ struct Draw {
         for (int it = 0; it < maxIter; ++it) {
             if (fabs(rhoN) < 1e-30) break;
             const double beta = (rhoN / (rho + 1e-30)) * (alpha / (omega + 1e-30));
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double pv = C(ci, V_kr) + beta * (C(ci, V_kp) - omega * C(ci, V_kv));
                 C(ci, V_kp) = pv;
                 C(ci, V_kph) = jac(pv, C(ci, V_diag));
             }
-            __syncthreads();
+            sync();
             double a1[1] = {0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double vv = spmv(ci, V_kph);
                 C(ci, V_kv) = vv;
                 a1[0] += C(ci, V_krh) * vv;
@@ -791,7 +795,7 @@ struct Draw {
             if (fabs(rv) < 1e-30) break;
             alpha = rhoN / rv;
             double a2[1] = {0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double sv = C(ci, V_kr) - alpha * C(ci, V_kv);
                 C(ci, V_ks) = sv;
                 C(ci, V_ksh) = jac(sv, C(ci, V_diag));
@@ -800,14 +804,14 @@ struct Draw {
             red.sum(a2);
             const double sn = sqrt(a2[0]);
             if (sn / r0 < tol) {
-                for (int ci = tid; ci < nc; ci += BS) C(ci, V_kx) += alpha * C(ci, V_kph);
-                __syncthreads();
+                for (int ci = gt; ci < nc; ci += gs) C(ci, V_kx) += alpha * C(ci, V_kph);
+                sync();
                 res.iterations = it + 1;
                 res.converged = 1;
                 return res;
             }
             double a3[2] = {0.0, 0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double tv = spmv(ci, V_ksh);
                 C(ci, V_kt) = tv;
                 a3[0] += tv * tv;
@@ -817,7 +821,7 @@ struct Draw {
             const double tt = a3[0];
             omega = (tt > 1e-30) ? a3[1] / tt : 0.0;
             double a4[2] = {0.0, 0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 C(ci, V_kx) += alpha * C(ci, V_kph) + omega * C(ci, V_ksh);
                 const double rv2 = C(ci, V_ks) - omega * C(ci, V_kt);
                 C(ci, V_kr) = rv2;
@@ -850,7 +854,7 @@ struct Draw {
             return res;
         }
         double a0[1] = {0.0};
-        for (int ci = tid; ci < nc; ci += BS) {
+        for (int ci = gt; ci < nc; ci += gs) {
             const double zv = jac(C(ci, V_kr), C(ci, V_diag));
             C(ci, z) = zv;
             C(ci, pv) = zv;
@@ -860,7 +864,7 @@ struct Draw {
         double rz = a0[0];
         for (int it = 0; it < maxIter; ++it) {
             double a1[1] = {0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double av = spmv(ci, pv);
                 C(ci, Ap) = av;
                 a1[0] += C(ci, pv) * av;
@@ -870,7 +874,7 @@ struct Draw {
             if (fabs(pAp) < 1e-30) break;
             const double alpha = rz / pAp;
             double a2[2] = {0.0, 0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 C(ci, V_kx) += alpha * C(ci, pv);
                 const double rv = C(ci, V_kr) - alpha * C(ci, Ap);
                 C(ci, V_kr) = rv;
@@ -888,8 +892,8 @@ struct Draw {
             }
             const double rzNew = a2[1];
             const double beta = rzNew / (rz + 1e-30);
-            for (int ci = tid; ci < nc; ci += BS) C(ci, pv) = C(ci, z) + beta * C(ci, pv);
-            __syncthreads();
+            for (int ci = gt; ci < nc; ci += gs) C(ci, pv) = C(ci, z) + beta * C(ci, pv);
+            sync();
             rz = rzNew;
         }
         return res;
@@ -1366,7 +1370,7 @@ struct Draw {
         for (int l = 1; l < G.levels; ++l) {
             const AmgLevel &F = G.lev[l - 1], &L = G.lev[l];
             double *val = amgw + L.val;
-            for (int e = tid; e < L.nnz; e += BS) {
+            for (int e = gt; e < L.nnz; e += gs) {
                 double a = 0.0;
                 for (int q = F.gptr[e]; q < F.gptr[e + 1]; ++q) {
                     const int fe = F.gidx[q];
@@ -1374,44 +1378,44 @@ struct Draw {
                 }
                 val[e] = a;
             }
-            __syncthreads();
-            for (int i = tid; i < L.n; i += BS) {
+            sync();
+            for (int i = gt; i < L.n; i += gs) {
                 const double d = val[L.diagpos[i]];
                 amgw[L.dinv + i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
             }
         }
-        __syncthreads();
+        sync();
         // dense Gauss-Jordan inverse of the coarsest level (symmetric positive
         // definite: no pivoting); A in inv[0, n^2), its inverse after it
         const int n = G.coarse_n;
         const AmgLevel &L = G.lev[G.levels - 1];
         double *A = amgw + G.inv, *X = A + n * n, *fac = X + n * n;
-        for (int e = tid; e < n * n; e += BS) {
+        for (int e = gt; e < n * n; e += gs) {
             A[e] = 0.0;
             X[e] = (e / n == e % n) ? 1.0 : 0.0;
         }
-        __syncthreads();
-        for (int i = tid; i < n; i += BS)
+        sync();
+        for (int i = gt; i < n; i += gs)
             for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q)
                 A[i * n + L.col[q]] = G.levels == 1 ? lev0_value(G.src0[q]) : amgw[L.val + q];
-        __syncthreads();
+        sync();
         for (int kk = 0; kk < n; ++kk) {
             const double piv = A[kk * n + kk];
             const double ip = fabs(piv) > 1e-300 ? 1.0 / piv : 1.0;
-            __syncthreads();
-            for (int j = tid; j < n; j += BS) {
+            sync();
+            for (int j = gt; j < n; j += gs) {
                 A[kk * n + j] *= ip;
                 X[kk * n + j] *= ip;
             }
-            for (int i = tid; i < n; i += BS) fac[i] = A[i * n + kk];
-            __syncthreads();
-            for (int e = tid; e < n * n; e += BS) {
+            for (int i = gt; i < n; i += gs) fac[i] = A[i * n + kk];
+            sync();
+            for (int e = gt; e < n * n; e += gs) {
                 const int i = e / n, j = e % n;
                 if (i == kk) continue;
                 A[e] -= fac[i] * A[kk * n + j];
                 X[e] -= fac[i] * X[kk * n + j];
             }
-            __syncthreads();
+            sync();
         }
     }
 
@@ -1432,87 +1436,87 @@ struct Draw {
         for (int l = 0; l < nl - 1; ++l) {
             const AmgLevel &L = G.lev[l];
             if (l == 0) {
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     const double d = C(ci, V_diag);
                     C(ci, V_krh) = fabs(d) > 1e-300 ? w * C(ci, V_kr) / d : 0.0;
                 }
-                __syncthreads();
-                for (int ci = tid; ci < nc; ci += BS) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
-                __syncthreads();
-                for (int ci = tid; ci < nc; ci += BS) {
+                sync();
+                for (int ci = gt; ci < nc; ci += gs) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+                sync();
+                for (int ci = gt; ci < nc; ci += gs) {
                     const double d = C(ci, V_diag);
                     if (fabs(d) > 1e-300) C(ci, V_krh) += w * C(ci, V_ks) / d;
                 }
-                __syncthreads();
-                for (int ci = tid; ci < nc; ci += BS) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+                sync();
+                for (int ci = gt; ci < nc; ci += gs) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
             } else {
                 double *x = amgw + L.x, *b = amgw + L.b, *r = amgw + L.r, *di = amgw + L.dinv;
-                for (int i = tid; i < L.n; i += BS) x[i] = w * b[i] * di[i];
-                __syncthreads();
-                for (int i = tid; i < L.n; i += BS) r[i] = b[i] - coarse_row(L, i, x);
-                __syncthreads();
-                for (int i = tid; i < L.n; i += BS) x[i] += w * r[i] * di[i];
-                __syncthreads();
-                for (int i = tid; i < L.n; i += BS) r[i] = b[i] - coarse_row(L, i, x);
+                for (int i = gt; i < L.n; i += gs) x[i] = w * b[i] * di[i];
+                sync();
+                for (int i = gt; i < L.n; i += gs) r[i] = b[i] - coarse_row(L, i, x);
+                sync();
+                for (int i = gt; i < L.n; i += gs) x[i] += w * r[i] * di[i];
+                sync();
+                for (int i = gt; i < L.n; i += gs) r[i] = b[i] - coarse_row(L, i, x);
             }
-            __syncthreads();
+            sync();
             const AmgLevel &Cl = G.lev[l + 1];
             double *bc = amgw + Cl.b;
-            for (int I = tid; I < Cl.n; I += BS) {
+            for (int I = gt; I < Cl.n; I += gs) {
                 double a = 0.0;
                 for (int q = L.mptr[I]; q < L.mptr[I + 1]; ++q)
                     a += l == 0 ? C(L.midx[q], V_ks) : amgw[L.r + L.midx[q]];
                 bc[I] = a;
             }
-            __syncthreads();
+            sync();
         }
         // coarsest: x = A^-1 b
         {
             const int n = G.coarse_n;
             const double *X = amgw + G.inv + n * n;
             if (nl == 1) {
-                for (int i = tid; i < n; i += BS) {
+                for (int i = gt; i < n; i += gs) {
                     double a = 0.0;
                     for (int j = 0; j < n; ++j) a += X[i * n + j] * C(j, V_kr);
                     C(i, V_krh) = a;
                 }
-                __syncthreads();
+                sync();
                 return;
             }
             const AmgLevel &L = G.lev[nl - 1];
             double *x = amgw + L.x, *b = amgw + L.b;
-            for (int i = tid; i < n; i += BS) {
+            for (int i = gt; i < n; i += gs) {
                 double a = 0.0;
                 for (int j = 0; j < n; ++j) a += X[i * n + j] * b[j];
                 x[i] = a;
             }
-            __syncthreads();
+            sync();
         }
         // up: prolong, post-smooth
         for (int l = nl - 2; l >= 0; --l) {
             const AmgLevel &L = G.lev[l];
             const double *xc = amgw + G.lev[l + 1].x;
             if (l == 0) {
-                for (int ci = tid; ci < nc; ci += BS) C(ci, V_krh) += xc[L.agg[ci]];
-                __syncthreads();
+                for (int ci = gt; ci < nc; ci += gs) C(ci, V_krh) += xc[L.agg[ci]];
+                sync();
                 for (int sweep = 0; sweep < 2; ++sweep) {
-                    for (int ci = tid; ci < nc; ci += BS) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
-                    __syncthreads();
-                    for (int ci = tid; ci < nc; ci += BS) {
+                    for (int ci = gt; ci < nc; ci += gs) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+                    sync();
+                    for (int ci = gt; ci < nc; ci += gs) {
                         const double d = C(ci, V_diag);
                         if (fabs(d) > 1e-300) C(ci, V_krh) += w * C(ci, V_ks) / d;
                     }
-                    __syncthreads();
+                    sync();
                 }
             } else {
                 double *x = amgw + L.x, *b = amgw + L.b, *r = amgw + L.r, *di = amgw + L.dinv;
-                for (int i = tid; i < L.n; i += BS) x[i] += xc[L.agg[i]];
-                __syncthreads();
+                for (int i = gt; i < L.n; i += gs) x[i] += xc[L.agg[i]];
+                sync();
                 for (int sweep = 0; sweep < 2; ++sweep) {
-                    for (int i = tid; i < L.n; i += BS) r[i] = b[i] - coarse_row(L, i, x);
-                    __syncthreads();
-                    for (int i = tid; i < L.n; i += BS) x[i] += w * r[i] * di[i];
-                    __syncthreads();
+                    for (int i = gt; i < L.n; i += gs) r[i] = b[i] - coarse_row(L, i, x);
+                    sync();
+                    for (int i = gt; i < L.n; i += gs) x[i] += w * r[i] * di[i];
+                    sync();
                 }
             }
         }
@@ -1530,7 +1534,7 @@ struct Draw {
         amg_refresh();
         amg_vcycle();
         double a0[1] = {0.0};
-        for (int ci = tid; ci < nc; ci += BS) {
+        for (int ci = gt; ci < nc; ci += gs) {
             C(ci, V_kp) = C(ci, V_krh);
             a0[0] += C(ci, V_kr) * C(ci, V_krh);
         }
@@ -1538,7 +1542,7 @@ struct Draw {
         double rz = a0[0];
         for (int it = 0; it < maxIter; ++it) {
             double a1[1] = {0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double av = spmv(ci, V_kp);
                 C(ci, V_kv) = av;
                 a1[0] += C(ci, V_kp) * av;
@@ -1548,7 +1552,7 @@ struct Draw {
             if (fabs(pAp) < 1e-30) break;
             const double alpha = rz / pAp;
             double a2[1] = {0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 C(ci, V_kx) += alpha * C(ci, V_kp);
                 const double rv = C(ci, V_kr) - alpha * C(ci, V_kv);
                 C(ci, V_kr) = rv;
@@ -1563,12 +1567,12 @@ struct Draw {
             }
             amg_vcycle();
             double a3[1] = {0.0};
-            for (int ci = tid; ci < nc; ci += BS) a3[0] += C(ci, V_kr) * C(ci, V_krh);
+            for (int ci = gt; ci < nc; ci += gs) a3[0] += C(ci, V_kr) * C(ci, V_krh);
             red.sum(a3);
             const double rzNew = a3[0];
             const double beta = rzNew / (rz + 1e-30);
-            for (int ci = tid; ci < nc; ci += BS) C(ci, V_kp) = C(ci, V_krh) + beta * C(ci, V_kp);
-            __syncthreads();
+            for (int ci = gt; ci < nc; ci += gs) C(ci, V_kp) = C(ci, V_krh) + beta * C(ci, V_kp);
+            sync();
             rz = rzNew;
         }
         return res;
@@ -1714,7 +1718,7 @@ struct Draw {
     // (SIMPLESolver.cpp:267-298).
     DEV void run() {
         const double nu = S.nu;
-        for (int ci = tid; ci < nc; ci += BS) {
+        for (int ci = gt; ci < nc; ci += gs) {
             C(ci, V_Ux) = S.init_ux;
             C(ci, V_Uy) = S.init_uy;
             C(ci, V_p) = S.init_p;
@@ -1748,31 +1752,31 @@ struct Draw {
             for (int e = 0; e < kPhases; ++e) bk->cyc[e] = 0;
             bk->tlast = clock64();
         }
-        __syncthreads();
+        sync();
         bc_velocity();
         bc_pressure();
         bc_k();
         bc_omega(0.075);
-        __syncthreads();
-        for (int f = tid; f < nf; f += BS) Fc(f, W_phi) = interp_flux(f);
-        __syncthreads();
+        sync();
+        for (int f = gt; f < nf; f += gs) Fc(f, W_phi) = interp_flux(f);
+        sync();
 
         const double tol = S.convergence_tol;
         const double negligible = 0.01 * tol;
         int turbEst = 0;
         for (int iter = 0; iter < S.max_iterations; ++iter) {
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 C(ci, V_UxP) = C(ci, V_Ux);
                 C(ci, V_UyP) = C(ci, V_Uy);
             }
-            for (int f = tid; f < nif; f += BS) Fc(f, W_phiBar) = interp_flux(f);
+            for (int f = gt; f < nif; f += gs) Fc(f, W_phiBar) = interp_flux(f);
             const bool turbActive = iter >= S.turb_start_iter;
             const bool turbUpdate =
                 turbActive && ((iter - S.turb_start_iter) % S.turb_update_interval == 0);
             if (turbUpdate) {
                 if (!turbEst && iter >= S.turb_start_iter + S.nut_floor_iters) turbEst = 1;
                 const double nuTMin = turbEst ? 0.0 : 0.1 * nu;
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     const int ph5[5] = {V_Ux, V_Uy, V_k, V_w, V_p};
                     const int pb5[5] = {B_Ux, B_Uy, B_k, B_w, B_p};
                     double gx[5], gy[5];
@@ -1807,7 +1811,7 @@ struct Draw {
                     C(ci, V_nuT) = smax(nt, nuTMin);
                 }
             } else {
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     const int ph3[3] = {V_Ux, V_Uy, V_p};
                     const int pb3[3] = {B_Ux, B_Uy, B_p};
                     double gx[3], gy[3];
@@ -1820,7 +1824,7 @@ struct Draw {
                     C(ci, V_gPy) = gy[2];
                 }
             }
-            __syncthreads();
+            sync();
             mark(0);
 
             // ---- x momentum (SIMPLESolver.cpp:1342-1379)
@@ -1828,7 +1832,7 @@ struct Draw {
             Krylov rUx, rUy;
             {
                 double a[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     double aPu;
                     bool ok;
                     const double r = momentum_row(ci, 0, aPu, ok);
@@ -1850,13 +1854,13 @@ struct Draw {
                     break;
                 }
                 rUx = solve(S.momentum_solver, a[3], false);
-                for (int ci = tid; ci < nc; ci += BS) C(ci, V_Ux) = C(ci, V_kx);
-                __syncthreads();
+                for (int ci = gt; ci < nc; ci += gs) C(ci, V_Ux) = C(ci, V_kx);
+                sync();
                 mark(2);
             }
             // ---- y momentum: its transpose stress sees the new Ux
             {
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     const int ph1[1] = {V_Ux};
                     const int pb1[1] = {B_Ux};
                     double gx[1], gy[1];
@@ -1864,9 +1868,9 @@ struct Draw {
                     C(ci, V_gUxx) = gx[0];
                     C(ci, V_gUxy) = gy[0];
                 }
-                __syncthreads();
+                sync();
                 double a[3] = {0.0, 0.0, 0.0};
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     double aPu;
                     bool ok;
                     const double r = momentum_row(ci, 1, aPu, ok);
@@ -1884,18 +1888,18 @@ struct Draw {
                     break;
                 }
                 rUy = solve(S.momentum_solver, a[2], false);
-                for (int ci = tid; ci < nc; ci += BS) C(ci, V_Uy) = C(ci, V_kx);
-                __syncthreads();
+                for (int ci = gt; ci < nc; ci += gs) C(ci, V_Uy) = C(ci, V_kx);
+                sync();
                 mark(4);
             }
             bc_velocity();
-            __syncthreads();
+            sync();
 
             // ---- predicted Rhie-Chow flux and the pressure correction
             //      (SIMPLESolver.cpp:1410-1432, 717-830, 870-884)
             const double relaxCorr = 1.0 - S.alpha_u;
             double ms[1] = {0.0};
-            for (int f = tid; f < nf; f += BS) {
+            for (int f = gt; f < nf; f += gs) {
                 double m = interp_flux(f);
                 double coeff = 0.0;
                 if (f < nif) {
@@ -1923,9 +1927,9 @@ struct Draw {
                 Fc(f, W_pCoeff) = coeff;
                 ms[0] += fabs(m);
             }
-            __syncthreads();
+            sync();
             double a8[3] = {0.0, 0.0, 0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 int code[kSlots], nbv[kSlots];
                 slots(ci, code, nbv);
                 double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
@@ -1971,12 +1975,12 @@ struct Draw {
             }
             const Krylov rP = solve(S.pressure_solver, a8[1], true);
             mark(6);
-            for (int ci = tid; ci < nc; ci += BS) C(ci, V_pP) = C(ci, V_kx);
-            __syncthreads();
-            for (int b = tid; b < nb; b += BS) Bd(b, B_pP) = (bflag[b] & 2) ? 0.0 : C(fon[nif + b].x, V_pP);
-            __syncthreads();
+            for (int ci = gt; ci < nc; ci += gs) C(ci, V_pP) = C(ci, V_kx);
+            sync();
+            for (int b = gt; b < nb; b += gs) Bd(b, B_pP) = (bflag[b] & 2) ? 0.0 : C(fon[nif + b].x, V_pP);
+            sync();
             // ---- correctVelocity / correctPressure (SIMPLESolver.cpp:916-940)
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const int ph1[1] = {V_pP};
                 const int pb1[1] = {B_pP};
                 double gx[1], gy[1];
@@ -1988,28 +1992,28 @@ struct Draw {
                 C(ci, V_Uy) -= rAP * gy[0] * V;
                 C(ci, V_p) += S.alpha_p * C(ci, V_pP);
             }
-            __syncthreads();
+            sync();
             bc_velocity();
             bc_pressure();
-            for (int f = tid; f < nf; f += BS) {
+            for (int f = gt; f < nf; f += gs) {
                 const v2i on = fon[f];
                 if (f >= nif) Fc(f, W_phi) = Fc(f, W_phiPred) + Fc(f, W_pCoeff) * C(on.x, V_pP);
                 else Fc(f, W_phi) = Fc(f, W_phiPred) - Fc(f, W_pCoeff) * (C(on.y, V_pP) - C(on.x, V_pP));
             }
-            __syncthreads();
+            sync();
             mark(7);
 
             // ---- turbulence transport (SIMPLESolver.cpp:1476-1538)
             if (turbUpdate) {
                 double a9[2] = {0.0, 0.0};
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     bool ok;
                     k_row(ci, ok);
                     C(ci, V_kx) = C(ci, V_k);
                     a9[1] += ok ? 0.0 : 1.0;
                 }
-                __syncthreads();
-                for (int ci = tid; ci < nc; ci += BS) {
+                sync();
+                for (int ci = gt; ci < nc; ci += gs) {
                     const double r = C(ci, V_src) - spmv(ci, V_kx);
                     C(ci, V_kr) = r;
                     a9[0] += r * r;
@@ -2023,24 +2027,24 @@ struct Draw {
                 const Krylov rK = solve(S.turbulence_solver, a9[0], false);
                 mark(9);
                 double mk[2] = {1e-30, 0.0};
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     const double kn = smax(S.k_min, smin(S.k_max, C(ci, V_kx)));
                     mk[0] = smax(mk[0], fabs(kn));
                     mk[1] = smax(mk[1], fabs(kn - C(ci, V_k)));
                     C(ci, V_k) = kn;
                 }
-                __syncthreads();
+                sync();
                 bc_k();
-                __syncthreads();
+                sync();
                 double a10[2] = {0.0, 0.0};
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     bool ok;
                     const double x0 = omega_row(ci, ok);
                     C(ci, V_kx) = x0;
                     a10[1] += ok ? 0.0 : 1.0;
                 }
-                __syncthreads();
-                for (int ci = tid; ci < nc; ci += BS) {
+                sync();
+                for (int ci = gt; ci < nc; ci += gs) {
                     const double r = C(ci, V_src) - spmv(ci, V_kx);
                     C(ci, V_kr) = r;
                     a10[0] += r * r;
@@ -2054,7 +2058,7 @@ struct Draw {
                 const Krylov rW = solve(S.turbulence_solver, a10[0], false);
                 mark(11);
                 double mw[2] = {1e-30, 0.0};
-                for (int ci = tid; ci < nc; ci += BS) {
+                for (int ci = gt; ci < nc; ci += gs) {
                     double wn = smax(S.omega_min, smin(1e15, C(ci, V_kx)));
                     const double wvc = C(ci, V_wallv);
                     if (isfinite(wvc)) wn = wvc;
@@ -2062,7 +2066,7 @@ struct Draw {
                     mw[1] = smax(mw[1], fabs(wn - C(ci, V_w)));
                     C(ci, V_w) = wn;
                 }
-                __syncthreads();
+                sync();
                 bc_omega(c[2]);
                 double m4[4] = {mk[0], mk[1], mw[0], mw[1]};
                 red.max(m4);
@@ -2079,7 +2083,7 @@ struct Draw {
 
             // ---- convergence bookkeeping (SIMPLESolver.cpp:1545-1705)
             double mu[2] = {1e-300, 0.0};
-            for (int ci = tid; ci < nc; ci += BS) {
+            for (int ci = gt; ci < nc; ci += gs) {
                 const double ux = C(ci, V_Ux), uy = C(ci, V_Uy);
                 mu[0] = smax(mu[0], sqrt(ux * ux + uy * uy));
                 const double dx = ux - C(ci, V_UxP), dy = uy - C(ci, V_UyP);
@@ -2110,7 +2114,7 @@ struct Draw {
             bk->stalled = isfinite(bk->rate) && bk->rate >= 1.0;
             bk->status = ST_UNCONVERGED;
         }
-        __syncthreads();
+        sync();
     }
 
     // The convergence decision of SIMPLESolver::solve (SIMPLESolver.cpp:
