@@ -60,7 +60,7 @@ class RqSettings(C.Structure):
                 ("omega_min", C.c_double), ("pressure_solver", C.c_int32),
                 ("momentum_solver", C.c_int32), ("turbulence_solver", C.c_int32),
                 ("sst_variant", C.c_int32), ("mixed_precision", C.c_int32),
-                ("block_size", C.c_int32), ("profile", C.c_int32)]
+                ("block_size", C.c_int32), ("profile", C.c_int32), ("grid_blocks", C.c_int32)]
 
 
 class RqParams(C.Structure):
@@ -116,7 +116,7 @@ def load_library(path: str = LIB_PATH):
                  "rq_engine_n_faces"):
         getattr(lib, name).argtypes = [C.c_void_p]
         getattr(lib, name).restype = C.c_int32
-    lib.rq_engine_last_timing.argtypes = [C.c_void_p, _dp, _ip, _ip, _i64p]
+    lib.rq_engine_last_timing.argtypes = [C.c_void_p, _dp, _ip, _ip, _i64p, _ip]
     lib.rq_engine_last_timing.restype = C.c_int
     lib.rq_engine_destroy.argtypes = [C.c_void_p]
     lib.rq_engine_destroy.restype = None
@@ -142,7 +142,7 @@ class _Keep:
 
 
 def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size: int = 0,
-                    profile: bool = False):
+                    profile: bool = False, grid_blocks: int = 0):
     r = RqSettings()
     r.max_iterations, r.convergence_tol, r.divergence_limit = s.max_iterations, \
         s.convergence_tol, s.divergence_limit
@@ -162,6 +162,7 @@ def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size
     r.mixed_precision = 1 if mixed_precision else 0
     r.block_size = block_size
     r.profile = 1 if profile else 0
+    r.grid_blocks = grid_blocks
     return r
 
 
@@ -279,7 +280,8 @@ class ForwardBatch:
                  obs_op: ObservationOperator, bcs: FlowBoundaryConditions, nu: float,
                  settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
-                 mixed_precision: bool = False, block_size: int = 0, profile: bool = False):
+                 mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
+                 grid_blocks: int = 0):
         settings = settings if settings is not None else SolverSettings()
         settings.validate()
         param_set.validate()
@@ -311,7 +313,7 @@ class ForwardBatch:
         p.base = keep.d(param_set.base)
         p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
         o = obs_struct(prepared, keep)
-        s = settings_struct(settings, mixed_precision, block_size, profile)
+        s = settings_struct(settings, mixed_precision, block_size, profile, grid_blocks)
         init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
         err = C.create_string_buffer(512)
         h = self.lib.rq_engine_create(C.byref(m), C.byref(p), C.byref(o), C.byref(b), float(nu),
@@ -334,10 +336,11 @@ class ForwardBatch:
 
     def last_timing(self):
         ms, bs, lds, nbytes = C.c_double(), C.c_int32(), C.c_int32(), C.c_int64()
+        gb = C.c_int32()
         self.lib.rq_engine_last_timing(self._h, C.byref(ms), C.byref(bs), C.byref(lds),
-                                       C.byref(nbytes))
+                                       C.byref(nbytes), C.byref(gb))
         return {"kernel_ms": ms.value, "block_size": bs.value, "arena_in_lds": bool(lds.value),
-                "arena_bytes": nbytes.value}
+                "arena_bytes": nbytes.value, "grid_blocks": gb.value}
 
     def evaluate(self, theta, workers: int = 1, fields: bool = False) -> ForwardBatchResult:
         theta = np.asarray(theta, dtype=np.float64)

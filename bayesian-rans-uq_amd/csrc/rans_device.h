@@ -18,7 +18,20 @@ enum Scheme { CONV_UPWIND = 0, CONV_LINEAR_UPWIND = 1, CONV_LIMITED = 2 };
 enum Status { ST_CONVERGED = 0, ST_UNCONVERGED, ST_DIVERGED, ST_INVALID, ST_UNKNOWN };
 enum ObsKind { OBS_VELOCITY = 0, OBS_PRESSURE = 1, OBS_SKIN_FRICTION = 2 };
 // device error codes (mapped to the reference's exception texts on the host)
-enum DevError { ERR_NONE = 0, ERR_NONFINITE_SYSTEM = 1, ERR_NONFINITE_MONITOR = 2 };
+enum DevError { ERR_NONE = 0, ERR_NONFINITE_SYSTEM = 1, ERR_NONFINITE_MONITOR = 2,
+                ERR_GRID_TIMEOUT = 3 };
+
+// Multi-workgroup draws: the workgroups of one draw meet at grid barriers on
+// this per-draw block (device memory, zeroed before every launch) and
+// exchange their reduction partials through it.
+constexpr int kMaxGridBlocks = 64;
+constexpr int kGridBlock = 256;      // threads per workgroup of a multi-workgroup draw
+constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
+struct GridState {
+    unsigned count, gen;  // barrier arrivals and generation
+    int abort, pad;       // 1: a barrier wait timed out; every workgroup leaves
+    double partials[2][kMaxGridBlocks][4];
+};
 
 // Per-draw arena, array-of-structures: every cell, face and boundary face is
 // one record of doubles with an odd stride (so consecutive records start in
@@ -119,6 +132,7 @@ struct DevSettings {
     double init_ux, init_uy, init_p, init_k, init_w;
     int mixed_precision;   // 1: FP32 Krylov with FP64 defect correction
     int profile;           // 1: accumulate per-phase clock counts in DevResult::cycles
+    int grid_blocks;       // workgroups per draw (1: one-workgroup draws)
 };
 
 // One draw to solve: its mesh, its arena and its coefficients.

@@ -28,6 +28,11 @@ hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                         unsigned long long *head, double *arenas, int64_t arena_stride,
                         double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
                         hipStream_t stream);
+int grid_co_resident(size_t lds_bytes);
+hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,
+                             DevResult *results, double *pred, int pred_stride, double *fields,
+                             int64_t field_stride, GridState *states, size_t lds_bytes,
+                             hipStream_t stream);
 }
 
 using namespace rq;
@@ -92,6 +97,10 @@ struct rq_engine {
     DevSlot *d_slots = nullptr;
     DevResult *d_results = nullptr;
     double *d_pred = nullptr, *d_fields = nullptr, *d_arena = nullptr, *d_amg = nullptr;
+    GridState *d_grid = nullptr;
+    size_t cap_grid = 0;
+    int cus = 0;
+    int last_nblk = 1;
     size_t cap_slots = 0, cap_results = 0, cap_pred = 0, cap_fields = 0, cap_arena = 0;
     size_t cap_amg = 0;
     double last_ms = 0.0;
@@ -105,6 +114,7 @@ struct rq_engine {
         if (d_fields) (void)hipFree(d_fields);
         if (d_arena) (void)hipFree(d_arena);
         if (d_amg) (void)hipFree(d_amg);
+        if (d_grid) (void)hipFree(d_grid);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -151,6 +161,10 @@ void validate_settings(const rq_settings &s) {
     if (s.mixed_precision && s.block_size == 512)
         throw Fail{RQ_EINVAL, "mixed precision runs with block sizes up to 256"};
     if (s.profile != 0 && s.profile != 1) throw Fail{RQ_EINVAL, "profile must be 0 or 1"};
+    if (s.grid_blocks < 0 || s.grid_blocks > kMaxGridBlocks)
+        throw Fail{RQ_EINVAL, "grid_blocks must be in [0, 64]"};
+    if (s.grid_blocks > 1 && s.mixed_precision)
+        throw Fail{RQ_EINVAL, "mixed precision runs with one workgroup per draw"};
     if (s.block_size != 0 && s.block_size != 64 && s.block_size != 128 && s.block_size != 256 &&
         s.block_size != 512)
         throw Fail{RQ_EINVAL, "block_size must be 0, 64, 128, 256 or 512"};
@@ -802,6 +816,8 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.init_w = init[4];
         S.mixed_precision = s.mixed_precision;
         S.profile = s.profile;
+        S.grid_blocks = s.grid_blocks;
+        HIPCHECK(hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device));
         if (s.mixed_precision && nc > 256)
             throw Fail{RQ_EINVAL, "mixed precision is available for meshes of up to 256 cells"};
         if (s.block_size) E->block = s.block_size;
@@ -851,9 +867,9 @@ struct RowFinal {
 RowFinal finalize_row(const rq_engine *E, const DevResult &d, const double *pred) {
     RowFinal f{d.status, -1e6, false, std::string()};
     if (f.status == ST_UNKNOWN)
-        f.msg = d.error == ERR_NONFINITE_MONITOR
-                    ? "convergence monitor returned a nonfinite observation"
-                    : "linear system coefficients and source must be finite";
+        f.msg = d.error == ERR_NONFINITE_MONITOR ? "convergence monitor returned a nonfinite observation"
+                : d.error == ERR_GRID_TIMEOUT   ? "device synchronisation of the draw timed out"
+                                                : "linear system coefficients and source must be finite";
     if (f.status == ST_CONVERGED || f.status == ST_UNCONVERGED) {
         double acc = 0.0;
         for (int j = 0; j < E->nobs; ++j) {
@@ -960,12 +976,47 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         grow(E->d_amg, E->cap_amg, (size_t)ns * amgw);
         for (int i = 0; i < ns; ++i) slots[i].amg = E->d_amg + (size_t)i * amgw;
     }
+    // workgroups per draw: meshes whose arena lives in HBM spread each draw
+    // over several co-resident workgroups when the batch leaves CUs idle
+    int nblk = 1;
+    const size_t grid_lds =
+        (8 * (size_t)E->settings.convergence_window + 2 * (size_t)std::max(nobs, 1)) * 8;
+    // Automatic selection keeps one workgroup per draw: on MI355X every grid
+    // barrier of a multi-workgroup draw pays an agent-scope release/acquire
+    // (L2 writeback and invalidate), about 60-120 us per barrier measured on
+    // the 120x64 hill, which outweighs the parallelism for the meshes here.
+    if (E->settings.grid_blocks > 1) nblk = E->settings.grid_blocks;
+    if (nblk > 1) {
+        // every workgroup of the padded grid (draws spread over the XCDs) must
+        // be co-resident
+        const int co = grid_co_resident(grid_lds);
+        const int per_xcd = (ns + kXcds - 1) / kXcds;
+        const int fit = co / (kXcds * per_xcd);
+        nblk = std::min(nblk, fit);
+        if (nblk < 2) nblk = 1;
+    }
+    if (nblk > 1 && E->lds) {
+        // the grid mode keeps the arena in HBM
+        grow(E->d_arena, E->cap_arena, (size_t)ns * E->lay.stride);
+        for (int i = 0; i < ns; ++i) slots[i].arena = E->d_arena + (size_t)i * E->lay.stride;
+    }
+    E->last_nblk = nblk;
     HIPCHECK(hipMemcpyAsync(E->d_slots, slots.data(), ns * sizeof(DevSlot), hipMemcpyHostToDevice,
                             E->stream));
+    if (nblk > 1) {
+        grow(E->d_grid, E->cap_grid, ns);
+        HIPCHECK(hipMemsetAsync(E->d_grid, 0, ns * sizeof(GridState), E->stream));
+    }
     HIPCHECK(hipEventRecord(E->ev0, E->stream));
-    HIPCHECK(launch_simple_solve(E->block, E->lds, E->d_slots, ns, E->settings, E->d_results,
-                                 E->d_pred, std::max(nobs, 1), out->fields ? E->d_fields : nullptr,
-                                 fstride, E->lds_bytes, E->stream));
+    if (nblk > 1)
+        HIPCHECK(launch_grid_solve(E->d_slots, ns, nblk, E->settings, E->d_results, E->d_pred,
+                                   std::max(nobs, 1), out->fields ? E->d_fields : nullptr, fstride,
+                                   E->d_grid, grid_lds, E->stream));
+    else
+        HIPCHECK(launch_simple_solve(E->block, E->lds, E->d_slots, ns, E->settings, E->d_results,
+                                     E->d_pred, std::max(nobs, 1),
+                                     out->fields ? E->d_fields : nullptr, fstride, E->lds_bytes,
+                                     E->stream));
     HIPCHECK(hipEventRecord(E->ev1, E->stream));
     std::vector<DevResult> res(ns);
     std::vector<double> pred((size_t)ns * std::max(nobs, 1));
@@ -1209,11 +1260,12 @@ int32_t rq_engine_n_cells(const rq_engine *e) { return e ? e->nc : -1; }
 int32_t rq_engine_n_faces(const rq_engine *e) { return e ? e->nf : -1; }
 
 int rq_engine_last_timing(const rq_engine *e, double *kernel_ms, int32_t *block_size,
-                          int32_t *arena_in_lds, int64_t *arena_bytes) {
+                          int32_t *arena_in_lds, int64_t *arena_bytes, int32_t *grid_blocks) {
     if (!e) return RQ_EINVAL;
     if (kernel_ms) *kernel_ms = e->last_ms;
-    if (block_size) *block_size = e->block;
-    if (arena_in_lds) *arena_in_lds = e->lds ? 1 : 0;
+    if (block_size) *block_size = e->last_nblk > 1 ? kGridBlock : e->block;
+    if (arena_in_lds) *arena_in_lds = (e->lds && e->last_nblk == 1) ? 1 : 0;
+    if (grid_blocks) *grid_blocks = e->last_nblk;
     if (arena_bytes) *arena_bytes = e->lay.stride * (int64_t)sizeof(double);
     return RQ_OK;
 }

@@ -79,11 +79,60 @@ DEV float wave_sumf(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Grid barrier of the workgroups of one draw. Every wait is bounded: past
+// kGridSpinLimit polls the waiting workgroup raises abort, and every
+// workgroup then leaves every later barrier at once (the draw ends as
+// Unknown), so no wave can be left waiting.
+constexpr long long kGridSpinLimit = 1LL << 25;
+DEV bool grid_barrier(GridState *g, int nblk, int *flag) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            ok = 0;
+        } else {
+            const unsigned gen =
+                __hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned prev = __hip_atomic_fetch_add(&g->count, 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == (unsigned)nblk - 1) {
+                __hip_atomic_store(&g->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&g->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                long long spins = 0;
+                while (__hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                       gen) {
+                    if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        ok = 0;
+                        break;
+                    }
+                    if (++spins > kGridSpinLimit) {
+                        __hip_atomic_store(&g->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        *flag = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *flag != 0;
+}
+
 template <int BS>
 struct Reducer {
     static constexpr int NW = BS / 64;
     double *buf;  // LDS scratch [2][NW][4]
     int parity;
+    // multi-workgroup draws: block totals meet in the draw's GridState and
+    // every workgroup adds them in rank order (the same bits everywhere)
+    GridState *grid = nullptr;
+    int nblk = 1, rank = 0, gparity = 0;
+    int *flag = nullptr;
 
     template <int NV, bool MAX>
     DEV void run(double (&v)[NV]) {
@@ -105,6 +154,20 @@ struct Reducer {
             parity ^= 1;
         } else {
             __syncthreads();  // orders the phase's LDS writes before later reads
+        }
+        if (grid) {
+            double *pt = &grid->partials[gparity][0][0];
+            if (threadIdx.x == 0)
+#pragma unroll
+                for (int i = 0; i < NV; ++i) pt[rank * 4 + i] = v[i];
+            grid_barrier(grid, nblk, flag);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double a = pt[i];
+                for (int r = 1; r < nblk; ++r) a = MAX ? smax(a, pt[r * 4 + i]) : a + pt[r * 4 + i];
+                v[i] = a;
+            }
+            gparity ^= 1;
         }
     }
     template <int NV>
@@ -153,7 +216,7 @@ struct Krylov {
     double initial;
 };
 
-template <int BS, bool kLDS>
+template <int BS, bool kLDS, bool kGrid = false>
 struct Draw {
     const DevMesh &M;
     const DevSettings &S;
@@ -168,18 +231,37 @@ struct Draw {
     // cells, faces and matrix rows are distributed over gs threads; a thread's
     // first index is gt (in one-workgroup mode: the workgroup's threads)
     int gt = threadIdx.x, gs = BS;
-    DEV void sync() { __syncthreads(); }
+    // multi-workgroup draws: this workgroup's rank among nblk
+    int rank = 0, nblk = 1;
+    GridState *grid = nullptr;
+    int *gflag = nullptr;
+    DEV void sync() {
+        if constexpr (kGrid) {
+            if (!grid_barrier(grid, nblk, gflag) && tid == 0 && !bk->done) {
+                bk->done = 1;
+                bk->status = ST_UNKNOWN;
+                bk->error = ERR_GRID_TIMEOUT;
+            }
+        } else {
+            __syncthreads();
+        }
+    }
     using D = typename Space<kLDS>::D;
     D *Cb, *Fb, *Bb;                            // record bases
     typename Space<kLDS>::I4 *slot4;            // [nc][2]: slot codes, neighbour cells
     typename Space<kLDS>::I2 *fon;              // [nf]: owner, neighbour
     typename Space<kLDS>::I *bflag;             // [nb]
-    D *ringb, *q, *qprev, *Ob, *Onb;
+    // contraction rings and monitor values: per workgroup (LDS) in
+    // multi-workgroup draws, so each workgroup's bookkeeping replica is private
+    using DL = typename Space<kLDS || kGrid>::D;
+    DL *ringb, *q, *qprev;
+    D *Ob, *Onb;
     typename Space<kLDS>::I *obi, *onbi;  // probe ints, stencil cell/face pairs
     int nobs;
 
+    static DEV int nobs_of(const DevMesh &m) { return m.nobs; }
     DEV Draw(const DevMesh &m, const DevSettings &s, const double *coeffs, double *A, Book *b,
-             double *redbuf, Krylov *kr, double *amg)
+             double *redbuf, Krylov *kr, double *amg, double *aux = nullptr)
         : M(m), S(s), c(coeffs), bk(b), kres(kr), amgw(amg) {
         red.buf = redbuf;
         red.parity = 0;
@@ -197,14 +279,21 @@ struct Draw {
         slot4 = (typename Space<kLDS>::I4 *)(ib + L.islot);
         fon = (typename Space<kLDS>::I2 *)(ib + L.iface);
         bflag = (typename Space<kLDS>::I *)(ib + L.ibflag);
-        ringb = (D *)(A + L.ring);
+        if constexpr (kGrid) {
+            ringb = (DL *)aux;
+            q = ringb + 8 * (int64_t)S.convergence_window;
+            qprev = q + (nobs_of(M) > 0 ? nobs_of(M) : 1);
+        } else {
+            ringb = (DL *)(A + L.ring);
+            q = (DL *)(A + L.q);
+            qprev = (DL *)(A + L.qprev);
+        }
         Ob = (D *)(A + L.obs);
         Onb = (D *)(A + L.onb);
         obi = (typename Space<kLDS>::I *)(ib + L.iobs);
         onbi = (typename Space<kLDS>::I *)(ib + L.ionb);
         nobs = M.nobs;
-        q = (D *)(A + L.q);
-        qprev = (D *)(A + L.qprev);
+
     }
 
     // record accessors: cell, face and boundary-face variables
@@ -1386,37 +1475,42 @@ struct Draw {
         }
         sync();
         // dense Gauss-Jordan inverse of the coarsest level (symmetric positive
-        // definite: no pivoting); A in inv[0, n^2), its inverse after it
-        const int n = G.coarse_n;
-        const AmgLevel &L = G.lev[G.levels - 1];
-        double *A = amgw + G.inv, *X = A + n * n, *fac = X + n * n;
-        for (int e = gt; e < n * n; e += gs) {
-            A[e] = 0.0;
-            X[e] = (e / n == e % n) ? 1.0 : 0.0;
+        // definite: no pivoting); A in inv[0, n^2), its inverse after it. One
+        // workgroup computes it (rank 0 of a multi-workgroup draw).
+        if (rank == 0) {
+            const int n = G.coarse_n;
+            const AmgLevel &L = G.lev[G.levels - 1];
+            double *A = amgw + G.inv, *X = A + n * n, *fac = X + n * n;
+            for (int e = tid; e < n * n; e += BS) {
+                A[e] = 0.0;
+                X[e] = (e / n == e % n) ? 1.0 : 0.0;
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += BS)
+                for (int q2 = L.rowptr[i]; q2 < L.rowptr[i + 1]; ++q2)
+                    A[i * n + L.col[q2]] =
+                        G.levels == 1 ? lev0_value(G.src0[q2]) : amgw[L.val + q2];
+            __syncthreads();
+            for (int kk = 0; kk < n; ++kk) {
+                const double piv = A[kk * n + kk];
+                const double ip = fabs(piv) > 1e-300 ? 1.0 / piv : 1.0;
+                __syncthreads();
+                for (int j = tid; j < n; j += BS) {
+                    A[kk * n + j] *= ip;
+                    X[kk * n + j] *= ip;
+                }
+                for (int i = tid; i < n; i += BS) fac[i] = A[i * n + kk];
+                __syncthreads();
+                for (int e = tid; e < n * n; e += BS) {
+                    const int i = e / n, j = e % n;
+                    if (i == kk) continue;
+                    A[e] -= fac[i] * A[kk * n + j];
+                    X[e] -= fac[i] * X[kk * n + j];
+                }
+                __syncthreads();
+            }
         }
         sync();
-        for (int i = gt; i < n; i += gs)
-            for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q)
-                A[i * n + L.col[q]] = G.levels == 1 ? lev0_value(G.src0[q]) : amgw[L.val + q];
-        sync();
-        for (int kk = 0; kk < n; ++kk) {
-            const double piv = A[kk * n + kk];
-            const double ip = fabs(piv) > 1e-300 ? 1.0 / piv : 1.0;
-            sync();
-            for (int j = gt; j < n; j += gs) {
-                A[kk * n + j] *= ip;
-                X[kk * n + j] *= ip;
-            }
-            for (int i = gt; i < n; i += gs) fac[i] = A[i * n + kk];
-            sync();
-            for (int e = gt; e < n * n; e += gs) {
-                const int i = e / n, j = e % n;
-                if (i == kk) continue;
-                A[e] -= fac[i] * A[kk * n + j];
-                X[e] -= fac[i] * X[kk * n + j];
-            }
-            sync();
-        }
     }
 
     // y = A x on a coarse level (CSR)
@@ -1583,7 +1677,7 @@ struct Draw {
     DEV Krylov solve(int kind, double rr, bool zeroStart) {
         if (kind == SOLVER_AMG && zeroStart && M.amg.levels > 0)
             return amg_pcg(rr, S.inner_iterations, S.inner_tolerance);
-        if constexpr (BS <= 256) {
+        if constexpr (BS <= 256 && !kGrid) {
             if (nc <= 256) {
                 if (tid < 64) {
                     Krylov r;
@@ -1605,7 +1699,7 @@ struct Draw {
     // last 2m samples in an LDS ring (older samples never enter its rate or
     // remainder). Window maxima are taken across the lanes of wave 0; a NaN
     // sample counts as 0, as in the reference's max fold starting from 0.
-    DEV D *ring(int e) const { return ringb + e * 2 * S.convergence_window; }
+    DEV DL *ring(int e) const { return ringb + e * 2 * S.convergence_window; }
     DEV void ce_push(int e, double d) {
         const int m2 = 2 * S.convergence_window;
         const int p = bk->nEst[e] == 0 ? 0 : (bk->pos[e] + 1 == m2 ? 0 : bk->pos[e] + 1);
@@ -2261,36 +2355,57 @@ struct BlockShared {
     double coeffs[11];
     DevSettings sset;
     Krylov kslot;
+    int gflag;
 };
 
-template <int BS, bool kLDS>
+// rank/nblk/grid/aux describe a multi-workgroup draw (kGrid): its workgroups
+// share the arena and meet at grid barriers; aux is this workgroup's LDS for
+// its bookkeeping replica (contraction rings and monitor values).
+template <int BS, bool kLDS, bool kGrid = false>
 DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &settings,
                     const double *coeffSrc, double *arena, double *amg, DevResult *res,
-                    double *pred, double *fields) {
+                    double *pred, double *fields, GridState *grid = nullptr, int rank = 0,
+                    int nblk = 1, double *aux = nullptr) {
     if (threadIdx.x < 11) sh.coeffs[threadIdx.x] = coeffSrc[threadIdx.x];
     if (threadIdx.x == 0) sh.sset = settings;
+    const int g0 = rank * BS + threadIdx.x, gstride = nblk * BS;
     {
         const int64_t n16 = M.lay.image_bytes / 16;
         const uint4 *gsrc = reinterpret_cast<const uint4 *>(M.img);
         uint4 *dst = reinterpret_cast<uint4 *>(arena);
-        for (int64_t i = threadIdx.x; i < n16; i += BS) dst[i] = gsrc[i];
+        for (int64_t i = g0; i < n16; i += gstride) dst[i] = gsrc[i];
     }
-    __syncthreads();
-    Draw<BS, kLDS> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg);
+    if constexpr (kGrid) grid_barrier(grid, nblk, &sh.gflag);
+    else __syncthreads();
+    Draw<BS, kLDS, kGrid> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
+                            aux);
+    if constexpr (kGrid) {
+        d.rank = rank;
+        d.nblk = nblk;
+        d.grid = grid;
+        d.gflag = &sh.gflag;
+        d.gt = g0;
+        d.gs = gstride;
+        d.red.grid = grid;
+        d.red.nblk = nblk;
+        d.red.rank = rank;
+        d.red.flag = &sh.gflag;
+    }
     d.run();
     const Book &book = sh.book;
     const bool observe = book.status == ST_CONVERGED || book.status == ST_UNCONVERGED;
-    for (int i = threadIdx.x; i < M.nobs; i += BS) pred[i] = observe ? d.observe(i) : NAN;
+    if (rank == 0)
+        for (int i = threadIdx.x; i < M.nobs; i += BS) pred[i] = observe ? d.observe(i) : NAN;
     if (fields) {
         const int nc = M.nc;
         const int vars[10] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
         for (int a = 0; a < 10; ++a)
-            for (int ci = threadIdx.x; ci < nc; ci += BS)
+            for (int ci = g0; ci < nc; ci += gstride)
                 fields[a * nc + ci] = arena[ci * kNA + vars[a]];
-        for (int f = threadIdx.x; f < M.nf; f += BS)
+        for (int f = g0; f < M.nf; f += gstride)
             fields[10 * nc + f] = arena[M.lay.face + f * kNF + W_phi];
     }
-    if (threadIdx.x == 0) {
+    if (rank == 0 && threadIdx.x == 0) {
         DevResult &r = *res;
         r.status = book.status;
         r.error = book.error;
@@ -2321,6 +2436,33 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
     solve_draw<BS, kLDS>(sh, *slot.mesh, settings, slot.coeffs, kLDS ? smem : slot.arena,
                          slot.amg, results + row, predictions + (int64_t)row * pred_stride,
                          fields ? fields + row * field_stride : nullptr);
+}
+
+// The multi-workgroup batch kernel: nblk co-resident workgroups per draw
+// (launched cooperatively), rank = blockIdx.x % nblk.
+template <int BS>
+__global__ void __launch_bounds__(BS) grid_solve_kernel(const DevSlot *__restrict__ slots,
+                                                        DevSettings settings,
+                                                        DevResult *__restrict__ results,
+                                                        double *__restrict__ predictions,
+                                                        int pred_stride, double *__restrict__ fields,
+                                                        int64_t field_stride, GridState *states,
+                                                        int nblk, int nslots) {
+    extern __shared__ double smem[];
+    __shared__ BlockShared<BS> sh;
+    // Workgroups are dispatched round-robin over the XCDs (workgroup b on XCD
+    // b % kXcds). A draw's workgroups are taken from one XCD, so its barriers
+    // and shared arena stay within that XCD's L2: draw x + kXcds * t owns the
+    // workgroups x + kXcds * j with t * nblk <= j < (t + 1) * nblk.
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    const int draw = x + kXcds * (j / nblk), rank = j % nblk;
+    if (draw >= nslots) return;  // padding workgroup of an XCD with fewer draws
+    const DevSlot &slot = slots[draw];
+    const int row = slot.row;
+    solve_draw<BS, false, true>(sh, *slot.mesh, settings, slot.coeffs, slot.arena, slot.amg,
+                                results + row, predictions + (int64_t)row * pred_stride,
+                                fields ? fields + row * field_stride : nullptr, states + draw,
+                                rank, nblk, smem);
 }
 
 // ----------------------------------------------------------------------------
@@ -2438,6 +2580,38 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
 #undef RQ_CASE
 }
 
+
+int grid_co_resident(size_t lds_bytes) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    auto kern = grid_solve_kernel<kGridBlock>;
+    if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kGridBlock, lds_bytes) !=
+        hipSuccess)
+        return 0;
+    return per * cus;
+}
+
+hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,
+                             DevResult *results, double *pred, int pred_stride, double *fields,
+                             int64_t field_stride, GridState *states, size_t lds_bytes,
+                             hipStream_t stream) {
+    auto kern = grid_solve_kernel<kGridBlock>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    DevSettings sc = s;
+    void *args[] = {(void *)&slots, (void *)&sc, (void *)&results, (void *)&pred,
+                    (void *)&pred_stride, (void *)&fields, (void *)&field_stride,
+                    (void *)&states, (void *)&nblk, (void *)&nslots};
+    const int per_xcd = (nslots + kXcds - 1) / kXcds;  // draws on the fullest XCD
+    return hipLaunchCooperativeKernel((const void *)kern, dim3(kXcds * per_xcd * nblk),
+                                      dim3(kGridBlock), args, (unsigned)lds_bytes, stream);
+}
 
 hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                         const ServeRequest *ring, ServeOutput *outs, int ring_size,

@@ -74,6 +74,8 @@ typedef struct rq_settings {
     int32_t mixed_precision; /* 1: FP32 Krylov vectors, FP64 defect correction */
     int32_t block_size;      /* threads per draw; 0 = tuned default */
     int32_t profile;         /* 1: per-phase clock counts in rq_batch_out.phase_cycles */
+    int32_t grid_blocks;     /* workgroups per draw: 0 or 1 = one workgroup, 2..64 = that many
+                                co-resident workgroups (grid barriers) */
 } rq_settings;
 
 /* InferenceParameterSet (core/include/InferenceParameters.hpp:18-188). */
@@ -138,9 +140,10 @@ int32_t rq_engine_n_cells(const rq_engine *engine);
 int32_t rq_engine_n_faces(const rq_engine *engine);
 
 /* Device time of the last evaluate's solve kernel in milliseconds (HIP
- * events on the engine's stream) and its launch shape. */
+ * events on the engine's stream) and its launch shape (workgroup size,
+ * arena placement and size, workgroups per draw). */
 int rq_engine_last_timing(const rq_engine *engine, double *kernel_ms, int32_t *block_size,
-                          int32_t *arena_in_lds, int64_t *arena_bytes);
+                          int32_t *arena_in_lds, int64_t *arena_bytes, int32_t *grid_blocks);
 
 void rq_engine_destroy(rq_engine *engine);
 

@@ -249,6 +249,35 @@ def test_converged_hill_amg_matches_reference():
         assert max_rel(res.predictions[i], row["predictions"]) <= 1e-4
 
 
+@pytest.mark.parametrize("solver", ["PCG", "AMG"])
+def test_multi_workgroup_draws_match_single_workgroup(solver):
+    """Four workgroups per draw (grid barriers, rank-ordered reductions)
+    against one workgroup per draw on the 30x16 hill, 400 capped iterations."""
+    prob = small_hill(solver)
+    theta = np.asarray([[0.31, 0.09, 0.85, 0.5], [0.3, 0.095, 0.9, 0.55]])
+    one = _batch(prob, grid_blocks=1).evaluate(theta, fields=True)
+    b4 = _batch(prob, grid_blocks=4)
+    four = b4.evaluate(theta, fields=True)
+    assert b4.last_timing()["grid_blocks"] == 4
+    nc = prob.mesh.n_cells
+    for i in range(len(theta)):
+        assert four.statuses[i] == one.statuses[i] and four.iteration_counts[i] == one.iteration_counts[i]
+        assert max_rel(four.predictions[i], one.predictions[i]) <= REL_OBS
+        f4, f1 = _fields(four, i, nc), _fields(one, i, nc)
+        for name in ("Ux", "Uy", "p", "k", "omega"):
+            assert max_scaled(f4[name], f1[name]) <= 1e-6, name
+
+
+def test_multi_workgroup_converged_hill_matches_reference():
+    d = load("ref_hill_small_converged.json")
+    prob = cases.hill_problem(nx=30, ny=16)
+    res = _batch(prob, grid_blocks=8).evaluate(np.asarray(d["thetas"]))
+    for i, row in enumerate(d["rows"]):
+        assert res.statuses[i] == row["status"] == 0
+        assert abs(int(res.iteration_counts[i]) - row["iters"]) <= max(50, int(0.02 * row["iters"]))
+        assert max_rel(res.predictions[i], row["predictions"]) <= 1e-4
+
+
 def test_batch_results_independent_of_batch_composition():
     """Each row is bitwise the same alone, inside a batch and in another order
     (test_batch_determinism.cpp:49-63)."""
