@@ -76,6 +76,36 @@ def channel_draws(n: int = 64, seed: int = CHANNEL_SEED) -> np.ndarray:
     return mu + scale * z * np.sqrt(4.0 / chi2)
 
 
+def channel_ratio(ny: int, first: float = 0.05 / (1.05 ** 64 - 1)) -> float:
+    """Geometric ratio that gives a ny-cell channel (ny/2 cells per half
+    height h = 1) the first-cell height of the 128-cell, ratio-1.05 mesh."""
+    half = ny // 2
+    lo, hi = 1.0 + 1e-12, 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first * (mid ** half - 1.0) / (mid - 1.0) > 1.0:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)
+
+
+SERVED_CHANNELS = (64, 128, 256, 512)
+SERVED_SEED = 4242
+
+
+def served_stream(n: int, rate: float, meshes=SERVED_CHANNELS, seed: int = SERVED_SEED):
+    """Seeded Poisson arrivals of single-solve requests: (arrival time s,
+    mesh key, theta) with the mesh drawn uniformly from `meshes` and theta
+    uniform over a1 in [0.28, 0.34], beta* in [0.084, 0.096]
+    (docs/figures/calibration_coverage.py:51)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    which = rng.integers(0, len(meshes), size=n)
+    theta = np.column_stack([rng.uniform(0.28, 0.34, size=n), rng.uniform(0.084, 0.096, size=n)])
+    return [(float(t[i]), meshes[which[i]], theta[i]) for i in range(n)]
+
+
 def hill_problem(nx: int = 120, ny: int = 64, tol: float = 1e-4, max_iterations: int = 25000,
                  pressure_solver: str = "AMG") -> Problem:
     mesh = make_hill_mesh(nx, ny)
