@@ -28,8 +28,9 @@ constexpr int kMaxGridBlocks = 64;
 constexpr int kGridBlock = 256;      // threads per workgroup of a multi-workgroup draw
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
 struct GridState {
-    unsigned count, gen;  // barrier arrivals and generation
-    int abort, pad;       // 1: a barrier wait timed out; every workgroup leaves
+    unsigned long long count;  // barrier arrivals since launch (never reset)
+    int abort, pad;            // 1: a barrier wait timed out; every workgroup leaves
+    int xcc[kMaxGridBlocks];   // XCD of each workgroup of the draw
     double partials[2][kMaxGridBlocks][4];
 };
 
@@ -106,7 +107,12 @@ struct AmgMesh {
 
 // Mesh and problem tables in device memory, shared by every draw on the mesh.
 struct DevMesh {
-    const double *img;  // pre-filled arena image (geometry, zero state, int tables)
+    // pre-filled arena image (geometry, zero state, int tables), twice: cell,
+    // face and boundary records as records (img, for arenas in LDS) and as
+    // one array per variable (img_soa, for arenas in HBM, where a wave's
+    // loads of one variable of 64 consecutive cells then share cache lines)
+    const double *img;
+    const double *img_soa;
     int nc, nif, nf, nb, npatch;
     int has_outlet, planar;
     const double *wall_y;        // [nc] wall-face distance of the constrained cell, NaN otherwise

@@ -711,6 +711,20 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         Blob B;
         DevMesh D{};
         const size_t o_img = B.add(image);
+        // the same image with the cell, face and boundary records stored as
+        // one array per variable, for arenas in HBM
+        std::vector<double> soa(image);
+        const int nbs = std::max(nb, 1);
+        for (int c = 0; c < nc; ++c)
+            for (int v = 0; v < kNA; ++v)
+                soa[(int64_t)v * nc + c] = image[(int64_t)c * kNA + v];
+        for (int f = 0; f < nf; ++f)
+            for (int v = 0; v < kNF; ++v)
+                soa[L.face + (int64_t)v * nf + f] = image[L.face + (int64_t)f * kNF + v];
+        for (int b = 0; b < nbs; ++b)
+            for (int v = 0; v < kNB; ++v)
+                soa[L.bnd + (int64_t)v * nbs + b] = image[L.bnd + (int64_t)b * kNB + v];
+        const size_t o_soa = B.add(soa);
         size_t o_wy = B.add(wall_y);
         struct LevOff {
             size_t rowptr, col, diagpos, agg, mptr, midx, gptr, gidx;
@@ -732,6 +746,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         char *b0 = E->d_blob;
         auto Dd = [&](size_t o) { return reinterpret_cast<const double *>(b0 + o); };
         D.img = Dd(o_img);
+        D.img_soa = Dd(o_soa);
         D.nc = nc; D.nif = nif; D.nf = nf; D.nb = nb; D.npatch = np;
         D.has_outlet = has_outlet; D.planar = planar;
         D.wall_y = Dd(o_wy);

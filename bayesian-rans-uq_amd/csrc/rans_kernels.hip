@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <type_traits>
 
 #include "rans_device.h"
 
@@ -79,49 +80,94 @@ DEV float wave_sumf(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// Grid barrier of the workgroups of one draw. Every wait is bounded: past
-// kGridSpinLimit polls the waiting workgroup raises abort, and every
-// workgroup then leaves every later barrier at once (the draw ends as
-// Unknown), so no wave can be left waiting.
+// Grid barrier of the workgroups of one draw. Arrivals go to one 64-bit
+// counter that is never reset: barrier k of the launch is passed when the
+// counter reaches nblk * k. Every wait is bounded: past kGridSpinLimit polls
+// the waiting workgroup raises abort, and every workgroup then leaves every
+// later barrier at once (the draw ends as Unknown), so no wave is left
+// waiting.
+//
+// Two visibility regimes. In general the workgroups may sit on different
+// XCDs, whose L2 caches are not coherent with each other, so the barrier is
+// an agent-scope release/acquire pair (the compiler emits an L2 write-back
+// before arriving and an L2 invalidate after leaving). When every workgroup
+// of the draw has been seen on the same XCD (GridLocal::local), they share
+// one L2, which then holds the latest value of every line they write (the
+// vector L1 is write-through). It is enough that each wave's stores have
+// completed before the arrival (s_waitcnt vmcnt(0)); the L2 keeps its
+// contents across the barrier. Stale lines can only sit in a CU's L1, so a
+// multi-workgroup draw reads every value another workgroup may have written
+// with device-scope loads, which miss the L1 (GRef below).
+struct GridLocal {
+    unsigned long long target;  // arrivals that end the next barrier (thread 0)
+    int flag;                   // 1: the last barrier completed; 0: aborted
+    int local;                  // 1: all workgroups of the draw share one XCD
+};
+
 constexpr long long kGridSpinLimit = 1LL << 25;
-DEV bool grid_barrier(GridState *g, int nblk, int *flag) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+// s_getreg_b32 operand of HW_REG_XCC_ID (register 20, bits 0..3): the XCD
+// the wave runs on
+constexpr int kHwRegXccId = 20 | (3 << 11);
+DEV bool grid_barrier(GridState *g, int nblk, GridLocal *gl) {
+    const bool local = gl->local != 0;
+    if (local)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    else
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
+        const unsigned long long target = gl->target + (unsigned long long)nblk;
+        gl->target = target;
         if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             ok = 0;
         } else {
-            const unsigned gen =
-                __hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned prev = __hip_atomic_fetch_add(&g->count, 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (prev == (unsigned)nblk - 1) {
-                __hip_atomic_store(&g->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&g->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                long long spins = 0;
-                while (__hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
-                       gen) {
-                    if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        ok = 0;
-                        break;
-                    }
-                    if (++spins > kGridSpinLimit) {
-                        __hip_atomic_store(&g->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = 0;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
+            __hip_atomic_fetch_add(&g->count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            long long spins = 0;
+            while ((long long)(__hip_atomic_load(&g->count, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+                if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
                 }
+                if (++spins > kGridSpinLimit) {
+                    __hip_atomic_store(&g->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
         }
-        *flag = ok;
+        gl->flag = ok;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return *flag != 0;
+    if (!local) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return gl->flag != 0;
 }
+
+// A double in global memory that other workgroups of a multi-workgroup draw
+// write: reads are device-scope loads (they miss the CU's L1 and are served
+// by the L2), writes are plain stores (the L1 is write-through).
+typedef __attribute__((address_space(1))) double GlobalDouble;
+struct GRef {
+    GlobalDouble *p;
+    DEV operator double() const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    DEV const GRef &operator=(double v) const {
+        *p = v;
+        return *this;
+    }
+    DEV const GRef &operator=(const GRef &o) const { return *this = double(o); }
+    DEV const GRef &operator+=(double v) const { return *this = double(*this) + v; }
+    DEV const GRef &operator-=(double v) const { return *this = double(*this) - v; }
+    DEV const GRef &operator*=(double v) const { return *this = double(*this) * v; }
+    DEV const GRef &operator/=(double v) const { return *this = double(*this) / v; }
+};
+struct GPtr {
+    GlobalDouble *p;
+    DEV GRef operator[](int i) const { return GRef{p + i}; }
+};
 
 template <int BS>
 struct Reducer {
@@ -132,7 +178,7 @@ struct Reducer {
     // every workgroup adds them in rank order (the same bits everywhere)
     GridState *grid = nullptr;
     int nblk = 1, rank = 0, gparity = 0;
-    int *flag = nullptr;
+    GridLocal *gl = nullptr;
 
     template <int NV, bool MAX>
     DEV void run(double (&v)[NV]) {
@@ -160,11 +206,15 @@ struct Reducer {
             if (threadIdx.x == 0)
 #pragma unroll
                 for (int i = 0; i < NV; ++i) pt[rank * 4 + i] = v[i];
-            grid_barrier(grid, nblk, flag);
+            grid_barrier(grid, nblk, gl);
+            // device-scope loads: the partials come from other workgroups
+            auto ld = [&](int k) {
+                return __hip_atomic_load(pt + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            };
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
-                double a = pt[i];
-                for (int r = 1; r < nblk; ++r) a = MAX ? smax(a, pt[r * 4 + i]) : a + pt[r * 4 + i];
+                double a = ld(i);
+                for (int r = 1; r < nblk; ++r) a = MAX ? smax(a, ld(r * 4 + i)) : a + ld(r * 4 + i);
                 v[i] = a;
             }
             gparity ^= 1;
@@ -225,7 +275,7 @@ struct Draw {
     Krylov *kres;  // LDS slot broadcasting a wave-resident solve's result
     double *amgw;  // multigrid workspace of this draw (global memory) or null
     Reducer<BS> red;
-    int nc, nif, nf, nb;
+    int nc, nif, nf, nb, nbs;  // nbs = max(nb, 1): boundary arrays' length
     bool planar, has_outlet;
     const int tid = threadIdx.x;
     // cells, faces and matrix rows are distributed over gs threads; a thread's
@@ -234,10 +284,10 @@ struct Draw {
     // multi-workgroup draws: this workgroup's rank among nblk
     int rank = 0, nblk = 1;
     GridState *grid = nullptr;
-    int *gflag = nullptr;
+    GridLocal *gl = nullptr;
     DEV void sync() {
         if constexpr (kGrid) {
-            if (!grid_barrier(grid, nblk, gflag) && tid == 0 && !bk->done) {
+            if (!grid_barrier(grid, nblk, gl) && tid == 0 && !bk->done) {
                 bk->done = 1;
                 bk->status = ST_UNKNOWN;
                 bk->error = ERR_GRID_TIMEOUT;
@@ -269,6 +319,7 @@ struct Draw {
         nif = M.nif;
         nf = M.nf;
         nb = M.nb;
+        nbs = nb > 1 ? nb : 1;
         planar = M.planar;
         has_outlet = M.has_outlet;
         const ArenaLayout &L = M.lay;
@@ -298,12 +349,39 @@ struct Draw {
 
     // record accessors: cell, face and boundary-face variables
     // (record indices stay below 2^23, so 24-bit multiplies are exact)
-    DEV D &C(int i, int v) const { return Cb[__mul24(i, kNA) + v]; }
-    DEV D &Fc(int f, int v) const { return Fb[__mul24(f, kNF) + v]; }
-    DEV D &Bd(int b, int v) const { return Bb[__mul24(b, kNB) + v]; }
+    // (multi-workgroup draws: coherent views, since any record may have been
+    // written by another workgroup since the last barrier)
+    using Ref = typename std::conditional<kGrid, GRef, D &>::type;
+    DEV Ref at(D *base, int k) const {
+        if constexpr (kGrid) return GRef{(GlobalDouble *)(base + k)};
+        else return base[k];
+    }
+    // LDS arenas hold records (odd strides spread them over the banks); HBM
+    // arenas hold one array per variable (nc, nf and max(nb, 1) long), so a
+    // wave's accesses to one variable of consecutive cells are coalesced
+    DEV Ref C(int i, int v) const {
+        if constexpr (kLDS) return at(Cb, __mul24(i, kNA) + v);
+        else return at(Cb, v * nc + i);
+    }
+    DEV Ref Fc(int f, int v) const {
+        if constexpr (kLDS) return at(Fb, __mul24(f, kNF) + v);
+        else return at(Fb, v * nf + f);
+    }
+    DEV Ref Bd(int b, int v) const {
+        if constexpr (kLDS) return at(Bb, __mul24(b, kNB) + v);
+        else return at(Bb, v * nbs + b);
+    }
 
     // off-diagonal coefficient of slot s of cell ci
-    DEV D &off(int ci, int s) const { return Cb[__mul24(ci, kNA) + V_off0 + s]; }
+    DEV Ref off(int ci, int s) const { return C(ci, V_off0 + s); }
+
+    // multigrid workspace arrays (global memory, shared by the workgroups of
+    // a multi-workgroup draw)
+    using WPtr = typename std::conditional<kGrid, GPtr, double *>::type;
+    DEV WPtr wp(int64_t o) const {
+        if constexpr (kGrid) return GPtr{(GlobalDouble *)(amgw + o)};
+        else return amgw + o;
+    }
 
     // ---------------------------------------------------------------- SST
     // SSTModel pointwise functions (core/src/SSTModel.cpp:15-108)
@@ -1458,25 +1536,26 @@ struct Draw {
         const AmgMesh &G = M.amg;
         for (int l = 1; l < G.levels; ++l) {
             const AmgLevel &F = G.lev[l - 1], &L = G.lev[l];
-            double *val = amgw + L.val;
+            const WPtr val = wp(L.val), fval = wp(F.val), dinv = wp(L.dinv);
             for (int e = gt; e < L.nnz; e += gs) {
                 double a = 0.0;
                 for (int q = F.gptr[e]; q < F.gptr[e + 1]; ++q) {
                     const int fe = F.gidx[q];
-                    a += l == 1 ? lev0_value(G.src0[fe]) : amgw[F.val + fe];
+                    a += l == 1 ? lev0_value(G.src0[fe]) : double(fval[fe]);
                 }
                 val[e] = a;
             }
             sync();
             for (int i = gt; i < L.n; i += gs) {
                 const double d = val[L.diagpos[i]];
-                amgw[L.dinv + i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+                dinv[i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
             }
         }
         sync();
         // dense Gauss-Jordan inverse of the coarsest level (symmetric positive
         // definite: no pivoting); A in inv[0, n^2), its inverse after it. One
-        // workgroup computes it (rank 0 of a multi-workgroup draw).
+        // workgroup computes it (rank 0 of a multi-workgroup draw), so within
+        // it A, X and fac are read back by the workgroup that wrote them.
         if (rank == 0) {
             const int n = G.coarse_n;
             const AmgLevel &L = G.lev[G.levels - 1];
@@ -1486,10 +1565,11 @@ struct Draw {
                 X[e] = (e / n == e % n) ? 1.0 : 0.0;
             }
             __syncthreads();
+            const WPtr lval = wp(L.val);
             for (int i = tid; i < n; i += BS)
                 for (int q2 = L.rowptr[i]; q2 < L.rowptr[i + 1]; ++q2)
                     A[i * n + L.col[q2]] =
-                        G.levels == 1 ? lev0_value(G.src0[q2]) : amgw[L.val + q2];
+                        G.levels == 1 ? lev0_value(G.src0[q2]) : double(lval[q2]);
             __syncthreads();
             for (int kk = 0; kk < n; ++kk) {
                 const double piv = A[kk * n + kk];
@@ -1513,107 +1593,164 @@ struct Draw {
         sync();
     }
 
-    // y = A x on a coarse level (CSR)
-    DEV double coarse_row(const AmgLevel &L, int i, const double *x) const {
+    // y = A x on a coarse level (CSR); x(j) gives the vector's entries
+    template <class F>
+    DEV double row_f(const AmgLevel &L, int i, F x) const {
         double a = 0.0;
-        const double *val = amgw + L.val;
-        for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q) a += val[q] * x[L.col[q]];
+        const WPtr val = wp(L.val);
+        for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q) a += val[q] * x(L.col[q]);
         return a;
+    }
+    DEV double coarse_row(const AmgLevel &L, int i, WPtr x) const {
+        return row_f(L, i, [&](int j) { return double(x[j]); });
+    }
+    // level-0 product with the vector given entry by entry
+    template <class F>
+    DEV double spmv_f(int ci, F x) const {
+        const v4i nb4 = slot4[2 * ci + 1];
+        const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        double y = C(ci, V_diag) * x(ci);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) y += off(ci, s) * x(nbv[s]);
+        return y;
+    }
+
+    // Workgroup barrier for work one workgroup does alone. In a
+    // multi-workgroup draw its reads are device-scope loads, which bypass
+    // the L1, so the workgroup's stores must have reached the L2 first.
+    DEV void bsync() const {
+        if constexpr (kGrid) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
+
+    // Levels with more than kAmgBlockRows rows are worked by every workgroup
+    // of a multi-workgroup draw; smaller ones by rank 0 alone, where a
+    // workgroup barrier replaces a grid barrier.
+    static constexpr int kAmgBlockRows = 1024;
+
+    // One level on the way down: two damped-Jacobi sweeps from zero, the
+    // residual, and its restriction (aggregate sums) into the next level's
+    // right-hand side. The first sweep's iterate w D^-1 b is formed where it
+    // is read, so both sweeps take one pass; the arithmetic is the two
+    // sweeps' own.
+    DEV void amg_down(int l, bool g) {
+        const AmgMesh &G = M.amg;
+        const AmgLevel &L = G.lev[l];
+        const double w = kJacobiOmega;
+        const int i0 = g ? gt : tid, is = g ? gs : BS;
+        if (l == 0) {
+            auto x0 = [&](int j) {
+                const double d = C(j, V_diag);
+                return fabs(d) > 1e-300 ? w * C(j, V_kr) / d : 0.0;
+            };
+            for (int ci = i0; ci < nc; ci += is) {
+                const double d = C(ci, V_diag), xi = x0(ci);
+                const double r0 = C(ci, V_kr) - spmv_f(ci, x0);
+                C(ci, V_krh) = fabs(d) > 1e-300 ? xi + w * r0 / d : xi;
+            }
+            g ? sync() : bsync();
+            for (int ci = i0; ci < nc; ci += is) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
+        } else {
+            const WPtr x = wp(L.x), b = wp(L.b), r = wp(L.r), di = wp(L.dinv);
+            auto x0 = [&](int j) { return w * b[j] * di[j]; };
+            for (int i = i0; i < L.n; i += is) {
+                const double xi = x0(i), r0 = b[i] - row_f(L, i, x0);
+                x[i] = xi + w * r0 * di[i];
+            }
+            g ? sync() : bsync();
+            for (int i = i0; i < L.n; i += is) r[i] = b[i] - coarse_row(L, i, x);
+        }
+        g ? sync() : bsync();
+        const AmgLevel &Cl = G.lev[l + 1];
+        const WPtr bc = wp(Cl.b), rf = wp(L.r);
+        for (int I = i0; I < Cl.n; I += is) {
+            double a = 0.0;
+            for (int q = L.mptr[I]; q < L.mptr[I + 1]; ++q)
+                a += l == 0 ? double(C(L.midx[q], V_ks)) : double(rf[L.midx[q]]);
+            bc[I] = a;
+        }
+        g ? sync() : bsync();
+    }
+
+    // One level on the way up: prolongation (x += xc of the cell's
+    // aggregate) and two damped-Jacobi sweeps. The prolonged iterate is
+    // formed where it is read and each sweep writes a second buffer (the
+    // level's residual array, or V_kt on level 0), so each sweep is one
+    // pass. The last pass of level 0 is not followed by a barrier: every
+    // caller reads only its own rows next.
+    DEV void amg_up(int l, bool g) {
+        const AmgMesh &G = M.amg;
+        const AmgLevel &L = G.lev[l];
+        const double w = kJacobiOmega;
+        const int i0 = g ? gt : tid, is = g ? gs : BS;
+        const WPtr xc = wp(G.lev[l + 1].x);
+        if (l == 0) {
+            auto xp = [&](int j) { return C(j, V_krh) + xc[L.agg[j]]; };
+            for (int ci = i0; ci < nc; ci += is) {
+                const double d = C(ci, V_diag), xi = xp(ci);
+                const double r = C(ci, V_kr) - spmv_f(ci, xp);
+                C(ci, V_kt) = fabs(d) > 1e-300 ? xi + w * r / d : xi;
+            }
+            g ? sync() : bsync();
+            for (int ci = i0; ci < nc; ci += is) {
+                const double d = C(ci, V_diag), xi = C(ci, V_kt);
+                const double r = C(ci, V_kr) - spmv(ci, V_kt);
+                C(ci, V_krh) = fabs(d) > 1e-300 ? xi + w * r / d : xi;
+            }
+        } else {
+            const WPtr x = wp(L.x), b = wp(L.b), t = wp(L.r), di = wp(L.dinv);
+            auto xp = [&](int j) { return x[j] + xc[L.agg[j]]; };
+            for (int i = i0; i < L.n; i += is) {
+                const double xi = xp(i), r = b[i] - row_f(L, i, xp);
+                t[i] = xi + w * r * di[i];
+            }
+            g ? sync() : bsync();
+            for (int i = i0; i < L.n; i += is) {
+                const double xi = t[i], r = b[i] - coarse_row(L, i, t);
+                x[i] = xi + w * r * di[i];
+            }
+            g ? sync() : bsync();
+        }
     }
 
     // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks
     DEV void amg_vcycle() {
         const AmgMesh &G = M.amg;
         const int nl = G.levels;
-        const double w = kJacobiOmega;
-        // down: pre-smooth, residual, restrict
-        for (int l = 0; l < nl - 1; ++l) {
-            const AmgLevel &L = G.lev[l];
-            if (l == 0) {
-                for (int ci = gt; ci < nc; ci += gs) {
-                    const double d = C(ci, V_diag);
-                    C(ci, V_krh) = fabs(d) > 1e-300 ? w * C(ci, V_kr) / d : 0.0;
-                }
-                sync();
-                for (int ci = gt; ci < nc; ci += gs) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
-                sync();
-                for (int ci = gt; ci < nc; ci += gs) {
-                    const double d = C(ci, V_diag);
-                    if (fabs(d) > 1e-300) C(ci, V_krh) += w * C(ci, V_ks) / d;
-                }
-                sync();
-                for (int ci = gt; ci < nc; ci += gs) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
-            } else {
-                double *x = amgw + L.x, *b = amgw + L.b, *r = amgw + L.r, *di = amgw + L.dinv;
-                for (int i = gt; i < L.n; i += gs) x[i] = w * b[i] * di[i];
-                sync();
-                for (int i = gt; i < L.n; i += gs) r[i] = b[i] - coarse_row(L, i, x);
-                sync();
-                for (int i = gt; i < L.n; i += gs) x[i] += w * r[i] * di[i];
-                sync();
-                for (int i = gt; i < L.n; i += gs) r[i] = b[i] - coarse_row(L, i, x);
-            }
-            sync();
-            const AmgLevel &Cl = G.lev[l + 1];
-            double *bc = amgw + Cl.b;
-            for (int I = gt; I < Cl.n; I += gs) {
-                double a = 0.0;
-                for (int q = L.mptr[I]; q < L.mptr[I + 1]; ++q)
-                    a += l == 0 ? C(L.midx[q], V_ks) : amgw[L.r + L.midx[q]];
-                bc[I] = a;
-            }
-            sync();
-        }
-        // coarsest: x = A^-1 b
-        {
+        if (nl == 1) {
             const int n = G.coarse_n;
-            const double *X = amgw + G.inv + n * n;
-            if (nl == 1) {
-                for (int i = gt; i < n; i += gs) {
-                    double a = 0.0;
-                    for (int j = 0; j < n; ++j) a += X[i * n + j] * C(j, V_kr);
-                    C(i, V_krh) = a;
-                }
-                sync();
-                return;
-            }
-            const AmgLevel &L = G.lev[nl - 1];
-            double *x = amgw + L.x, *b = amgw + L.b;
+            const WPtr X = wp(G.inv + (int64_t)n * n);
             for (int i = gt; i < n; i += gs) {
+                double a = 0.0;
+                for (int j = 0; j < n; ++j) a += X[i * n + j] * C(j, V_kr);
+                C(i, V_krh) = a;
+            }
+            sync();
+            return;
+        }
+        // levels [0, ls) are grid levels (kGrid), the rest rank 0's
+        int ls = 1;
+        if constexpr (kGrid)
+            while (ls < nl - 1 && G.lev[ls].n > kAmgBlockRows) ++ls;
+        for (int l = 0; l < ls; ++l) amg_down(l, kGrid);
+        if (!kGrid || rank == 0) {
+            for (int l = ls; l < nl - 1; ++l) amg_down(l, false);
+            // coarsest: x = A^-1 b
+            const int n = G.coarse_n;
+            const WPtr X = wp(G.inv + (int64_t)n * n);
+            const AmgLevel &L = G.lev[nl - 1];
+            const WPtr x = wp(L.x), b = wp(L.b);
+            for (int i = tid; i < n; i += BS) {
                 double a = 0.0;
                 for (int j = 0; j < n; ++j) a += X[i * n + j] * b[j];
                 x[i] = a;
             }
-            sync();
+            bsync();
+            for (int l = nl - 2; l >= ls; --l) amg_up(l, false);
         }
-        // up: prolong, post-smooth
-        for (int l = nl - 2; l >= 0; --l) {
-            const AmgLevel &L = G.lev[l];
-            const double *xc = amgw + G.lev[l + 1].x;
-            if (l == 0) {
-                for (int ci = gt; ci < nc; ci += gs) C(ci, V_krh) += xc[L.agg[ci]];
-                sync();
-                for (int sweep = 0; sweep < 2; ++sweep) {
-                    for (int ci = gt; ci < nc; ci += gs) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
-                    sync();
-                    for (int ci = gt; ci < nc; ci += gs) {
-                        const double d = C(ci, V_diag);
-                        if (fabs(d) > 1e-300) C(ci, V_krh) += w * C(ci, V_ks) / d;
-                    }
-                    sync();
-                }
-            } else {
-                double *x = amgw + L.x, *b = amgw + L.b, *r = amgw + L.r, *di = amgw + L.dinv;
-                for (int i = gt; i < L.n; i += gs) x[i] += xc[L.agg[i]];
-                sync();
-                for (int sweep = 0; sweep < 2; ++sweep) {
-                    for (int i = gt; i < L.n; i += gs) r[i] = b[i] - coarse_row(L, i, x);
-                    sync();
-                    for (int i = gt; i < L.n; i += gs) x[i] += w * r[i] * di[i];
-                    sync();
-                }
-            }
-        }
+        if constexpr (kGrid) sync();  // rank 0's levels are visible to every workgroup
+        for (int l = ls - 1; l >= 0; --l) amg_up(l, kGrid);
     }
 
     // AMGSolver::solve from x = 0 (LinearSolver.hpp:908-970): kr holds b.
@@ -2355,7 +2492,7 @@ struct BlockShared {
     double coeffs[11];
     DevSettings sset;
     Krylov kslot;
-    int gflag;
+    GridLocal gl;
 };
 
 // rank/nblk/grid/aux describe a multi-workgroup draw (kGrid): its workgroups
@@ -2371,27 +2508,47 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
     const int g0 = rank * BS + threadIdx.x, gstride = nblk * BS;
     {
         const int64_t n16 = M.lay.image_bytes / 16;
-        const uint4 *gsrc = reinterpret_cast<const uint4 *>(M.img);
+        const uint4 *gsrc = reinterpret_cast<const uint4 *>(kLDS ? M.img : M.img_soa);
         uint4 *dst = reinterpret_cast<uint4 *>(arena);
         for (int64_t i = g0; i < n16; i += gstride) dst[i] = gsrc[i];
     }
-    if constexpr (kGrid) grid_barrier(grid, nblk, &sh.gflag);
-    else __syncthreads();
+    if constexpr (kGrid) {
+        // the first barrier is agent-scope; it also publishes each
+        // workgroup's XCD, and later barriers stay within the L2 when the
+        // draw's workgroups all share one XCD
+        if (threadIdx.x == 0) {
+            sh.gl.target = 0;
+            sh.gl.local = 0;
+            grid->xcc[rank] = (int)__builtin_amdgcn_s_getreg(kHwRegXccId);
+        }
+        __syncthreads();
+        grid_barrier(grid, nblk, &sh.gl);
+        if (threadIdx.x == 0) {
+            int same = 1;
+            for (int r = 1; r < nblk; ++r) same &= grid->xcc[r] == grid->xcc[0];
+            sh.gl.local = same;
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+    }
     Draw<BS, kLDS, kGrid> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
                             aux);
     if constexpr (kGrid) {
         d.rank = rank;
         d.nblk = nblk;
         d.grid = grid;
-        d.gflag = &sh.gflag;
+        d.gl = &sh.gl;
         d.gt = g0;
         d.gs = gstride;
         d.red.grid = grid;
         d.red.nblk = nblk;
         d.red.rank = rank;
-        d.red.flag = &sh.gflag;
+        d.red.gl = &sh.gl;
     }
     d.run();
+    // every workgroup's last writes are complete before rank 0 observes
+    if constexpr (kGrid) grid_barrier(grid, nblk, &sh.gl);
     const Book &book = sh.book;
     const bool observe = book.status == ST_CONVERGED || book.status == ST_UNCONVERGED;
     if (rank == 0)
@@ -2399,11 +2556,19 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
     if (fields) {
         const int nc = M.nc;
         const int vars[10] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
+        auto ld = [&](int64_t k) {
+            if constexpr (kGrid)
+                return __hip_atomic_load(arena + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                return arena[k];
+        };
         for (int a = 0; a < 10; ++a)
             for (int ci = g0; ci < nc; ci += gstride)
-                fields[a * nc + ci] = arena[ci * kNA + vars[a]];
+                fields[a * nc + ci] = ld(kLDS ? (int64_t)ci * kNA + vars[a]
+                                              : (int64_t)vars[a] * nc + ci);
         for (int f = g0; f < M.nf; f += gstride)
-            fields[10 * nc + f] = arena[M.lay.face + f * kNF + W_phi];
+            fields[10 * nc + f] = ld(M.lay.face + (kLDS ? (int64_t)f * kNF + W_phi
+                                                        : (int64_t)W_phi * M.nf + f));
     }
     if (rank == 0 && threadIdx.x == 0) {
         DevResult &r = *res;
