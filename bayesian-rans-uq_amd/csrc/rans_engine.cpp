@@ -29,6 +29,9 @@ hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                         double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
                         hipStream_t stream);
 int grid_co_resident(size_t lds_bytes);
+
+// meshes from this size up run multi-workgroup draws when the batch leaves CUs idle
+constexpr int kGridAutoCells = 4096;
 hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,
                              DevResult *results, double *pred, int pred_stride, double *fields,
                              int64_t field_stride, GridState *states, size_t lds_bytes,
@@ -100,6 +103,7 @@ struct rq_engine {
     GridState *d_grid = nullptr;
     size_t cap_grid = 0;
     int cus = 0;
+    bool block_user = false;  // block_size given explicitly
     int last_nblk = 1;
     size_t cap_slots = 0, cap_results = 0, cap_pred = 0, cap_fields = 0, cap_arena = 0;
     size_t cap_amg = 0;
@@ -380,15 +384,25 @@ void amg_build(const rq_mesh_desc &m, const int *ISL, const int *IBF, bool has_o
         std::vector<std::map<int, double>> rows(ncoarse);
         for (int i = 0; i < F.n; ++i)
             for (int j = F.rowptr[i]; j < F.rowptr[i + 1]; ++j) rows[map[i]][map[F.col[j]]] += F.val[j];
+        // Coarse rows are padded to a multiple of 4 entries with (column =
+        // row, value 0) entries after the sorted real ones, so the device
+        // reads a row 4 entries at a time; the padding never receives a
+        // Galerkin contribution and adds exact zeros.
         AmgHostLevel Cl;
         Cl.n = ncoarse;
         Cl.rowptr.assign(ncoarse + 1, 0);
         Cl.diagpos.assign(ncoarse, -1);
+        std::vector<int> nreal(ncoarse, 0);
         for (int I = 0; I < ncoarse; ++I) {
             for (const auto &kv : rows[I]) {
                 if (kv.first == I) Cl.diagpos[I] = (int)Cl.col.size();
                 Cl.col.push_back(kv.first);
                 Cl.val.push_back(kv.second);
+            }
+            nreal[I] = (int)Cl.col.size() - Cl.rowptr[I];
+            while ((Cl.col.size() - Cl.rowptr[I]) % 4) {
+                Cl.col.push_back(I);
+                Cl.val.push_back(0.0);
             }
             Cl.rowptr[I + 1] = (int)Cl.col.size();
         }
@@ -399,7 +413,7 @@ void amg_build(const rq_mesh_desc &m, const int *ISL, const int *IBF, bool has_o
         for (int i = 0; i < F.n; ++i)
             for (int j = F.rowptr[i]; j < F.rowptr[i + 1]; ++j) {
                 const int I = map[i], J = map[F.col[j]];
-                const int *b = Cl.col.data() + Cl.rowptr[I], *e2 = Cl.col.data() + Cl.rowptr[I + 1];
+                const int *b = Cl.col.data() + Cl.rowptr[I], *e2 = b + nreal[I];
                 const int pos = (int)(std::lower_bound(b, e2, J) - Cl.col.data());
                 contrib[pos].push_back(j);
             }
@@ -411,9 +425,11 @@ void amg_build(const rq_mesh_desc &m, const int *ISL, const int *IBF, bool has_o
         F.agg = map;
         std::vector<std::vector<int>> members(ncoarse);
         for (int i = 0; i < F.n; ++i) members[map[i]].push_back(i);
+        // member lists padded to a multiple of 4 with -1 (read 4 at a time)
         F.mptr.assign(ncoarse + 1, 0);
         for (int I = 0; I < ncoarse; ++I) {
             F.midx.insert(F.midx.end(), members[I].begin(), members[I].end());
+            while ((F.midx.size() - F.mptr[I]) % 4) F.midx.push_back(-1);
             F.mptr[I + 1] = (int)F.midx.size();
         }
         H.lev.push_back(std::move(Cl));
@@ -835,6 +851,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         HIPCHECK(hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device));
         if (s.mixed_precision && nc > 256)
             throw Fail{RQ_EINVAL, "mixed precision is available for meshes of up to 256 cells"};
+        E->block_user = s.block_size != 0;
         if (s.block_size) E->block = s.block_size;
         else E->block = nc <= 64 ? 64 : nc <= 128 ? 128 : nc <= 256 ? 256 : 512;
         int maxLds = 0;
@@ -996,11 +1013,19 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
     int nblk = 1;
     const size_t grid_lds =
         (8 * (size_t)E->settings.convergence_window + 2 * (size_t)std::max(nobs, 1)) * 8;
-    // Automatic selection keeps one workgroup per draw: on MI355X every grid
-    // barrier of a multi-workgroup draw pays an agent-scope release/acquire
-    // (L2 writeback and invalidate), about 60-120 us per barrier measured on
-    // the 120x64 hill, which outweighs the parallelism for the meshes here.
-    if (E->settings.grid_blocks > 1) nblk = E->settings.grid_blocks;
+    // Automatic selection (grid_blocks 0, no explicit block size): arenas in
+    // HBM of at least kGridAutoCells cells spread each draw over the CUs the
+    // batch leaves idle, one workgroup per 256 cells at most. Below that size
+    // one workgroup per draw is faster (measured: 60x32 hill 2.3 ms per outer
+    // iteration on one workgroup against 2.8 on eight; 120x64 9.3 against
+    // 4.7 on sixteen).
+    if (E->settings.grid_blocks > 1) {
+        nblk = E->settings.grid_blocks;
+    } else if (E->settings.grid_blocks == 0 && !E->lds && !E->block_user &&
+               E->nc >= kGridAutoCells) {
+        const int per_xcd = (ns + kXcds - 1) / kXcds;
+        nblk = std::min({kMaxGridBlocks, E->cus / (kXcds * per_xcd), (E->nc + 255) / 256});
+    }
     if (nblk > 1) {
         // every workgroup of the padded grid (draws spread over the XCDs) must
         // be co-resident

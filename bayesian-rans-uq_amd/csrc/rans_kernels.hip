@@ -102,6 +102,7 @@ struct GridLocal {
     unsigned long long target;  // arrivals that end the next barrier (thread 0)
     int flag;                   // 1: the last barrier completed; 0: aborted
     int local;                  // 1: all workgroups of the draw share one XCD
+    long long wait, count;      // shader clocks thread 0 spent in barriers, barriers
 };
 
 constexpr long long kGridSpinLimit = 1LL << 25;
@@ -116,6 +117,7 @@ DEV bool grid_barrier(GridState *g, int nblk, GridLocal *gl) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
+        const long long t0 = clock64();
         int ok = 1;
         const unsigned long long target = gl->target + (unsigned long long)nblk;
         gl->target = target;
@@ -139,6 +141,8 @@ DEV bool grid_barrier(GridState *g, int nblk, GridLocal *gl) {
             }
         }
         gl->flag = ok;
+        gl->wait += clock64() - t0;
+        gl->count += 1;
     }
     __syncthreads();
     if (!local) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -274,6 +278,7 @@ struct Draw {
     Book *bk;
     Krylov *kres;  // LDS slot broadcasting a wave-resident solve's result
     double *amgw;  // multigrid workspace of this draw (global memory) or null
+    double *cbuf = nullptr;  // LDS, kMaxAmgCoarse doubles: the coarsest right-hand side
     Reducer<BS> red;
     int nc, nif, nf, nb, nbs;  // nbs = max(nb, 1): boundary arrays' length
     bool planar, has_outlet;
@@ -1566,9 +1571,10 @@ struct Draw {
             }
             __syncthreads();
             const WPtr lval = wp(L.val);
+            // (+=: the padding entries of a coarse row add zeros to its diagonal)
             for (int i = tid; i < n; i += BS)
                 for (int q2 = L.rowptr[i]; q2 < L.rowptr[i + 1]; ++q2)
-                    A[i * n + L.col[q2]] =
+                    A[i * n + L.col[q2]] +=
                         G.levels == 1 ? lev0_value(G.src0[q2]) : double(lval[q2]);
             __syncthreads();
             for (int kk = 0; kk < n; ++kk) {
@@ -1594,11 +1600,21 @@ struct Draw {
     }
 
     // y = A x on a coarse level (CSR); x(j) gives the vector's entries
+    // (coarse rows are padded to a multiple of 4 entries, so a row is read
+    // 4 entries at a time and the loads of a chunk are in flight together)
     template <class F>
     DEV double row_f(const AmgLevel &L, int i, F x) const {
         double a = 0.0;
         const WPtr val = wp(L.val);
-        for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; ++q) a += val[q] * x(L.col[q]);
+        for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; q += 4) {
+            const v4i c = *reinterpret_cast<const v4i *>(L.col + q);
+            const double v0 = val[q], v1 = val[q + 1], v2 = val[q + 2], v3 = val[q + 3];
+            const double x0 = x(c.x), x1 = x(c.y), x2 = x(c.z), x3 = x(c.w);
+            a += v0 * x0;
+            a += v1 * x1;
+            a += v2 * x2;
+            a += v3 * x3;
+        }
         return a;
     }
     DEV double coarse_row(const AmgLevel &L, int i, WPtr x) const {
@@ -1664,10 +1680,21 @@ struct Draw {
         g ? sync() : bsync();
         const AmgLevel &Cl = G.lev[l + 1];
         const WPtr bc = wp(Cl.b), rf = wp(L.r);
+        // member lists are padded to a multiple of 4 with -1
+        auto res = [&](int m) { return l == 0 ? double(C(m, V_ks)) : double(rf[m]); };
         for (int I = i0; I < Cl.n; I += is) {
             double a = 0.0;
-            for (int q = L.mptr[I]; q < L.mptr[I + 1]; ++q)
-                a += l == 0 ? double(C(L.midx[q], V_ks)) : double(rf[L.midx[q]]);
+            for (int q = L.mptr[I]; q < L.mptr[I + 1]; q += 4) {
+                const v4i m = *reinterpret_cast<const v4i *>(L.midx + q);
+                const double r0 = res(m.x);
+                const double r1 = m.y >= 0 ? res(m.y) : 0.0;
+                const double r2 = m.z >= 0 ? res(m.z) : 0.0;
+                const double r3 = m.w >= 0 ? res(m.w) : 0.0;
+                a += r0;
+                if (m.y >= 0) a += r1;
+                if (m.z >= 0) a += r2;
+                if (m.w >= 0) a += r3;
+            }
             bc[I] = a;
         }
         g ? sync() : bsync();
@@ -1736,14 +1763,17 @@ struct Draw {
         for (int l = 0; l < ls; ++l) amg_down(l, kGrid);
         if (!kGrid || rank == 0) {
             for (int l = ls; l < nl - 1; ++l) amg_down(l, false);
-            // coarsest: x = A^-1 b
+            // coarsest: x = A^-1 b, with b staged in LDS; the inverse was
+            // written by this workgroup, so its rows are read with plain loads
             const int n = G.coarse_n;
-            const WPtr X = wp(G.inv + (int64_t)n * n);
+            const double *X = amgw + G.inv + (int64_t)n * n;
             const AmgLevel &L = G.lev[nl - 1];
             const WPtr x = wp(L.x), b = wp(L.b);
+            for (int i = tid; i < n; i += BS) cbuf[i] = b[i];
+            __syncthreads();
             for (int i = tid; i < n; i += BS) {
                 double a = 0.0;
-                for (int j = 0; j < n; ++j) a += X[i * n + j] * b[j];
+                for (int j = 0; j < n; ++j) a += X[i * n + j] * cbuf[j];
                 x[i] = a;
             }
             bsync();
@@ -2493,6 +2523,7 @@ struct BlockShared {
     DevSettings sset;
     Krylov kslot;
     GridLocal gl;
+    double coarse_b[kMaxAmgCoarse];
 };
 
 // rank/nblk/grid/aux describe a multi-workgroup draw (kGrid): its workgroups
@@ -2519,6 +2550,8 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
         if (threadIdx.x == 0) {
             sh.gl.target = 0;
             sh.gl.local = 0;
+            sh.gl.wait = 0;
+            sh.gl.count = 0;
             grid->xcc[rank] = (int)__builtin_amdgcn_s_getreg(kHwRegXccId);
         }
         __syncthreads();
@@ -2534,6 +2567,7 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
     }
     Draw<BS, kLDS, kGrid> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
                             aux);
+    d.cbuf = sh.coarse_b;
     if constexpr (kGrid) {
         d.rank = rank;
         d.nblk = nblk;
@@ -2582,6 +2616,10 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
         r.diverged = book.status == ST_DIVERGED;
         for (int e = 0; e < 5; ++e) r.krylov[e] = book.kry[e];
         for (int e = 0; e < kPhases; ++e) r.cycles[e] = book.cyc[e];
+        if (kGrid && settings.profile) {  // grid barriers: clocks waited, count
+            r.cycles[kPhases - 2] = sh.gl.wait;
+            r.cycles[kPhases - 1] = sh.gl.count;
+        }
     }
 }
 

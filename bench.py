@@ -18,6 +18,12 @@ run in one persistent-kernel launch. The line printed is
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 
+Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
+and FP64 defect correction), served (Poisson arrivals of single requests on
+64-512 cell channels against the plain one-launch-per-request path), hills
+(the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
+--hills-full adds the 120x64 batch to convergence).
+
 Profiling helpers: --max-iterations caps every solve (a short, fixed amount of
 work for counter runs); --copy-calibration MIB runs only a device-to-device
 copy of that size, whose FETCH_SIZE/WRITE_SIZE readings scale the counters of
@@ -246,6 +252,69 @@ def served_leg(n_requests, rate):
                     "256- and 512-cell channels end Unconverged at the cap on every path"}
 
 
+def hills_leg(full, cores, cpu):
+    """Hill meshes: the 16-draw 120x64 batch (AMG pressure, tol 1e-4) and the
+    8-draw 480x256 batch, each timed per outer iteration on solves capped at
+    a few hundred (resp. ten) iterations; the engine picks the workgroups per
+    draw. The CPU oracle runs the 120x64 draws one per thread with the same
+    cap. With full=True the 16-draw 120x64 batch also runs to convergence
+    (cap 25000; minutes)."""
+    from cfdr_amd import cases, engine
+    out = {}
+    for key, nx, ny, nd, cap in (("hill_120x64", 120, 64, 16, 200),
+                                 ("hill_480x256", 480, 256, 8, 10)):
+        prob = cases.hill_problem(nx=nx, ny=ny, max_iterations=cap)
+        th = cases.hill_draws(nd)
+        b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
+                                prob.settings, prob.u_init, prob.p_init, prob.k_init,
+                                prob.omega_init)
+        r = b.evaluate(th)
+        t = b.last_timing()
+        kry = np.asarray(r.krylov_iterations, dtype=float).mean(axis=0) / cap
+        out[key] = {"draws": nd, "cells": prob.mesh.n_cells, "capped_iterations": cap,
+                    "ms_per_outer_iteration": round(r.kernel_ms / cap, 3),
+                    "draw_iterations_per_s": round(nd * cap / (r.kernel_ms / 1e3), 1),
+                    "grid_blocks": t["grid_blocks"], "block_size": t["block_size"],
+                    "krylov_per_outer_iteration": [round(float(k), 1) for k in kry]}
+        print(f"# {key}: {r.kernel_ms / cap:.2f} ms per outer iteration", file=sys.stderr,
+              flush=True)
+        if cpu and key == "hill_120x64":
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py
+            ccap = 60
+            prob.settings.max_iterations = ccap
+            O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
+                                        prob.init_vector())
+            rows = [th[i % nd] for i in range(cores)]
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as ex:
+                list(ex.map(lambda v: O.evaluate(prob.coefficients(v)), rows))
+            dt = time.perf_counter() - t0
+            out[key]["cpu_baseline"] = {
+                "value": round(cores * ccap / dt, 1), "unit": "draw-iterations/s",
+                "cores": cores, "kind": "port",
+                "sample": f"{cores} draws capped at {ccap} outer iterations, one per thread, "
+                          f"{dt:.1f} s wall"}
+    if full:
+        prob = cases.hill_problem(nx=120, ny=64)
+        th = cases.hill_draws(16)
+        b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
+                                prob.settings, prob.u_init, prob.p_init, prob.k_init,
+                                prob.omega_init)
+        print("# hill_120x64 to convergence (cap 25000) ...", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        r = b.evaluate(th)
+        dt = time.perf_counter() - t0
+        out["hill_120x64_full"] = {
+            "draws": 16, "seconds": round(dt, 1),
+            "converged_per_s": round(int(np.sum(r.convergence_mask)) / dt, 4),
+            "completed_per_s": round(16 / dt, 4),
+            "statuses": {s.name: int(sum(1 for x in r.statuses if x == s))
+                         for s in set(r.statuses)},
+            "iterations": [int(i) for i in r.iteration_counts]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +331,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=16.0)
     ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-hills", action="store_true")
+    ap.add_argument("--hills-full", action="store_true",
+                    help="also run the 16-draw 120x64 hill batch to convergence (minutes)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -378,6 +450,10 @@ def main():
     if not args.no_serve:
         served = served_leg(args.serve_requests, args.serve_rate)
 
+    hills = None
+    if not args.no_hills:
+        hills = hills_leg(args.hills_full, _cores(), not args.no_cpu_baseline)
+
     line = {
         "metric": "converged SST solves/s (served vs plain) at <=1e-6 rel err; p50/p99 latency",
         "value": round(value, 4),
@@ -409,6 +485,7 @@ def main():
         "max_rel_err_vs_oracle": err,
         "mixed": mixed,
         "served": served,
+        "hills": hills,
         "block_size": batch.last_timing()["block_size"],
         "arena_in_lds": batch.last_timing()["arena_in_lds"],
     }

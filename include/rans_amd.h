@@ -74,8 +74,10 @@ typedef struct rq_settings {
     int32_t mixed_precision; /* 1: FP32 Krylov vectors, FP64 defect correction */
     int32_t block_size;      /* threads per draw; 0 = tuned default */
     int32_t profile;         /* 1: per-phase clock counts in rq_batch_out.phase_cycles */
-    int32_t grid_blocks;     /* workgroups per draw: 0 or 1 = one workgroup, 2..64 = that many
-                                co-resident workgroups (grid barriers) */
+    int32_t grid_blocks;     /* workgroups per draw: 0 = automatic (one, except arenas in HBM of
+                                >= 4096 cells with no explicit block_size, which spread over
+                                the CUs the batch leaves idle, one per 256 cells at most);
+                                1 = one; 2..64 = that many co-resident workgroups */
 } rq_settings;
 
 /* InferenceParameterSet (core/include/InferenceParameters.hpp:18-188). */
@@ -118,7 +120,9 @@ typedef struct rq_batch_out {
     size_t row_error_stride;
     double *fields;
     int64_t *krylov_iterations; /* n_rows x 5: Ux, Uy, p', k, omega */
-    int64_t *phase_cycles;      /* n_rows x 16 shader clocks per solve phase (profile = 1) */
+    int64_t *phase_cycles;      /* n_rows x 16 shader clocks per solve phase (profile = 1);
+                                   multi-workgroup draws: [14] clocks rank 0 waited in grid
+                                   barriers, [15] the number of grid barriers */
 } rq_batch_out;
 
 /* ForwardBatch constructor: copies every input (the caller may free or
