@@ -88,7 +88,13 @@ EXPORTS = ["rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
            "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
            "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info",
            "rq_server_create", "rq_server_add_problem", "rq_server_submit", "rq_server_poll",
-           "rq_server_stats", "rq_server_destroy"]
+           "rq_server_stats", "rq_server_destroy",
+           "rq_model_create", "rq_model_evaluate", "rq_model_evaluate_batch", "rq_model_engine",
+           "rq_model_cache_size", "rq_model_cache_retained_bytes", "rq_model_cache_byte_budget",
+           "rq_model_cache_set_byte_budget", "rq_model_cache_clear", "rq_model_destroy",
+           "rq_warm_cache_create", "rq_warm_cache_store", "rq_warm_cache_nearest",
+           "rq_warm_cache_size", "rq_warm_cache_retained_bytes", "rq_warm_cache_set_byte_budget",
+           "rq_warm_cache_clear", "rq_warm_cache_entry_bytes", "rq_warm_cache_destroy"]
 
 _lib = None
 
@@ -273,6 +279,112 @@ class ForwardBatchResult:
         return np.asarray([int(s) for s in self.statuses], dtype=np.int32)
 
 
+class Problem:
+    """The problem inputs flattened into the C-ABI descriptors (validated as
+    the reference's constructors do, ForwardBatch.hpp:34-50)."""
+
+    def __init__(self, mesh: Mesh, param_set: InferenceParameterSet,
+                 obs_op: ObservationOperator, bcs: FlowBoundaryConditions, nu: float,
+                 settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
+                 p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
+                 mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
+                 grid_blocks: int = 0, owner: str = "ForwardBatch"):
+        settings = settings if settings is not None else SolverSettings()
+        settings.validate()
+        param_set.validate()
+        bcs.validate(mesh)
+        mesh.validate_2d()
+        if not mesh.wall_distance_valid():
+            raise ValueError(f"{owner} needs compute_wall_distance before construction")
+        if not math.isfinite(nu) or nu <= 0:
+            raise ValueError("SIMPLE kinematic viscosity must be finite and positive")
+        if isinstance(u_init, (tuple, list)):
+            u_init = Vec3(*u_init)
+        if u_init.z != 0.0:
+            raise ValueError("planar solver: initial velocity must have zero z component")
+        prepared = obs_op.prepare(mesh)
+        self.keep = keep = _Keep()
+        self.m = mesh_struct(mesh, keep)
+        b = self.b = RqBc()
+        ut, uv, pt, pv, kt, kv, wt, wv = bcs.arrays()
+        b.u_type, b.p_type, b.k_type, b.w_type = keep.i(ut), keep.i(pt), keep.i(kt), keep.i(wt)
+        b.u_val, b.p_val, b.k_val, b.w_val = keep.d(uv), keep.d(pv), keep.d(kv), keep.d(wv)
+        p = self.p = RqParams()
+        p.n_active = param_set.n_active()
+        p.active_indices = keep.i(param_set.active_indices)
+        p.base = keep.d(param_set.base)
+        p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
+        self.o = obs_struct(prepared, keep)
+        self.s = settings_struct(settings, mixed_precision, block_size, profile, grid_blocks)
+        self.init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
+        self.nu = float(nu)
+
+    def args(self):
+        return (C.byref(self.m), C.byref(self.p), C.byref(self.o), C.byref(self.b), self.nu,
+                C.byref(self.s), self.init.ctypes.data_as(_dp))
+
+
+class BatchBuffers:
+    """Host arrays behind one rq_batch_out of n rows, and their decoding into
+    a ForwardBatchResult."""
+
+    stride = 256
+
+    def __init__(self, n: int, nobs: int, n_cells: int, n_faces: int, fields: bool = False):
+        self.n, self.nobs = n, nobs
+        self.status = np.zeros(n, np.int32)
+        self.loglik = np.zeros(n)
+        self.pred = np.full((n, max(nobs, 0)), np.nan)
+        self.iters = np.zeros(n, np.int32)
+        self.worst = np.zeros(n)
+        self.rate = np.zeros(n)
+        self.stalled = np.zeros(n, np.uint8)
+        self.conv = np.zeros(n, np.uint8)
+        self.kry = np.zeros((n, 5), np.int64)
+        self.cyc = np.zeros((n, 16), np.int64)
+        self.errs = C.create_string_buffer(max(1, n * self.stride))
+        self.fld = np.zeros((n, 10 * n_cells + n_faces)) if fields else None
+        out = self.out = RqBatchOut()
+        out.status = self.status.ctypes.data_as(_ip)
+        out.loglik = self.loglik.ctypes.data_as(_dp)
+        out.predictions = self.pred.ctypes.data_as(_dp)
+        out.iterations = self.iters.ctypes.data_as(_ip)
+        out.final_worst_measure = self.worst.ctypes.data_as(_dp)
+        out.contraction_rate_u = self.rate.ctypes.data_as(_dp)
+        out.stalled = self.stalled.ctypes.data_as(_u8p)
+        out.converged = self.conv.ctypes.data_as(_u8p)
+        out.row_errors = C.cast(self.errs, C.c_char_p)
+        out.row_error_stride = self.stride
+        out.fields = self.fld.ctypes.data_as(_dp) if fields else None
+        out.krylov_iterations = self.kry.ctypes.data_as(_i64p)
+        out.phase_cycles = self.cyc.ctypes.data_as(_i64p)
+
+    @staticmethod
+    def check(rc: int, err) -> None:
+        if rc == 1:
+            raise ValueError(err.value.decode())
+        if rc == 3:
+            raise MemoryError(err.value.decode())
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+
+    def result(self, kernel_ms: float = 0.0) -> "ForwardBatchResult":
+        n, st = self.n, self.stride
+        raw = self.errs.raw
+        errors = [raw[i * st:(i + 1) * st].split(b"\0", 1)[0].decode() for i in range(n)]
+        statuses = [EvaluationStatus(int(v)) for v in self.status]
+        rows = []
+        for i in range(n):
+            res = EvaluationResult(statuses[i], float(self.loglik[i]), self.pred[i].copy(),
+                                   int(self.iters[i]))
+            rows.append(ForwardBatchRow(res, float(self.worst[i]), float(self.rate[i]),
+                                        bool(self.stalled[i]), errors[i]))
+        return ForwardBatchResult(self.nobs, rows, self.pred, self.loglik, statuses, self.iters,
+                                  errors, self.conv.astype(bool), self.worst, self.rate,
+                                  self.stalled.astype(bool), self.kry, self.fld, kernel_ms,
+                                  self.cyc)
+
+
 class ForwardBatch:
     """Synchronous coefficient batch on one GPU (ForwardBatch.hpp:23-125)."""
 
@@ -282,42 +394,15 @@ class ForwardBatch:
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
                  mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
                  grid_blocks: int = 0):
-        settings = settings if settings is not None else SolverSettings()
-        settings.validate()
-        param_set.validate()
-        bcs.validate(mesh)
-        mesh.validate_2d()
-        if not mesh.wall_distance_valid():
-            raise ValueError("ForwardBatch needs compute_wall_distance before construction")
-        if not math.isfinite(nu) or nu <= 0:
-            raise ValueError("SIMPLE kinematic viscosity must be finite and positive")
-        if isinstance(u_init, (tuple, list)):
-            u_init = Vec3(*u_init)
-        if u_init.z != 0.0:
-            raise ValueError("planar solver: initial velocity must have zero z component")
         self.lib = load_library()
+        pr = Problem(mesh, param_set, obs_op, bcs, nu, settings, u_init, p_init, k_init,
+                     omega_init, mixed_precision, block_size, profile, grid_blocks, "ForwardBatch")
         self.param_set = param_set
         self.n_obs = obs_op.n_obs()
         self.n_cells = mesh.n_cells
         self.n_faces = mesh.n_faces
-        prepared = obs_op.prepare(mesh)
-        keep = _Keep()
-        m = mesh_struct(mesh, keep)
-        b = RqBc()
-        ut, uv, pt, pv, kt, kv, wt, wv = bcs.arrays()
-        b.u_type, b.p_type, b.k_type, b.w_type = keep.i(ut), keep.i(pt), keep.i(kt), keep.i(wt)
-        b.u_val, b.p_val, b.k_val, b.w_val = keep.d(uv), keep.d(pv), keep.d(kv), keep.d(wv)
-        p = RqParams()
-        p.n_active = param_set.n_active()
-        p.active_indices = keep.i(param_set.active_indices)
-        p.base = keep.d(param_set.base)
-        p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
-        o = obs_struct(prepared, keep)
-        s = settings_struct(settings, mixed_precision, block_size, profile, grid_blocks)
-        init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
         err = C.create_string_buffer(512)
-        h = self.lib.rq_engine_create(C.byref(m), C.byref(p), C.byref(o), C.byref(b), float(nu),
-                                      C.byref(s), init.ctypes.data_as(_dp), err, 512)
+        h = self.lib.rq_engine_create(*pr.args(), err, 512)
         if not h:
             raise ValueError(err.value.decode())
         self._h = h
@@ -351,55 +436,13 @@ class ForwardBatch:
         if workers < 1:
             raise ValueError("workers must be positive")
         theta = np.ascontiguousarray(theta)
-        n, nobs = theta.shape[0], self.n_obs
-        status = np.zeros(n, np.int32)
-        loglik = np.zeros(n)
-        pred = np.full((n, max(nobs, 0)), np.nan)
-        iters = np.zeros(n, np.int32)
-        worst = np.zeros(n)
-        rate = np.zeros(n)
-        stalled = np.zeros(n, np.uint8)
-        conv = np.zeros(n, np.uint8)
-        kry = np.zeros((n, 5), np.int64)
-        cyc = np.zeros((n, 16), np.int64)
-        stride = 256
-        errs = C.create_string_buffer(max(1, n * stride))
-        fld = np.zeros((n, 10 * self.n_cells + self.n_faces)) if fields else None
-        out = RqBatchOut()
-        out.status = status.ctypes.data_as(_ip)
-        out.loglik = loglik.ctypes.data_as(_dp)
-        out.predictions = pred.ctypes.data_as(_dp)
-        out.iterations = iters.ctypes.data_as(_ip)
-        out.final_worst_measure = worst.ctypes.data_as(_dp)
-        out.contraction_rate_u = rate.ctypes.data_as(_dp)
-        out.stalled = stalled.ctypes.data_as(_u8p)
-        out.converged = conv.ctypes.data_as(_u8p)
-        out.row_errors = C.cast(errs, C.c_char_p)
-        out.row_error_stride = stride
-        out.fields = fld.ctypes.data_as(_dp) if fields else None
-        out.krylov_iterations = kry.ctypes.data_as(_i64p)
-        out.phase_cycles = cyc.ctypes.data_as(_i64p)
+        bb = BatchBuffers(theta.shape[0], self.n_obs, self.n_cells, self.n_faces, fields)
         err = C.create_string_buffer(512)
-        rc = self.lib.rq_engine_evaluate(self._h, theta.ctypes.data_as(_dp), n,
-                                         self.n_parameters(), int(workers), C.byref(out), err, 512)
-        if rc == 1:
-            raise ValueError(err.value.decode())
-        if rc == 3:
-            raise MemoryError(err.value.decode())
-        if rc != 0:
-            raise RuntimeError(err.value.decode())
-        raw = errs.raw
-        errors = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(n)]
-        rows = []
-        statuses = [EvaluationStatus(int(v)) for v in status]
-        for i in range(n):
-            res = EvaluationResult(statuses[i], float(loglik[i]), pred[i].copy(), int(iters[i]))
-            rows.append(ForwardBatchRow(res, float(worst[i]), float(rate[i]), bool(stalled[i]),
-                                        errors[i]))
-        timing = self.last_timing()
-        return ForwardBatchResult(nobs, rows, pred, loglik, statuses, iters, errors,
-                                  conv.astype(bool), worst, rate, stalled.astype(bool), kry,
-                                  fld, timing["kernel_ms"], cyc)
+        rc = self.lib.rq_engine_evaluate(self._h, theta.ctypes.data_as(_dp), theta.shape[0],
+                                         self.n_parameters(), int(workers), C.byref(bb.out), err,
+                                         512)
+        BatchBuffers.check(rc, err)
+        return bb.result(self.last_timing()["kernel_ms"])
 
 
 def build_info() -> str:

@@ -146,9 +146,20 @@ struct DevSlot {
     const DevMesh *mesh;
     double *arena;          // used when the arena lives in global memory
     double *amg;            // multigrid workspace (mesh.amg.workspace doubles) or null
+    const double *warm;     // flow state to start from (warm start) or null
+    double *save;           // where to write the final flow state, or null
     double coeffs[11];
     int row;                // output row index
 };
+
+// A flow state (warm start source and export), in doubles: the interior
+// fields Ux, Uy, p, k, omega, nuT, F1, F2, Pk, CDkw (nc each), the boundary
+// values Ux, Uy, p, k, omega (max(nb, 1) each), then the turbulence-
+// established flag (FlowFields, core/include/FlowFields.hpp:12-35).
+constexpr int kStateInterior = 10, kStateBoundary = 5;
+inline int64_t state_doubles(int nc, int nb) {
+    return (int64_t)kStateInterior * nc + (int64_t)kStateBoundary * (nb > 1 ? nb : 1) + 2;
+}
 
 // Per-draw results written by the kernel.
 struct DevResult {

@@ -17,6 +17,7 @@
 
 #include "../../include/rans_amd.h"
 #include "rans_device.h"
+#include "warm_cache.h"
 
 namespace rq {
 hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int nslots,
@@ -949,8 +950,11 @@ int admit_row(const rq_engine *E, const double *row, double *coeffs, std::string
     return 0;
 }
 
+// warm/save: per input row, the device flow state a row starts from and the
+// device buffer its final state is written to (either may be null)
 int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
-                  int32_t workers, rq_batch_out *out) {
+                  int32_t workers, rq_batch_out *out, const double *const *warm = nullptr,
+                  double *const *save = nullptr) {
     if (!E || !out || !out->status) throw Fail{RQ_EINVAL, "null argument"};
     if (workers < 1) throw Fail{RQ_EINVAL, "workers must be positive"};
     if (n_params != E->n_active)
@@ -988,6 +992,8 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
             continue;
         }
         sl.row = (int)slots.size();
+        sl.warm = warm ? warm[r] : nullptr;
+        sl.save = save ? save[r] : nullptr;
         valid.push_back(r);
         slots.push_back(sl);
     }
@@ -1099,6 +1105,132 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
             std::memcpy(out->fields + r * fstride, fields.data() + (size_t)i * fstride,
                         fstride * sizeof(double));
     }
+    return RQ_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Single-row model with a warm-start cache: ForwardModel
+// (rans-uq/include/ForwardModel.hpp:191-247, rans-uq/src/ForwardModel.cpp:35-137).
+// Converged flow states stay in device memory; a warm-started row copies the
+// nearest cached state into its arena instead of the uniform initialisation.
+// ----------------------------------------------------------------------------
+struct rq_model {
+    rq_engine *E = nullptr;
+    rq::WarmCache cache;
+    int64_t state_n = 0;              // doubles per flow state
+    std::vector<double *> pool;       // device state buffers not in the cache
+    std::vector<double *> owned;      // every device state buffer
+    explicit rq_model(const rq::WarmCache &c) : cache(c) {}
+    ~rq_model() {
+        for (double *p : owned) (void)hipFree(p);
+        delete E;
+    }
+    double *take() {
+        if (!pool.empty()) {
+            double *p = pool.back();
+            pool.pop_back();
+            return p;
+        }
+        double *p = nullptr;
+        if (hipMalloc(&p, (size_t)state_n * sizeof(double)) != hipSuccess)
+            throw Fail{RQ_ENOMEM, "device allocation of a flow state failed"};
+        owned.push_back(p);
+        return p;
+    }
+    void give_back(const std::vector<void *> &freed) {
+        for (void *p : freed) pool.push_back(static_cast<double *>(p));
+    }
+    size_t entry_bytes() const {
+        return rq::WarmCache::entry_bytes((size_t)E->n_active, E->nc, E->nb);
+    }
+};
+
+namespace {
+
+// Rows whose coefficients fail the host checks take no cache lookup
+// (ForwardModel.cpp:45-49 returns before it).
+bool row_admissible(const rq_engine *E, const double *row) {
+    double coeffs[11];
+    std::string msg;
+    return admit_row(E, row, coeffs, msg) == 0;
+}
+
+// batch = false: rows one after another, each looking up the cache as it
+// stands after the rows before it (the reference's sequence of evaluate
+// calls); batch = true: every row looks up the cache as it stands at the
+// call, all rows run in one launch, and converged rows are stored in row
+// order afterwards.
+int model_evaluate_impl(rq_model *M, const double *theta, int64_t n_rows, int32_t n_params,
+                        rq_batch_out *out, uint8_t *used, bool batch) {
+    if (!M || !out || !out->status) throw Fail{RQ_EINVAL, "null argument"};
+    rq_engine *E = M->E;
+    if (n_params != E->n_active)
+        throw Fail{RQ_EINVAL, "batch rows must all have n_parameters entries"};
+    if (n_rows < 0) throw Fail{RQ_EINVAL, "row count must be nonnegative"};
+    if (n_rows > 0 && !theta) throw Fail{RQ_EINVAL, "null coefficient rows"};
+    HIPCHECK(hipSetDevice(E->device));
+    auto after = [&](int64_t r, double *save) {
+        std::vector<void *> freed;
+        if (out->status[r] == ST_CONVERGED &&
+            M->cache.store(theta + r * n_params, (size_t)n_params, save, M->entry_bytes(), freed)) {
+        } else {
+            freed.push_back(save);
+        }
+        M->give_back(freed);
+    };
+    if (!batch) {
+        for (int64_t r = 0; r < n_rows; ++r) {
+            const double *row = theta + r * n_params;
+            const rq::WarmCache::Entry *e =
+                row_admissible(E, row) ? M->cache.nearest(row, (size_t)n_params) : nullptr;
+            if (used) used[r] = e != nullptr;
+            const double *warm = e ? static_cast<const double *>(e->payload) : nullptr;
+            double *save = M->take();
+            rq_batch_out o1 = *out;
+            const int64_t nobs = E->nobs;
+            o1.status = out->status + r;
+            if (out->loglik) o1.loglik = out->loglik + r;
+            if (out->predictions) o1.predictions = out->predictions + r * nobs;
+            if (out->iterations) o1.iterations = out->iterations + r;
+            if (out->final_worst_measure) o1.final_worst_measure = out->final_worst_measure + r;
+            if (out->contraction_rate_u) o1.contraction_rate_u = out->contraction_rate_u + r;
+            if (out->stalled) o1.stalled = out->stalled + r;
+            if (out->converged) o1.converged = out->converged + r;
+            if (out->row_errors && out->row_error_stride)
+                o1.row_errors = out->row_errors + r * out->row_error_stride;
+            if (out->fields) o1.fields = out->fields + r * (10 * (int64_t)E->nc + E->nf);
+            if (out->krylov_iterations) o1.krylov_iterations = out->krylov_iterations + r * 5;
+            if (out->phase_cycles) o1.phase_cycles = out->phase_cycles + r * kPhases;
+            try {
+                evaluate_impl(E, row, 1, n_params, 1, &o1, &warm, &save);
+            } catch (...) {
+                M->pool.push_back(save);
+                throw;
+            }
+            after(r, save);
+        }
+        return RQ_OK;
+    }
+    std::vector<const double *> warm(n_rows, nullptr);
+    std::vector<double *> save(n_rows, nullptr);
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *row = theta + r * n_params;
+        const rq::WarmCache::Entry *e =
+            row_admissible(E, row) ? M->cache.nearest(row, (size_t)n_params) : nullptr;
+        if (used) used[r] = e != nullptr;
+        warm[r] = e ? static_cast<const double *>(e->payload) : nullptr;
+    }
+    try {
+        for (int64_t r = 0; r < n_rows; ++r) save[r] = M->take();
+        evaluate_impl(E, theta, n_rows, n_params, 1, out, warm.data(), save.data());
+    } catch (...) {
+        for (double *p : save)
+            if (p) M->pool.push_back(p);
+        throw;
+    }
+    for (int64_t r = 0; r < n_rows; ++r) after(r, save[r]);
     return RQ_OK;
 }
 
@@ -1479,4 +1611,140 @@ int rq_server_stats(const rq_server *S, int32_t *slots, int64_t *slot_arena_byte
 }
 
 void rq_server_destroy(rq_server *S) { delete S; }
+
+// ---------------------------------------------------------------- model
+rq_model *rq_model_create(const rq_mesh_desc *mesh, const rq_param_desc *params,
+                          const rq_obs_desc *obs, const rq_bc_desc *bcs, double nu,
+                          const rq_settings *settings, const double init[5],
+                          const rq_cache_options *cache, char *err, size_t errcap) {
+    try {
+        rq::WarmCache c = cache ? rq::WarmCache(cache->max_size, cache->threshold,
+                                                cache->byte_budget < 0 ? 0 : (size_t)cache->byte_budget)
+                                : rq::WarmCache();
+        if (cache && cache->byte_budget < 0)
+            throw Fail{RQ_EINVAL, "Invalid warm-start cache limits"};
+        auto *M = new rq_model(c);
+        try {
+            M->E = create_impl(mesh, params, obs, bcs, nu, settings, init);
+        } catch (...) {
+            delete M;
+            throw;
+        }
+        M->state_n = state_doubles(M->E->nc, M->E->nb);
+        set_err(err, errcap, "");
+        return M;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+    } catch (const std::invalid_argument &e) {
+        set_err(err, errcap, e.what());
+    } catch (const std::bad_alloc &) {
+        set_err(err, errcap, "host allocation failed");
+    } catch (const std::exception &e) {
+        set_err(err, errcap, e.what());
+    }
+    return nullptr;
+}
+
+static int model_call(rq_model *m, const double *theta, int64_t n_rows, int32_t n_params,
+                      rq_batch_out *out, uint8_t *used, bool batch, char *err, size_t errcap) {
+    try {
+        int rc = model_evaluate_impl(m, theta, n_rows, n_params, out, used, batch);
+        set_err(err, errcap, "");
+        return rc;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+        return f.code;
+    } catch (const std::bad_alloc &) {
+        set_err(err, errcap, "host allocation failed");
+        return RQ_ENOMEM;
+    } catch (const std::exception &e) {
+        set_err(err, errcap, e.what());
+        return RQ_EDEVICE;
+    }
+}
+
+int rq_model_evaluate(rq_model *m, const double *theta, int64_t n_rows, int32_t n_params,
+                      rq_batch_out *out, uint8_t *used_warm_start, char *err, size_t errcap) {
+    return model_call(m, theta, n_rows, n_params, out, used_warm_start, false, err, errcap);
+}
+
+int rq_model_evaluate_batch(rq_model *m, const double *theta, int64_t n_rows, int32_t n_params,
+                            rq_batch_out *out, uint8_t *used_warm_start, char *err,
+                            size_t errcap) {
+    return model_call(m, theta, n_rows, n_params, out, used_warm_start, true, err, errcap);
+}
+
+const rq_engine *rq_model_engine(const rq_model *m) { return m ? m->E : nullptr; }
+int32_t rq_model_cache_size(const rq_model *m) { return m ? m->cache.size() : -1; }
+int64_t rq_model_cache_retained_bytes(const rq_model *m) {
+    return m ? (int64_t)m->cache.retained_bytes() : -1;
+}
+int64_t rq_model_cache_byte_budget(const rq_model *m) {
+    return m ? (int64_t)m->cache.byte_budget() : -1;
+}
+int rq_model_cache_set_byte_budget(rq_model *m, int64_t bytes) {
+    if (!m || bytes < 0) return RQ_EINVAL;
+    std::vector<void *> freed;
+    m->cache.set_byte_budget((size_t)bytes, freed);
+    m->give_back(freed);
+    return RQ_OK;
+}
+int rq_model_cache_clear(rq_model *m) {
+    if (!m) return RQ_EINVAL;
+    std::vector<void *> freed;
+    m->cache.clear(freed);
+    m->give_back(freed);
+    return RQ_OK;
+}
+void rq_model_destroy(rq_model *m) { delete m; }
+
+// ------------------------------------------- cache bookkeeping without a GPU
+struct rq_warm_cache {
+    rq::WarmCache c;
+};
+
+rq_warm_cache *rq_warm_cache_create(int32_t max_size, double threshold, int64_t byte_budget,
+                                    char *err, size_t errcap) {
+    try {
+        if (byte_budget < 0) throw std::invalid_argument("Invalid warm-start cache limits");
+        auto *w = new rq_warm_cache{rq::WarmCache(max_size, threshold, (size_t)byte_budget)};
+        set_err(err, errcap, "");
+        return w;
+    } catch (const std::exception &e) {
+        set_err(err, errcap, e.what());
+        return nullptr;
+    }
+}
+int32_t rq_warm_cache_store(rq_warm_cache *w, const double *theta, int32_t n, int64_t tag,
+                            int64_t bytes) {
+    if (!w || n < 0 || (n > 0 && !theta) || bytes < 0) return -1;
+    std::vector<void *> freed;
+    return w->c.store(theta, (size_t)n, reinterpret_cast<void *>(tag), (size_t)bytes, freed) ? 1
+                                                                                           : 0;
+}
+int64_t rq_warm_cache_nearest(const rq_warm_cache *w, const double *theta, int32_t n) {
+    if (!w || n < 0 || (n > 0 && !theta)) return -1;
+    const rq::WarmCache::Entry *e = w->c.nearest(theta, (size_t)n);
+    return e ? reinterpret_cast<int64_t>(e->payload) : -1;
+}
+int32_t rq_warm_cache_size(const rq_warm_cache *w) { return w ? w->c.size() : -1; }
+int64_t rq_warm_cache_retained_bytes(const rq_warm_cache *w) {
+    return w ? (int64_t)w->c.retained_bytes() : -1;
+}
+int rq_warm_cache_set_byte_budget(rq_warm_cache *w, int64_t bytes) {
+    if (!w || bytes < 0) return RQ_EINVAL;
+    std::vector<void *> freed;
+    w->c.set_byte_budget((size_t)bytes, freed);
+    return RQ_OK;
+}
+int rq_warm_cache_clear(rq_warm_cache *w) {
+    if (!w) return RQ_EINVAL;
+    std::vector<void *> freed;
+    w->c.clear(freed);
+    return RQ_OK;
+}
+int64_t rq_warm_cache_entry_bytes(int32_t n_theta, int32_t n_cells, int32_t n_boundary) {
+    return (int64_t)rq::WarmCache::entry_bytes((size_t)n_theta, n_cells, n_boundary);
+}
+void rq_warm_cache_destroy(rq_warm_cache *w) { delete w; }
 }  // extern "C"

@@ -1977,23 +1977,47 @@ struct Draw {
     // ----------------------------------------------------------- the solve
     // SIMPLESolver::solve (SIMPLESolver.cpp:1193-1716) after initUniform
     // (SIMPLESolver.cpp:267-298).
-    DEV void run() {
-        const double nu = S.nu;
+    // warm start: the cached state replaces the uniform initialisation and
+    // the boundary-condition pass (ForwardModel.cpp:85-88: a warm-started
+    // solve begins from the stored fields, boundary values and turbulence
+    // flag as they are)
+    const double *warm = nullptr;
+    DEV void load_state() {
+        const int64_t nbo = (int64_t)kStateInterior * nc;
+        const int vars[kStateInterior] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
+        const int bvars[kStateBoundary] = {B_Ux, B_Uy, B_p, B_k, B_w};
         for (int ci = gt; ci < nc; ci += gs) {
-            C(ci, V_Ux) = S.init_ux;
-            C(ci, V_Uy) = S.init_uy;
-            C(ci, V_p) = S.init_p;
-            C(ci, V_k) = S.init_k;
-            C(ci, V_F1) = 1.0;
-            C(ci, V_F2) = 1.0;
-            C(ci, V_Pk) = 0.0;
-            C(ci, V_CDkw) = 0.0;
-            const double y = smax(C(ci, V_wd), 1e-20);
-            const double om = smax(resolved_wall_omega(y), S.init_w);
-            C(ci, V_w) = om;
-            C(ci, V_nuT) = c[9] * S.init_k / smax(c[9] * om, 1e-20);
+#pragma unroll
+            for (int a = 0; a < kStateInterior; ++a) C(ci, vars[a]) = warm[(int64_t)a * nc + ci];
             const double wy = M.wall_y[ci];
             C(ci, V_wallv) = isfinite(wy) ? resolved_wall_omega(wy) : NAN;
+        }
+        for (int b = gt; b < nb; b += gs)
+#pragma unroll
+            for (int a = 0; a < kStateBoundary; ++a) Bd(b, bvars[a]) = warm[nbo + (int64_t)a * nbs + b];
+    }
+
+    DEV void run() {
+        const double nu = S.nu;
+        if (warm) {
+            load_state();
+        } else {
+            for (int ci = gt; ci < nc; ci += gs) {
+                C(ci, V_Ux) = S.init_ux;
+                C(ci, V_Uy) = S.init_uy;
+                C(ci, V_p) = S.init_p;
+                C(ci, V_k) = S.init_k;
+                C(ci, V_F1) = 1.0;
+                C(ci, V_F2) = 1.0;
+                C(ci, V_Pk) = 0.0;
+                C(ci, V_CDkw) = 0.0;
+                const double y = smax(C(ci, V_wd), 1e-20);
+                const double om = smax(resolved_wall_omega(y), S.init_w);
+                C(ci, V_w) = om;
+                C(ci, V_nuT) = c[9] * S.init_k / smax(c[9] * om, 1e-20);
+                const double wy = M.wall_y[ci];
+                C(ci, V_wallv) = isfinite(wy) ? resolved_wall_omega(wy) : NAN;
+            }
         }
         if (tid == 0) {
             bk->done = 0;
@@ -2002,7 +2026,8 @@ struct Draw {
             bk->finalIter = 0;
             bk->stalled = 0;
             bk->haveQPrev = 0;
-            bk->turbEst = 0;
+            bk->turbEst = warm ? (warm[(int64_t)kStateInterior * nc +
+                                       (int64_t)kStateBoundary * nbs] != 0.0) : 0;
             bk->lastK = bk->lastOm = 1.0;
             bk->normUx0 = bk->normUy0 = bk->normP0 = 1.0;
             bk->pfRef = 0.0;
@@ -2014,17 +2039,19 @@ struct Draw {
             bk->tlast = clock64();
         }
         sync();
-        bc_velocity();
-        bc_pressure();
-        bc_k();
-        bc_omega(0.075);
-        sync();
+        if (!warm) {
+            bc_velocity();
+            bc_pressure();
+            bc_k();
+            bc_omega(0.075);
+            sync();
+        }
         for (int f = gt; f < nf; f += gs) Fc(f, W_phi) = interp_flux(f);
         sync();
 
         const double tol = S.convergence_tol;
         const double negligible = 0.01 * tol;
-        int turbEst = 0;
+        int turbEst = bk->turbEst;
         for (int iter = 0; iter < S.max_iterations; ++iter) {
             for (int ci = gt; ci < nc; ci += gs) {
                 C(ci, V_UxP) = C(ci, V_Ux);
@@ -2533,7 +2560,8 @@ template <int BS, bool kLDS, bool kGrid = false>
 DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &settings,
                     const double *coeffSrc, double *arena, double *amg, DevResult *res,
                     double *pred, double *fields, GridState *grid = nullptr, int rank = 0,
-                    int nblk = 1, double *aux = nullptr) {
+                    int nblk = 1, double *aux = nullptr, const double *warm = nullptr,
+                    double *save = nullptr) {
     if (threadIdx.x < 11) sh.coeffs[threadIdx.x] = coeffSrc[threadIdx.x];
     if (threadIdx.x == 0) sh.sset = settings;
     const int g0 = rank * BS + threadIdx.x, gstride = nblk * BS;
@@ -2568,6 +2596,7 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
     Draw<BS, kLDS, kGrid> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
                             aux);
     d.cbuf = sh.coarse_b;
+    d.warm = warm;
     if constexpr (kGrid) {
         d.rank = rank;
         d.nblk = nblk;
@@ -2604,6 +2633,20 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
             fields[10 * nc + f] = ld(M.lay.face + (kLDS ? (int64_t)f * kNF + W_phi
                                                         : (int64_t)W_phi * M.nf + f));
     }
+    if (save) {  // the final flow state (its layout: state_doubles)
+        const int nc = M.nc, nb = M.nb, nbs = nb > 1 ? nb : 1;
+        const int64_t nbo = (int64_t)kStateInterior * nc;
+        const int vars[kStateInterior] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
+        const int bvars[kStateBoundary] = {B_Ux, B_Uy, B_p, B_k, B_w};
+        for (int a = 0; a < kStateInterior; ++a)
+            for (int ci = g0; ci < nc; ci += gstride) save[(int64_t)a * nc + ci] = d.C(ci, vars[a]);
+        for (int a = 0; a < kStateBoundary; ++a)
+            for (int b = g0; b < nb; b += gstride) save[nbo + (int64_t)a * nbs + b] = d.Bd(b, bvars[a]);
+        if (rank == 0 && threadIdx.x == 0) {
+            save[nbo + (int64_t)kStateBoundary * nbs] = sh.book.turbEst ? 1.0 : 0.0;
+            save[nbo + (int64_t)kStateBoundary * nbs + 1] = 0.0;
+        }
+    }
     if (rank == 0 && threadIdx.x == 0) {
         DevResult &r = *res;
         r.status = book.status;
@@ -2638,7 +2681,8 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
     const int row = slot.row;
     solve_draw<BS, kLDS>(sh, *slot.mesh, settings, slot.coeffs, kLDS ? smem : slot.arena,
                          slot.amg, results + row, predictions + (int64_t)row * pred_stride,
-                         fields ? fields + row * field_stride : nullptr);
+                         fields ? fields + row * field_stride : nullptr, nullptr, 0, 1, nullptr,
+                         slot.warm, slot.save);
 }
 
 // The multi-workgroup batch kernel: nblk co-resident workgroups per draw
@@ -2665,7 +2709,7 @@ __global__ void __launch_bounds__(BS) grid_solve_kernel(const DevSlot *__restric
     solve_draw<BS, false, true>(sh, *slot.mesh, settings, slot.coeffs, slot.arena, slot.amg,
                                 results + row, predictions + (int64_t)row * pred_stride,
                                 fields ? fields + row * field_stride : nullptr, states + draw,
-                                rank, nblk, smem);
+                                rank, nblk, smem, slot.warm, slot.save);
 }
 
 // ----------------------------------------------------------------------------

@@ -206,6 +206,75 @@ int rq_server_stats(const rq_server *server, int32_t *slots, int64_t *slot_arena
 
 void rq_server_destroy(rq_server *server);
 
+/* ---------------------------------------------------------------------------
+ * Single-row model with a warm-start cache: ForwardModel
+ * (rans-uq/include/ForwardModel.hpp:191-247, rans-uq/src/ForwardModel.cpp:35-137)
+ * and its WarmStartCache (ForwardModel.hpp:21-157). Converged flow states stay
+ * in device memory; a row whose coefficients lie within the threshold of a
+ * cached row starts from the nearest cached state instead of the uniform
+ * initialisation.
+ * ------------------------------------------------------------------------- */
+typedef struct rq_model rq_model;
+
+/* WarmStartCache(maxSize, threshold, byteBudget) (ForwardModel.hpp:33-37);
+ * ForwardModel's defaults are 50, 0.5 and 64 MiB (ForwardModel.cpp:17). */
+typedef struct rq_cache_options {
+    int32_t max_size;
+    double threshold;
+    int64_t byte_budget;
+} rq_cache_options;
+
+/* ForwardModel constructor; cache NULL = the defaults. */
+rq_model *rq_model_create(const rq_mesh_desc *mesh, const rq_param_desc *params,
+                          const rq_obs_desc *obs, const rq_bc_desc *bcs, double nu,
+                          const rq_settings *settings, const double init[5],
+                          const rq_cache_options *cache, char *err, size_t errcap);
+
+/* ForwardModel::evaluateWithDiagnostics on each row in turn (each row sees
+ * the states the rows before it stored); used_warm_start (may be NULL) gets
+ * ForwardEvaluationResult::usedWarmStart per row. out as for
+ * rq_engine_evaluate. */
+int rq_model_evaluate(rq_model *model, const double *theta, int64_t n_rows, int32_t n_params,
+                      rq_batch_out *out, uint8_t *used_warm_start, char *err, size_t errcap);
+
+/* No counterpart in the reference: every row looks up the cache as it stands
+ * at the call, all rows run in one launch, and the converged ones are stored
+ * afterwards in row order (for callers that evaluate a stencil of
+ * neighbouring points at once, such as a pattern-search poll). */
+int rq_model_evaluate_batch(rq_model *model, const double *theta, int64_t n_rows,
+                            int32_t n_params, rq_batch_out *out, uint8_t *used_warm_start,
+                            char *err, size_t errcap);
+
+/* The model's engine, for rq_engine_n_* and rq_engine_last_timing. */
+const rq_engine *rq_model_engine(const rq_model *model);
+
+/* WarmStartCache::size / retainedBytes / byteBudget / setByteBudget / clear
+ * (ForwardModel.hpp:93-116). */
+int32_t rq_model_cache_size(const rq_model *model);
+int64_t rq_model_cache_retained_bytes(const rq_model *model);
+int64_t rq_model_cache_byte_budget(const rq_model *model);
+int rq_model_cache_set_byte_budget(rq_model *model, int64_t bytes);
+int rq_model_cache_clear(rq_model *model);
+void rq_model_destroy(rq_model *model);
+
+/* The cache's bookkeeping on its own, without a GPU: entries carry an integer
+ * tag in place of a flow state. Same rules as rq_model's cache
+ * (WarmStartCache::store / nearest / setByteBudget / clear / entryBytes). */
+typedef struct rq_warm_cache rq_warm_cache;
+rq_warm_cache *rq_warm_cache_create(int32_t max_size, double threshold, int64_t byte_budget,
+                                    char *err, size_t errcap);
+/* 1 stored, 0 not retained (non-finite row, zero capacity or over budget). */
+int32_t rq_warm_cache_store(rq_warm_cache *cache, const double *theta, int32_t n, int64_t tag,
+                            int64_t bytes);
+/* Tag of the nearest entry within the threshold, or -1. */
+int64_t rq_warm_cache_nearest(const rq_warm_cache *cache, const double *theta, int32_t n);
+int32_t rq_warm_cache_size(const rq_warm_cache *cache);
+int64_t rq_warm_cache_retained_bytes(const rq_warm_cache *cache);
+int rq_warm_cache_set_byte_budget(rq_warm_cache *cache, int64_t bytes);
+int rq_warm_cache_clear(rq_warm_cache *cache);
+int64_t rq_warm_cache_entry_bytes(int32_t n_theta, int32_t n_cells, int32_t n_boundary);
+void rq_warm_cache_destroy(rq_warm_cache *cache);
+
 #ifdef __cplusplus
 }
 #endif

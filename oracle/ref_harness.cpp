@@ -23,6 +23,8 @@
 //   params idx...
 //   theta v...                                 one coefficient row (repeatable)
 //   dump <prefix>                              binary field dump per row
+//   warm_cache <bytes>                         keep ForwardModel's warm-start cache with
+//                                              this byte budget (default: off)
 #include "BoundaryCondition.hpp"
 #include "FlowFields.hpp"
 #include "ForwardModel.hpp"
@@ -125,6 +127,7 @@ int main(int argc, char **argv) {
     std::vector<int> params;
     std::vector<std::vector<double>> thetas;
     std::string dumpPrefix, savePath;
+    long long warmBudget = 0;
 
     std::string line;
     while (std::getline(in, line)) {
@@ -197,6 +200,8 @@ int main(int argc, char **argv) {
             thetas.push_back(row);
         } else if (cmd == "dump") {
             ls >> dumpPrefix;
+        } else if (cmd == "warm_cache") {
+            ls >> warmBudget;
         } else {
             std::cerr << "unknown directive " << cmd << "\n";
             return 2;
@@ -213,7 +218,7 @@ int main(int argc, char **argv) {
     InferenceParameterSet ps = InferenceParameterSet::fromIndices("harness", params.empty() ? std::vector<int>{9, 8} : params);
 
     ForwardModel model(mesh, ps, obs, bcs, nu, s, U0, p0, k0, w0);
-    model.cache().setByteBudget(0);
+    model.cache().setByteBudget((size_t)warmBudget);
     std::cout << "{\"n_cells\": " << mesh.nCells() << ", \"n_faces\": " << mesh.nFaces()
               << ", \"n_internal\": " << mesh.nInternalFaces() << ", \"rows\": [\n";
     for (size_t r = 0; r < thetas.size(); ++r) {
@@ -229,6 +234,7 @@ int main(int argc, char **argv) {
                   << ", \"stalled\": " << (ev.convergence.stalled ? "true" : "false")
                   << ", \"converged\": " << (ev.convergence.converged ? "true" : "false")
                   << ", \"diverged\": " << (ev.convergence.diverged ? "true" : "false")
+                  << ", \"used_warm_start\": " << (ev.usedWarmStart ? "true" : "false")
                   << ", \"wall_seconds\": " << num(wall) << ", \"predictions\": [";
         for (size_t i = 0; i < res.predictions.size(); ++i)
             std::cout << (i ? ", " : "") << num(res.predictions[i]);

@@ -269,5 +269,27 @@ def main():
                                             "thetas": thetas, "rows": res["rows"]})
 
 
+def warm_fixture(tmp):
+    """ForwardModel with its warm-start cache (ForwardModel.cpp:79-117): one
+    model evaluates the rows in order; the first solves cold, later rows
+    start from the nearest converged state, the out-of-bounds row takes no
+    lookup, and the repeated row replaces its entry."""
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
+    mp = os.path.join(tmp, "ch32w.msh")
+    prob.mesh.save_binary(mp)
+    thetas = [[0.31, 0.09], [0.315, 0.0915], [0.9, 0.09], [0.31, 0.09], [0.2, 0.05]]
+    res = run_case(problem_lines(prob, mp, thetas, extra=["warm_cache 67108864"]))
+    write("ref_channel_warm.json", {"case": "channel_problem(ny=32, ratio=1.15, n_probes=30, "
+                                            "tol=1e-6, max_iterations=40000), ForwardModel "
+                                            "cache (50, 0.5, 64 MiB)",
+                                    "thetas": thetas, "rows": res["rows"]})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["warm"]:
+        if not os.path.exists(HARNESS):
+            raise SystemExit("build the reference harness first: make -C oracle ref")
+        warm_fixture(tempfile.mkdtemp())
+    else:
+        main()
+        warm_fixture(tempfile.mkdtemp())
