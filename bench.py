@@ -22,7 +22,9 @@ Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
 and FP64 defect correction), served (Poisson arrivals of single requests on
 64-512 cell channels against the plain one-launch-per-request path), hills
 (the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
---hills-full adds the 120x64 batch to convergence).
+--hills-full adds the 120x64 batch to convergence), warm_start (a
+pattern-search poll of 4 neighbours of a converged point, cold in one batch
+launch against warm-started from the device-resident state in one launch).
 
 Profiling helpers: --max-iterations caps every solve (a short, fixed amount of
 work for counter runs); --copy-calibration MIB runs only a device-to-device
@@ -315,6 +317,57 @@ def hills_leg(full, cores, cpu):
     return out
 
 
+def warm_leg(steps):
+    """Warm starts on the 1x32 channel (ratio 1.15, tol 1e-6): a converged base
+    point (0.31, 0.09), then a pattern-search poll of its 4 neighbours at a
+    relative step of 1 % (and of 10 %), solved cold in one batch launch and
+    warm-started from the base state in one launch (rq_model_evaluate_batch).
+    Each poll is repeated `steps` times; every warm poll starts from a model
+    holding only the base state. (On the 1x128 channel at tol 1e-8 the
+    reference classifies such warm starts as diverged, and so does this
+    engine.)"""
+    from cfdr_amd import cases, engine
+    from cfdr_amd.model import ForwardModel
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6,
+                                 max_iterations=40000)
+    base = np.asarray([0.31, 0.09])
+    args = (prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings, prob.u_init,
+            prob.p_init, prob.k_init, prob.omega_init)
+    batch = engine.ForwardBatch(*args)
+    out = {}
+    for label, rel in (("poll_1pct", 0.01), ("poll_10pct", 0.10)):
+        stencil = np.asarray([base * [1 + rel, 1], base * [1 - rel, 1], base * [1, 1 + rel],
+                              base * [1, 1 - rel]])
+        cold_s, warm_s = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            rc = batch.evaluate(stencil)
+            cold_s.append(time.perf_counter() - t0)
+            m = ForwardModel(*args)
+            m.evaluate(base)
+            t0 = time.perf_counter()
+            rw, used = m.evaluate_batch(stencil)
+            warm_s.append(time.perf_counter() - t0)
+        ok = (rc.status_codes == 0) & (rw.status_codes == 0)
+        err = float(np.max(np.abs(rw.predictions[ok] - rc.predictions[ok]) /
+                           np.abs(rc.predictions[ok]))) if ok.any() else None
+        out[label] = {
+            "rows": len(stencil), "warm_started": int(np.sum(used)),
+            "cold_ms": round(1e3 * statistics.mean(cold_s), 1),
+            "warm_ms": round(1e3 * statistics.mean(warm_s), 1),
+            "speedup": round(statistics.mean(cold_s) / statistics.mean(warm_s), 2),
+            "converged_cold": int(np.sum(rc.convergence_mask)),
+            "converged_warm": int(np.sum(rw.convergence_mask)),
+            "cold_iterations_mean": float(np.mean(rc.iteration_counts)),
+            "warm_iterations_mean": float(np.mean(rw.iteration_counts)),
+            "statuses_cold": [s.name for s in rc.statuses],
+            "statuses_warm": [s.name for s in rw.statuses],
+            "max_rel_diff_warm_vs_cold": err}
+        print(f"# warm {label}: cold {out[label]['cold_ms']} ms, warm {out[label]['warm_ms']} ms",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +385,7 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-hills", action="store_true")
+    ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--hills-full", action="store_true",
                     help="also run the 16-draw 120x64 hill batch to convergence (minutes)")
     args = ap.parse_args()
@@ -450,6 +504,10 @@ def main():
     if not args.no_serve:
         served = served_leg(args.serve_requests, args.serve_rate)
 
+    warm = None
+    if not args.no_warm:
+        warm = warm_leg(2)
+
     hills = None
     if not args.no_hills:
         hills = hills_leg(args.hills_full, _cores(), not args.no_cpu_baseline)
@@ -485,6 +543,7 @@ def main():
         "max_rel_err_vs_oracle": err,
         "mixed": mixed,
         "served": served,
+        "warm_start": warm,
         "hills": hills,
         "block_size": batch.last_timing()["block_size"],
         "arena_in_lds": batch.last_timing()["arena_in_lds"],

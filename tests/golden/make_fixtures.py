@@ -283,6 +283,17 @@ def warm_fixture(tmp):
                                             "tol=1e-6, max_iterations=40000), ForwardModel "
                                             "cache (50, 0.5, 64 MiB)",
                                     "thetas": thetas, "rows": res["rows"]})
+    # the configuration-1 channel: warm starts from the converged base are
+    # classified diverged by the reference (the iteration-0 normalisation of
+    # an already converged state)
+    prob = cases.channel_problem()
+    mp = os.path.join(tmp, "c1w.msh")
+    prob.mesh.save_binary(mp)
+    thetas = [[0.31, 0.09], [0.3131, 0.09], [0.31, 0.0909], [0.341, 0.09]]
+    res = run_case(problem_lines(prob, mp, thetas, extra=["warm_cache 67108864"]))
+    write("ref_channel_c1_warm.json", {"case": "channel_problem(), ForwardModel cache "
+                                               "(50, 0.5, 64 MiB)",
+                                       "thetas": thetas, "rows": res["rows"]})
 
 
 if __name__ == "__main__":

@@ -1,14 +1,15 @@
 """The single-row model with its device-resident warm-start cache (rq_model_*)
 on the GPU: a warm-started sequence against the reference's own ForwardModel
-run with its cache (rans-uq/src/ForwardModel.cpp:79-117, fixture
-ref_channel_warm.json), cold rows equal to the batch path, the byte budget,
-and the one-launch batch mode against single warm-started rows."""
+run with its cache (rans-uq/src/ForwardModel.cpp:79-117, fixtures
+ref_channel_warm.json and ref_channel_c1_warm.json), cold rows equal to the
+batch path, the byte budget, and the one-launch batch mode against single
+warm-started rows."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
-from cfdr_amd import engine
+from cfdr_amd import cases, engine
 from cfdr_amd.model import ForwardModel
 from fixtures import load, max_rel, small_channel
 
@@ -42,6 +43,25 @@ def test_warm_sequence_matches_reference_forward_model():
     # converged rows are cached; the repeated row replaced its entry
     assert m.cache().size() == 2
     assert m.cache().retained_bytes() == 2 * (2 * 8 + (32 + 2) * 24 + 8 * (32 + 2) * 8)
+
+
+def test_config1_warm_rows_are_classified_as_the_reference_does():
+    """On the 1x128 channel (tol 1e-8) the reference classifies warm starts
+    from the converged base as diverged (iterations 5, 332, 5): failed solves
+    match exactly."""
+    d = load("ref_channel_c1_warm.json")
+    m = _model(cases.channel_problem())
+    res, used = m.evaluate_sequence(np.asarray(d["thetas"]))
+    for i, row in enumerate(d["rows"]):
+        assert bool(used[i]) == row["used_warm_start"], i
+        assert res.statuses[i] == row["status"], (i, res.errors[i])
+        if row["status"] == 0:
+            assert abs(int(res.iteration_counts[i]) - row["iters"]) <= max(20, int(0.005 * row["iters"]))
+            assert max_rel(res.predictions[i], row["predictions"]) <= REL_OBS
+        else:
+            assert res.iteration_counts[i] == row["iters"], i
+            assert res.log_likelihoods[i] == row["loglik"]
+    assert m.cache().size() == 1
 
 
 def test_cold_row_equals_batch_row_and_warm_restart_is_short():
