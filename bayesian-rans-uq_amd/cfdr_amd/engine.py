@@ -94,7 +94,10 @@ EXPORTS = ["rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
            "rq_model_cache_set_byte_budget", "rq_model_cache_clear", "rq_model_destroy",
            "rq_warm_cache_create", "rq_warm_cache_store", "rq_warm_cache_nearest",
            "rq_warm_cache_size", "rq_warm_cache_retained_bytes", "rq_warm_cache_set_byte_budget",
-           "rq_warm_cache_clear", "rq_warm_cache_entry_bytes", "rq_warm_cache_destroy"]
+           "rq_warm_cache_clear", "rq_warm_cache_entry_bytes", "rq_warm_cache_destroy",
+           "rq_sensitivity_create", "rq_sensitivity_solve_state", "rq_sensitivity_has_state",
+           "rq_sensitivity_eta_jacobian_warm_fd", "rq_sensitivity_engine",
+           "rq_sensitivity_destroy"]
 
 _lib = None
 

@@ -149,8 +149,15 @@ struct DevSlot {
     const double *warm;     // flow state to start from (warm start) or null
     double *save;           // where to write the final flow state, or null
     double coeffs[11];
+    double tol;             // convergence tolerance of this row (0: the settings')
+    int max_iter;           // iteration cap of this row (0: the settings')
+    int flags;              // kSlot* bits
     int row;                // output row index
 };
+// slot flags: no observation convergence monitor (a solver without
+// setConvergenceMonitor, as ParameterSensitivity's); re-apply the boundary
+// conditions to a warm state before solving (ParameterSensitivity.cpp:790-794)
+constexpr int kSlotNoMonitor = 1, kSlotWarmBCs = 2;
 
 // A flow state (warm start source and export), in doubles: the interior
 // fields Ux, Uy, p, k, omega, nuT, F1, F2, Pk, CDkw (nc each), the boundary

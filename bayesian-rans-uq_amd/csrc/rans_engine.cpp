@@ -950,11 +950,20 @@ int admit_row(const rq_engine *E, const double *row, double *coeffs, std::string
     return 0;
 }
 
+// Per-row solve options of the sensitivity path: tolerance and iteration
+// cap (0: the engine's settings) and kSlot* flags.
+struct RowOpts {
+    double tol = 0.0;
+    int max_iter = 0;
+    int flags = 0;
+};
+
 // warm/save: per input row, the device flow state a row starts from and the
-// device buffer its final state is written to (either may be null)
+// device buffer its final state is written to (either may be null); opts:
+// per input row solve options, or null
 int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
                   int32_t workers, rq_batch_out *out, const double *const *warm = nullptr,
-                  double *const *save = nullptr) {
+                  double *const *save = nullptr, const RowOpts *opts = nullptr) {
     if (!E || !out || !out->status) throw Fail{RQ_EINVAL, "null argument"};
     if (workers < 1) throw Fail{RQ_EINVAL, "workers must be positive"};
     if (n_params != E->n_active)
@@ -994,6 +1003,11 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         sl.row = (int)slots.size();
         sl.warm = warm ? warm[r] : nullptr;
         sl.save = save ? save[r] : nullptr;
+        if (opts) {
+            sl.tol = opts[r].tol;
+            sl.max_iter = opts[r].max_iter;
+            sl.flags = opts[r].flags;
+        }
         valid.push_back(r);
         slots.push_back(sl);
     }
@@ -1232,6 +1246,211 @@ int model_evaluate_impl(rq_model *M, const double *theta, int64_t n_rows, int32_
     }
     for (int64_t r = 0; r < n_rows; ++r) after(r, save[r]);
     return RQ_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Total sensitivities by warm-started central differences:
+// ParameterSensitivity::solveState and ::etaJacobianWarmFD
+// (rans-uq/src/ParameterSensitivity.cpp:35-59,729-840). The 22 perturbed
+// solves of one Jacobian start from the converged base state, which stays in
+// device memory, and run in one launch.
+// ----------------------------------------------------------------------------
+struct rq_sensitivity {
+    rq_engine *E = nullptr;     // all eleven coefficients active
+    double *state = nullptr;    // converged base state (device)
+    double *scratch = nullptr;  // the next base solve's state (device)
+    std::vector<double> theta;  // coefficients of the base state
+    bool has_state = false;
+    ~rq_sensitivity() {
+        if (state) (void)hipFree(state);
+        if (scratch) (void)hipFree(scratch);
+        delete E;
+    }
+};
+
+namespace {
+
+// InferenceParameterSet::all11 bounds (core/include/InferenceParameters.hpp:103-122)
+const double kAll11Lower[11] = {0.1, 0.1, 0.01, 0.1, 0.1, 0.1, 0.01, 0.1, 0.01, 0.1, 0.35};
+const double kAll11Upper[11] = {2.0, 2.0, 0.2, 1.5, 2.0, 2.0, 0.2, 1.5, 0.2, 0.8, 0.50};
+
+rq_batch_out one_row_out(int32_t *st, double *ll, double *pred, int32_t *it, double *worst,
+                         double *rate) {
+    rq_batch_out o{};
+    o.status = st;
+    o.loglik = ll;
+    o.predictions = pred;
+    o.iterations = it;
+    o.final_worst_measure = worst;
+    o.contraction_rate_u = rate;
+    return o;
+}
+
+int solve_state_impl(rq_sensitivity *S, const double *theta11, int32_t *status) {
+    if (!S || !theta11 || !status) throw Fail{RQ_EINVAL, "null argument"};
+    rq_engine *E = S->E;
+    S->has_state = false;
+    S->theta.clear();
+    for (int j = 0; j < 11; ++j)
+        if (!(theta11[j] >= kAll11Lower[j] && theta11[j] <= kAll11Upper[j])) {
+            *status = ST_INVALID;
+            return RQ_OK;
+        }
+    HIPCHECK(hipSetDevice(E->device));
+    const int64_t sn = state_doubles(E->nc, E->nb);
+    if (!S->scratch) HIPCHECK(hipMalloc(&S->scratch, (size_t)sn * sizeof(double)));
+    const int nobs = std::max(E->nobs, 1);
+    std::vector<double> pred(nobs);
+    int32_t st = 0, it = 0;
+    double ll = 0, worst = 0, rate = 0;
+    rq_batch_out o = one_row_out(&st, &ll, pred.data(), &it, &worst, &rate);
+    RowOpts opt;
+    opt.flags = kSlotNoMonitor;
+    double *save = S->scratch;
+    evaluate_impl(E, theta11, 1, 11, 1, &o, nullptr, &save, &opt);
+    // a solve whose observation fails becomes Unknown here, where the
+    // reference's solveState returns Converged (it does not observe)
+    *status = st;
+    if (st == ST_CONVERGED) {
+        std::swap(S->state, S->scratch);
+        S->theta.assign(theta11, theta11 + 11);
+        S->has_state = true;
+    }
+    return RQ_OK;
+}
+
+int warm_fd_impl(rq_sensitivity *S, const double *theta11, double hRel, double hFloor,
+                 int32_t warmMaxIter, double warmTol, const double *absSteps,
+                 rq_warm_fd_out *out) {
+    if (!S || !theta11 || !out) throw Fail{RQ_EINVAL, "null argument"};
+    rq_engine *E = S->E;
+    // requireMatchingState (ParameterSensitivity.cpp:61-72)
+    if (!S->has_state) throw Fail{RQ_EINVAL, "Sensitivity requires a converged base solveState"};
+    if (!std::equal(S->theta.begin(), S->theta.end(), theta11))
+        throw Fail{RQ_EINVAL, "Total sensitivity requires the coefficients of the converged base state"};
+    if (!std::isfinite(hRel) || hRel <= 0 || !std::isfinite(hFloor) || hFloor <= 0 ||
+        warmMaxIter < 0 || !std::isfinite(warmTol) || warmTol < 0)
+        throw Fail{RQ_EINVAL, "Invalid warm finite-difference step/tolerance/iteration cap"};
+    if (absSteps)
+        for (int j = 0; j < 11; ++j)
+            if (!std::isfinite(absSteps[j]) || absSteps[j] <= 0)
+                throw Fail{RQ_EINVAL, "finite-difference steps require eleven positive finite entries"};
+    const int nobs = E->nobs;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const double inf = std::numeric_limits<double>::infinity();
+    for (int i = 0; i < nobs * 11; ++i)
+        if (out->d_obs_d_theta) out->d_obs_d_theta[i] = nan;
+    for (int j = 0; j < 11; ++j) {
+        if (out->log_lik_gradient) out->log_lik_gradient[j] = nan;
+        if (out->iterations) out->iterations[j] = 0;
+        if (out->worst_measure) out->worst_measure[j] = 0.0;
+        if (out->converged) out->converged[j] = 0;
+        if (out->positive_status) out->positive_status[j] = ST_UNKNOWN;
+        if (out->negative_status) out->negative_status[j] = ST_UNKNOWN;
+    }
+    if (out->n_residual_evals) *out->n_residual_evals = 0;
+    // steps and tolerances (ParameterSensitivity.cpp:769-778,813-822); the
+    // rows are + then - for each coefficient
+    const double baseTol = E->settings.convergence_tol;
+    std::vector<double> rows(22 * 11), h(11), tol(11);
+    for (int j = 0; j < 11; ++j) {
+        double hj;
+        if (absSteps) {
+            hj = absSteps[j];
+        } else {
+            const double hr = (j == 8 || j == 9) ? std::min(hRel, 1e-5) : hRel;
+            hj = std::max(hr * std::abs(theta11[j]), hFloor);
+        }
+        double *tp = &rows[(2 * j) * 11], *tm = &rows[(2 * j + 1) * 11];
+        std::copy(theta11, theta11 + 11, tp);
+        std::copy(theta11, theta11 + 11, tm);
+        tp[j] += hj;
+        tm[j] -= hj;
+        hj = 0.5 * (tp[j] - tm[j]);
+        if (!(hj > 0) || !std::isfinite(hj))
+            throw Fail{RQ_EINVAL, "finite-difference step is not representable"};
+        h[j] = hj;
+        const double rel = hj / std::max(std::abs(theta11[j]), hFloor);
+        tol[j] = warmTol > 0 ? warmTol : std::min(baseTol, 0.01 * rel);
+        if (out->steps) out->steps[j] = hj;
+        if (out->tolerances) out->tolerances[j] = tol[j];
+    }
+    // rows with a nonpositive or nonfinite coefficient are not solved
+    // (ParameterSensitivity.cpp:781-788)
+    std::vector<int64_t> solved;
+    std::vector<double> thetaS;
+    std::vector<RowOpts> opts;
+    std::vector<const double *> warm;
+    std::vector<int32_t> st(22, ST_INVALID), it(22, 0);
+    std::vector<double> worst(22, inf), ll(22, nan), pred(22 * (size_t)std::max(nobs, 1), nan);
+    for (int r = 0; r < 22; ++r) {
+        const double *t = &rows[r * 11];
+        bool ok = true;
+        for (int j = 0; j < 11; ++j) ok = ok && std::isfinite(t[j]) && t[j] > 0;
+        if (!ok) continue;
+        solved.push_back(r);
+        thetaS.insert(thetaS.end(), t, t + 11);
+        RowOpts o;
+        o.tol = tol[r / 2];
+        o.max_iter = warmMaxIter;
+        o.flags = kSlotNoMonitor | kSlotWarmBCs;
+        opts.push_back(o);
+        warm.push_back(S->state);
+    }
+    const int64_t ns = (int64_t)solved.size();
+    if (ns > 0) {
+        std::vector<int32_t> st2(ns), it2(ns);
+        std::vector<double> ll2(ns), w2(ns), rate2(ns), p2(ns * (size_t)std::max(nobs, 1));
+        rq_batch_out o = one_row_out(st2.data(), ll2.data(), p2.data(), it2.data(), w2.data(),
+                                     rate2.data());
+        evaluate_impl(E, thetaS.data(), ns, 11, 1, &o, warm.data(), nullptr, opts.data());
+        int64_t evals = 0;
+        for (int64_t k = 0; k < ns; ++k) {
+            const int64_t r = solved[k];
+            st[r] = st2[k];
+            it[r] = it2[k];
+            evals += it2[k];
+            worst[r] = (st2[k] != ST_UNKNOWN && std::isfinite(w2[k])) ? w2[k] : inf;
+            ll[r] = ll2[k];
+            for (int i = 0; i < nobs; ++i) pred[r * nobs + i] = p2[k * nobs + i];
+        }
+        if (out->n_residual_evals) *out->n_residual_evals = evals;
+    }
+    for (int j = 0; j < 11; ++j) {
+        const int p = 2 * j, m = 2 * j + 1;
+        if (out->positive_status) out->positive_status[j] = st[p];
+        if (out->negative_status) out->negative_status[j] = st[m];
+        if (out->iterations) out->iterations[j] = it[p] + it[m];
+        if (out->worst_measure) out->worst_measure[j] = std::max(worst[p], worst[m]);
+        const bool ok = st[p] == ST_CONVERGED && st[m] == ST_CONVERGED;
+        if (out->converged) out->converged[j] = ok;
+        if (!ok) continue;
+        for (int i = 0; i < nobs; ++i)
+            if (out->d_obs_d_theta)
+                out->d_obs_d_theta[i * 11 + j] = (pred[p * nobs + i] - pred[m * nobs + i]) / (2.0 * h[j]);
+        if (out->log_lik_gradient) out->log_lik_gradient[j] = (ll[p] - ll[m]) / (2.0 * h[j]);
+    }
+    return RQ_OK;
+}
+
+template <class F>
+int guarded(char *err, size_t errcap, F &&f) {
+    try {
+        int rc = f();
+        set_err(err, errcap, "");
+        return rc;
+    } catch (const Fail &e) {
+        set_err(err, errcap, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_err(err, errcap, "host allocation failed");
+        return RQ_ENOMEM;
+    } catch (const std::exception &e) {
+        set_err(err, errcap, e.what());
+        return RQ_EDEVICE;
+    }
 }
 
 }  // namespace
@@ -1747,4 +1966,53 @@ int64_t rq_warm_cache_entry_bytes(int32_t n_theta, int32_t n_cells, int32_t n_bo
     return (int64_t)rq::WarmCache::entry_bytes((size_t)n_theta, n_cells, n_boundary);
 }
 void rq_warm_cache_destroy(rq_warm_cache *w) { delete w; }
+
+// ---------------------------------------------------------------- sensitivity
+rq_sensitivity *rq_sensitivity_create(const rq_mesh_desc *mesh, const rq_obs_desc *obs,
+                                      const rq_bc_desc *bcs, double nu,
+                                      const rq_settings *settings, const double init[5],
+                                      char *err, size_t errcap) {
+    rq_sensitivity *S = nullptr;
+    const int rc = guarded(err, errcap, [&]() {
+        int32_t idx[11];
+        double base[11], lo[11], hi[11];
+        for (int j = 0; j < 11; ++j) {
+            idx[j] = j;
+            base[j] = kAll11Lower[j];
+            lo[j] = 0.0;  // admission of perturbed rows is decided before the launch
+            hi[j] = std::numeric_limits<double>::infinity();
+        }
+        rq_param_desc P{11, idx, base, lo, hi};
+        auto *s = new rq_sensitivity();
+        try {
+            s->E = create_impl(mesh, &P, obs, bcs, nu, settings, init);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        S = s;
+        return RQ_OK;
+    });
+    return rc == RQ_OK ? S : nullptr;
+}
+
+int rq_sensitivity_solve_state(rq_sensitivity *s, const double theta11[11], int32_t *status,
+                               char *err, size_t errcap) {
+    return guarded(err, errcap, [&]() { return solve_state_impl(s, theta11, status); });
+}
+
+int32_t rq_sensitivity_has_state(const rq_sensitivity *s) { return s && s->has_state ? 1 : 0; }
+
+int rq_sensitivity_eta_jacobian_warm_fd(rq_sensitivity *s, const double theta11[11],
+                                        double h_rel, double h_floor, int32_t warm_max_iter,
+                                        double warm_tol, const double *absolute_steps,
+                                        rq_warm_fd_out *out, char *err, size_t errcap) {
+    return guarded(err, errcap, [&]() {
+        return warm_fd_impl(s, theta11, h_rel, h_floor, warm_max_iter, warm_tol, absolute_steps,
+                            out);
+    });
+}
+
+const rq_engine *rq_sensitivity_engine(const rq_sensitivity *s) { return s ? s->E : nullptr; }
+void rq_sensitivity_destroy(rq_sensitivity *s) { delete s; }
 }  // extern "C"

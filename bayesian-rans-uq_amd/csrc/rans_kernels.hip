@@ -1982,6 +1982,7 @@ struct Draw {
     // solve begins from the stored fields, boundary values and turbulence
     // flag as they are)
     const double *warm = nullptr;
+    bool warm_bcs = false;
     DEV void load_state() {
         const int64_t nbo = (int64_t)kStateInterior * nc;
         const int vars[kStateInterior] = {V_Ux, V_Uy, V_p, V_k, V_w, V_nuT, V_F1, V_F2, V_Pk, V_CDkw};
@@ -2044,6 +2045,12 @@ struct Draw {
             bc_pressure();
             bc_k();
             bc_omega(0.075);
+            sync();
+        } else if (warm_bcs) {  // with the row's own beta1
+            bc_velocity();
+            bc_pressure();
+            bc_k();
+            bc_omega(c[2]);
             sync();
         }
         for (int f = gt; f < nf; f += gs) Fc(f, W_phi) = interp_flux(f);
@@ -2561,9 +2568,13 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
                     const double *coeffSrc, double *arena, double *amg, DevResult *res,
                     double *pred, double *fields, GridState *grid = nullptr, int rank = 0,
                     int nblk = 1, double *aux = nullptr, const double *warm = nullptr,
-                    double *save = nullptr) {
+                    double *save = nullptr, double tol = 0.0, int max_iter = 0, int flags = 0) {
     if (threadIdx.x < 11) sh.coeffs[threadIdx.x] = coeffSrc[threadIdx.x];
-    if (threadIdx.x == 0) sh.sset = settings;
+    if (threadIdx.x == 0) {
+        sh.sset = settings;
+        if (tol > 0.0) sh.sset.convergence_tol = tol;
+        if (max_iter > 0) sh.sset.max_iterations = max_iter;
+    }
     const int g0 = rank * BS + threadIdx.x, gstride = nblk * BS;
     {
         const int64_t n16 = M.lay.image_bytes / 16;
@@ -2597,6 +2608,8 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
                             aux);
     d.cbuf = sh.coarse_b;
     d.warm = warm;
+    d.warm_bcs = (flags & kSlotWarmBCs) != 0;
+    if (flags & kSlotNoMonitor) d.nobs = 0;  // the final observation still runs
     if constexpr (kGrid) {
         d.rank = rank;
         d.nblk = nblk;
@@ -2682,7 +2695,7 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
     solve_draw<BS, kLDS>(sh, *slot.mesh, settings, slot.coeffs, kLDS ? smem : slot.arena,
                          slot.amg, results + row, predictions + (int64_t)row * pred_stride,
                          fields ? fields + row * field_stride : nullptr, nullptr, 0, 1, nullptr,
-                         slot.warm, slot.save);
+                         slot.warm, slot.save, slot.tol, slot.max_iter, slot.flags);
 }
 
 // The multi-workgroup batch kernel: nblk co-resident workgroups per draw
@@ -2709,7 +2722,8 @@ __global__ void __launch_bounds__(BS) grid_solve_kernel(const DevSlot *__restric
     solve_draw<BS, false, true>(sh, *slot.mesh, settings, slot.coeffs, slot.arena, slot.amg,
                                 results + row, predictions + (int64_t)row * pred_stride,
                                 fields ? fields + row * field_stride : nullptr, states + draw,
-                                rank, nblk, smem, slot.warm, slot.save);
+                                rank, nblk, smem, slot.warm, slot.save, slot.tol,
+                                slot.max_iter, slot.flags);
 }
 
 // ----------------------------------------------------------------------------

@@ -275,6 +275,59 @@ int rq_warm_cache_clear(rq_warm_cache *cache);
 int64_t rq_warm_cache_entry_bytes(int32_t n_theta, int32_t n_cells, int32_t n_boundary);
 void rq_warm_cache_destroy(rq_warm_cache *cache);
 
+/* ---------------------------------------------------------------------------
+ * Total sensitivities by warm-started central differences: ParameterSensitivity
+ * (rans-uq/include/ParameterSensitivity.hpp:68-133) with solveState and
+ * etaJacobianWarmFD (rans-uq/src/ParameterSensitivity.cpp:35-59,729-840).
+ * Coefficient vectors are the eleven SST coefficients. The converged base
+ * state stays in device memory; the 22 perturbed solves of a Jacobian start
+ * from it (boundary conditions re-applied with each row's beta1, no
+ * observation convergence monitor, each coefficient's own tolerance) and run
+ * in one launch.
+ * ------------------------------------------------------------------------- */
+typedef struct rq_sensitivity rq_sensitivity;
+
+/* TangentGradientResult of the warm-FD path (ParameterSensitivity.hpp:24-36).
+ * Arrays are caller-owned; any may be NULL. Failed columns hold NaN. */
+typedef struct rq_warm_fd_out {
+    double *d_obs_d_theta;      /* n_obs x 11 (row-major): dObsDTheta */
+    double *log_lik_gradient;   /* 11 */
+    int32_t *iterations;        /* 11: SIMPLE iterations of the + and - solves together */
+    double *worst_measure;      /* 11: the larger final convergence measure (inf if none) */
+    int32_t *converged;         /* 11: both solves converged */
+    double *steps;              /* 11: central half-widths */
+    double *tolerances;         /* 11: perturbed-solve tolerances */
+    int32_t *positive_status;   /* 11 */
+    int32_t *negative_status;   /* 11 */
+    int64_t *n_residual_evals;  /* 1: SIMPLE iterations of every solved row */
+} rq_warm_fd_out;
+
+/* ParameterSensitivity constructor (no parameter set: all eleven coefficients). */
+rq_sensitivity *rq_sensitivity_create(const rq_mesh_desc *mesh, const rq_obs_desc *obs,
+                                      const rq_bc_desc *bcs, double nu,
+                                      const rq_settings *settings, const double init[5],
+                                      char *err, size_t errcap);
+
+/* solveState: a cold solve without the observation monitor; *status gets the
+ * EvaluationStatus code, and a converged state becomes the base. */
+int rq_sensitivity_solve_state(rq_sensitivity *sens, const double theta11[11], int32_t *status,
+                               char *err, size_t errcap);
+
+int32_t rq_sensitivity_has_state(const rq_sensitivity *sens);
+
+/* etaJacobianWarmFD(theta11, hRel, hFloor, warmMaxIter, warmTol, absoluteSteps);
+ * theta11 must be the base state's coefficients. absolute_steps: NULL or 11
+ * positive half-widths. RQ_EINVAL for the reference's std::invalid_argument
+ * and logic_error cases. */
+int rq_sensitivity_eta_jacobian_warm_fd(rq_sensitivity *sens, const double theta11[11],
+                                        double h_rel, double h_floor, int32_t warm_max_iter,
+                                        double warm_tol, const double *absolute_steps,
+                                        rq_warm_fd_out *out, char *err, size_t errcap);
+
+/* The sensitivity's engine, for rq_engine_n_observations and timing. */
+const rq_engine *rq_sensitivity_engine(const rq_sensitivity *sens);
+void rq_sensitivity_destroy(rq_sensitivity *sens);
+
 #ifdef __cplusplus
 }
 #endif

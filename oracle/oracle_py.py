@@ -6,6 +6,7 @@ bench.py's cpu_baseline leg, never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -84,6 +85,11 @@ def lib():
         _lib.or_evaluate.argtypes = [C.POINTER(OrMesh), C.POINTER(OrBc), C.POINTER(OrSettings),
                                      C.c_double, _dp, _dp, C.POINTER(OrObs), C.POINTER(OrResult)]
         _lib.or_evaluate.restype = C.c_int
+        _lib.or_evaluate_ex.argtypes = [C.POINTER(OrMesh), C.POINTER(OrBc),
+                                        C.POINTER(OrSettings), C.c_double, _dp, _dp,
+                                        C.POINTER(OrObs), _dp, _dp, C.c_int, C.c_double,
+                                        C.c_int, C.POINTER(OrResult)]
+        _lib.or_evaluate_ex.restype = C.c_int
         for name, nargs in (("or_sst_f1", 6), ("or_sst_f2", 5), ("or_sst_pk", 5),
                             ("or_sst_pomega", 5)):
             f = getattr(_lib, name)
@@ -193,7 +199,16 @@ class OracleProblem:
         self.nu = float(nu)
         self.init = _d(init)
 
-    def evaluate(self, coeffs, fields: bool = False):
+    def state_size(self) -> int:
+        """Doubles of a flow state (warm start and save layout)."""
+        nb = self.mesh.n_faces - self.mesh.n_internal
+        return 10 * self.mesh.n_cells + 5 * max(nb, 1) + 2
+
+    def evaluate(self, coeffs, fields: bool = False, warm=None, save: bool = False,
+                 flags: int = 0, tol: float = 0.0, max_iter: int = 0):
+        """One solve; warm: a state to start from; save: return the final
+        state as "state"; flags: OR_FLAG_* bits; tol / max_iter > 0 override
+        the settings."""
         L = lib()
         m, b, s, o = self.structs
         r = OrResult()
@@ -207,12 +222,90 @@ class OracleProblem:
             out["flux"] = np.zeros(self.mesh.n_faces)
             r.flux = out["flux"].ctypes.data_as(_dp)
         c = _d(coeffs)
-        rc = L.or_evaluate(C.byref(m), C.byref(b), C.byref(s), self.nu,
-                           c.ctypes.data_as(_dp), self.init.ctypes.data_as(_dp), C.byref(o),
-                           C.byref(r))
+        w = None if warm is None else _d(warm)
+        st = np.zeros(self.state_size()) if save else None
+        rc = L.or_evaluate_ex(C.byref(m), C.byref(b), C.byref(s), self.nu,
+                              c.ctypes.data_as(_dp), self.init.ctypes.data_as(_dp), C.byref(o),
+                              None if w is None else w.ctypes.data_as(_dp),
+                              None if st is None else st.ctypes.data_as(_dp), int(flags),
+                              float(tol), int(max_iter), C.byref(r))
         res = {"rc": rc, "status": r.status, "iters": r.iters, "stalled": bool(r.stalled),
                "converged": bool(r.converged), "diverged": bool(r.diverged),
                "worst": r.worst, "rate": r.rate, "loglik": r.loglik, "predictions": pred,
                "krylov_iters": list(r.krylov_iters), "error": r.error.decode()}
         res.update(out)
+        if save:
+            res["state"] = st
         return res
+
+
+OR_FLAG_NO_MONITOR = 1
+OR_FLAG_WARM_BCS = 2
+ALL11_LOWER = (0.1, 0.1, 0.01, 0.1, 0.1, 0.1, 0.01, 0.1, 0.01, 0.1, 0.35)
+ALL11_UPPER = (2.0, 2.0, 0.2, 1.5, 2.0, 2.0, 0.2, 1.5, 0.2, 0.8, 0.50)
+
+
+def solve_state(problem: OracleProblem, theta11):
+    """ParameterSensitivity::solveState (rans-uq/src/ParameterSensitivity.cpp:35-59):
+    all11 bounds, a cold solve without the observation monitor. Returns
+    (status, state or None)."""
+    t = np.asarray(theta11, float)
+    if not all(lo <= v <= hi for v, lo, hi in zip(t, ALL11_LOWER, ALL11_UPPER)):
+        return 3, None
+    r = problem.evaluate(t, save=True, flags=OR_FLAG_NO_MONITOR)
+    return r["status"], (r["state"] if r["status"] == 0 else None)
+
+
+def eta_jacobian_warm_fd(problem: OracleProblem, state, theta11, h_rel=5e-4, h_floor=1e-7,
+                         warm_max_iter=0, warm_tol=0.0, absolute_steps=None):
+    """ParameterSensitivity::etaJacobianWarmFD (rans-uq/src/ParameterSensitivity.cpp:729-840)
+    on the oracle: central differences of re-converged solves started from the
+    base state with the boundary conditions re-applied, no observation monitor
+    and the per-coefficient tolerance min(tol, 0.01 h / |theta|)."""
+    t0 = np.asarray(theta11, float)
+    s = problem.structs[2]
+    n_obs = problem.prepared.n_obs
+    out = {"d_obs_d_theta": np.full((n_obs, 11), np.nan), "log_lik_gradient": np.full(11, np.nan),
+           "iterations": np.zeros(11, int), "worst_measure": np.zeros(11),
+           "converged": np.zeros(11, int), "steps": np.zeros(11), "tolerances": np.zeros(11),
+           "positive_status": np.full(11, 4), "negative_status": np.full(11, 4),
+           "n_residual_evals": 0}
+
+    def step(j):
+        if absolute_steps is not None:
+            return float(absolute_steps[j])
+        hr = min(h_rel, 1e-5) if j in (8, 9) else h_rel
+        return max(hr * abs(t0[j]), h_floor)
+
+    def warm_eta(tp, tol):
+        if not all(np.isfinite(v) and v > 0 for v in tp):
+            return 3, 0, math.inf, math.nan, None
+        r = problem.evaluate(tp, warm=state, flags=OR_FLAG_NO_MONITOR | OR_FLAG_WARM_BCS,
+                             tol=tol, max_iter=warm_max_iter)
+        out["n_residual_evals"] += r["iters"]
+        worst = r["worst"] if np.isfinite(r["worst"]) else math.inf
+        if r["status"] != 0:
+            return r["status"], r["iters"], worst, math.nan, None
+        return 0, r["iters"], worst, r["loglik"], np.array(r["predictions"])
+
+    for j in range(11):
+        h = step(j)
+        tp, tm = t0.copy(), t0.copy()
+        tp[j] += h
+        tm[j] -= h
+        h = 0.5 * (tp[j] - tm[j])
+        out["steps"][j] = h
+        rel = h / max(abs(t0[j]), h_floor)
+        tol = warm_tol if warm_tol > 0 else min(s.convergence_tol, 0.01 * rel)
+        out["tolerances"][j] = tol
+        sp, ip, wp, lp, ep = warm_eta(tp, tol)
+        sm, im, wm, lm, em = warm_eta(tm, tol)
+        out["positive_status"][j], out["negative_status"][j] = sp, sm
+        out["iterations"][j] = ip + im
+        out["worst_measure"][j] = max(wp, wm)
+        ok = sp == 0 and sm == 0
+        out["converged"][j] = int(ok)
+        if ok:
+            out["d_obs_d_theta"][:, j] = (ep - em) / (2.0 * h)
+            out["log_lik_gradient"][j] = (lp - lm) / (2.0 * h)
+    return out

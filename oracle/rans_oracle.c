@@ -1498,6 +1498,17 @@ static void set_error(or_result_t *r, const char *msg) {
 int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *st, double nu,
                 const double coeffs[11], const double init[5], const or_obs_t *obs,
                 or_result_t *R) {
+    return or_evaluate_ex(mesh, bc, st, nu, coeffs, init, obs, NULL, NULL, 0, 0.0, 0, R);
+}
+
+int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *st_in,
+                   double nu, const double coeffs[11], const double init[5], const or_obs_t *obs,
+                   const double *warm, double *save, int flags, double tol_override,
+                   int max_iter_override, or_result_t *R) {
+    or_settings_t st_local = *st_in;
+    if (tol_override > 0.0) st_local.convergence_tol = tol_override;
+    if (max_iter_override > 0) st_local.max_iterations = max_iter_override;
+    const or_settings_t *st = &st_local;
     ctx_t Xs;
     ctx_t *X = &Xs;
     memset(X, 0, sizeof *X);
@@ -1551,6 +1562,24 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
 
     prepare_planar(X);
 
+    const int nbs = nb > 1 ? nb : 1;
+    if (warm) {
+        /* warm start: the state as stored (ForwardModel.cpp:85-88); the
+         * sensitivity path re-applies the boundary conditions with this
+         * solve's beta1 (ParameterSensitivity.cpp:790-794) */
+        double *dst[10] = {X->Ux, X->Uy, X->p, X->k, X->w, X->nuT, X->F1, X->F2, X->Pk, X->CDkw};
+        double *bdst[5] = {X->Uxb, X->Uyb, X->pb, X->kb, X->wb};
+        for (int a = 0; a < 10; ++a) memcpy(dst[a], warm + (size_t)a * nc, sizeof(double) * nc);
+        for (int a = 0; a < 5; ++a)
+            memcpy(bdst[a], warm + (size_t)10 * nc + (size_t)a * nbs, sizeof(double) * nb);
+        X->turbEstablished = warm[(size_t)10 * nc + (size_t)5 * nbs] != 0.0;
+        if (flags & OR_FLAG_WARM_BCS) {
+            apply_velocity_bc(X);
+            apply_pressure_bc(X);
+            apply_k_bc(X);
+            apply_omega_bc(X, X->c[C_B1]);
+        }
+    } else {
     /* initUniform (SIMPLESolver.cpp:267-298) */
     for (int i = 0; i < nc; ++i) {
         X->Ux[i] = init[0];
@@ -1582,6 +1611,7 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
     apply_pressure_bc(X);
     apply_k_bc(X);
     apply_omega_bc(X, 0.075);
+    }
 
     /* solve() */
     double lastKChange = 1.0, lastOmChange = 1.0;
@@ -1593,6 +1623,9 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
     ce_init(&estOm, st->convergence_window);
     ce_init(&estQ, st->convergence_window);
     const int nObs = obs->n_obs;
+    /* the observation convergence monitor (ForwardModel.cpp:60-72); a
+     * ParameterSensitivity solver has none */
+    const int nMon = (flags & OR_FLAG_NO_MONITOR) ? 0 : nObs;
     double *q = ALLOC(nObs), *qPrev = ALLOC(nObs);
     int haveQPrev = 0;
     double worst = INFINITY;
@@ -1768,7 +1801,7 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
         const double changeU = duMax / uMax;
         ce_push(&estU, changeU);
         const double remainingU = ce_remaining(&estU, negligible);
-        if (nObs > 0 && iter % (st->monitor_interval > 1 ? st->monitor_interval : 1) == 0) {
+        if (nMon > 0 && iter % (st->monitor_interval > 1 ? st->monitor_interval : 1) == 0) {
             observe(X, q);
             int finite = 1;
             for (int i = 0; i < nObs; ++i) finite = finite && isfinite(q[i]);
@@ -1797,7 +1830,7 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
             worst = smax(worst, lastOmChange);
             worst = smax(worst, ce_remaining(&estOm, negligible));
         }
-        if (nObs > 0) {
+        if (nMon > 0) {
             worst = smax(worst, ce_last(&estQ));
             worst = smax(worst, ce_remaining(&estQ, negligible));
         }
@@ -1889,6 +1922,16 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
     for (int i = 0; i < 10; ++i)
         if (outs[i]) memcpy(outs[i], srcs[i], sizeof(double) * nc);
     if (R->flux) memcpy(R->flux, X->phi, sizeof(double) * nf);
+    if (save) {
+        for (int a = 0; a < 10; ++a) memcpy(save + (size_t)a * nc, srcs[a], sizeof(double) * nc);
+        double *bsrc[5] = {X->Uxb, X->Uyb, X->pb, X->kb, X->wb};
+        for (int a = 0; a < 5; ++a) {
+            memcpy(save + (size_t)10 * nc + (size_t)a * nbs, bsrc[a], sizeof(double) * nb);
+            for (int b = nb; b < nbs; ++b) save[(size_t)10 * nc + (size_t)a * nbs + b] = 0.0;
+        }
+        save[(size_t)10 * nc + (size_t)5 * nbs] = X->turbEstablished ? 1.0 : 0.0;
+        save[(size_t)10 * nc + (size_t)5 * nbs + 1] = 0.0;
+    }
     free(q);
     free(qPrev);
     free(estU.ring);

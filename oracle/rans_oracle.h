@@ -100,6 +100,20 @@ int or_evaluate(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *s
                 double nu, const double coeffs[11], const double init[5], const or_obs_t *obs,
                 or_result_t *result);
 
+/* or_evaluate with a warm start and options. warm (NULL: cold start) and
+ * save (NULL: not written) hold a flow state: the interior fields Ux, Uy, p,
+ * k, omega, nuT, F1, F2, Pk, CDkw (n_cells each), the boundary values Ux, Uy,
+ * p, k, omega (max(n_boundary, 1) each), the turbulence-established flag and
+ * one pad. flags: OR_FLAG_NO_MONITOR drops the observation convergence
+ * monitor, OR_FLAG_WARM_BCS re-applies the boundary conditions to the warm
+ * state with this solve's beta1 (ParameterSensitivity.cpp:790-794).
+ * tol_override / max_iter_override > 0 replace the settings' values. */
+enum { OR_FLAG_NO_MONITOR = 1, OR_FLAG_WARM_BCS = 2 };
+int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t *settings,
+                   double nu, const double coeffs[11], const double init[5], const or_obs_t *obs,
+                   const double *warm, double *save, int flags, double tol_override,
+                   int max_iter_override, or_result_t *result);
+
 /* Closed-form pointwise SST functions, exported for known-answer tests
  * (core/src/SSTModel.cpp:15-85,102-108). c[11] is the coefficient vector. */
 double or_sst_f1(const double c[11], double k, double omega, double y, double nu, double cdkw);

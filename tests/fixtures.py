@@ -88,3 +88,24 @@ def max_scaled(a, b):
     b = np.asarray(b, float)
     scale = max(float(np.max(np.abs(b))), 1e-300)
     return float(np.max(np.abs(a - b))) / scale
+
+
+def sens_channel():
+    """The periodic problem of the reference's warm-FD refinement test
+    (rans-uq/tests/test_warm_fd_refinement.cpp:29-47): makePeriodicChannel2D(3, 4,
+    1, 0.5, 20), channelDefaults(1, 1e-4, 1), nu 0.1, body force 0.2, upwind
+    convection, tol 1e-9, no nuT floor window, one velocity probe."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    mesh = Mesh.make_periodic_channel_2d(3, 4, 1.0, 0.5, 20.0)
+    mesh.compute_wall_distance()
+    bcs = FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 1.0)
+    s = SolverSettings(max_iterations=15000, convergence_tol=1e-9, nut_floor_iters=0,
+                       momentum_convection="upwind", turbulence_convection="upwind",
+                       body_force=Vec3(0.2, 0.0, 0.0))
+    obs = ObservationOperator()
+    obs.add_velocity_profile((0.43, 0.38, 0.0), 0, 0.0, 1.0)
+    return cases.Problem(mesh=mesh, bcs=bcs, settings=s, obs=obs,
+                         params=InferenceParameterSet.a1_betaStar(), nu=0.1,
+                         u_init=Vec3(1.0, 0.0, 0.0), p_init=0.0, k_init=1e-4, omega_init=1.0)
