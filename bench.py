@@ -24,7 +24,9 @@ and FP64 defect correction), served (Poisson arrivals of single requests on
 (the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
 --hills-full adds the 120x64 batch to convergence), warm_start (a
 pattern-search poll of 4 neighbours of a converged point, cold in one batch
-launch against warm-started from the device-resident state in one launch).
+launch against warm-started from the device-resident state in one launch),
+warm_fd_jacobian (the 22 perturbed solves of the eleven-coefficient warm-FD
+Jacobian in one launch, against the oracle running them in sequence).
 
 Profiling helpers: --max-iterations caps every solve (a short, fixed amount of
 work for counter runs); --copy-calibration MIB runs only a device-to-device
@@ -368,6 +370,46 @@ def warm_leg(steps):
     return out
 
 
+def warm_fd_leg(cpu):
+    """The warm-FD Jacobian (eleven SST coefficients, 22 perturbed solves
+    started from the converged base) on the 1x128 channel at tol 1e-8: the
+    GPU runs the 22 solves in one launch; the CPU oracle runs them one after
+    another, as the reference does."""
+    from cfdr_amd import cases
+    from cfdr_amd.config import COEFF_DEFAULTS
+    from cfdr_amd.sensitivity import ParameterSensitivity
+    prob = cases.channel_problem()
+    theta = np.asarray(COEFF_DEFAULTS)
+    s = ParameterSensitivity(prob.mesh, prob.obs, prob.bcs, prob.nu, prob.settings, prob.u_init,
+                             prob.p_init, prob.k_init, prob.omega_init)
+    if s.solve_state(theta) != 0:
+        raise SystemExit("bench.py: the warm-FD base state did not converge")
+    t0 = time.perf_counter()
+    g = s.eta_jacobian_warm_fd(theta)
+    gpu_s = time.perf_counter() - t0
+    out = {"solves": 22, "columns_converged": int(np.sum(g.krylov_converged)),
+           "simple_iterations": int(g.n_residual_evals), "gpu_ms": round(1e3 * gpu_s, 1),
+           "kernel_ms": round(s.last_kernel_ms(), 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
+                                    prob.init_vector())
+        st, state = oracle_py.solve_state(O, theta)
+        t0 = time.perf_counter()
+        r = oracle_py.eta_jacobian_warm_fd(O, state, theta)
+        seq = time.perf_counter() - t0
+        scale = np.maximum(np.max(np.abs(r["d_obs_d_theta"]), axis=0), 1e-3)
+        out["cpu_baseline"] = {"value_ms": round(1e3 * seq, 1), "cores": 1, "kind": "port",
+                               "sample": "the 22 warm solves one after another (the reference's "
+                                         "order)"}
+        out["max_scaled_diff_vs_oracle"] = float(np.max(np.abs(g.d_obs_d_theta -
+                                                               r["d_obs_d_theta"]) / scale))
+        out["speedup_vs_cpu_sequential"] = round(seq / gpu_s, 2)
+    print(f"# warm-FD Jacobian: {out['gpu_ms']} ms on the GPU", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,6 +428,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-hills", action="store_true")
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-warm-fd", action="store_true")
     ap.add_argument("--hills-full", action="store_true",
                     help="also run the 16-draw 120x64 hill batch to convergence (minutes)")
     args = ap.parse_args()
@@ -508,6 +551,10 @@ def main():
     if not args.no_warm:
         warm = warm_leg(2)
 
+    warm_fd = None
+    if not args.no_warm_fd:
+        warm_fd = warm_fd_leg(not args.no_cpu_baseline)
+
     hills = None
     if not args.no_hills:
         hills = hills_leg(args.hills_full, _cores(), not args.no_cpu_baseline)
@@ -544,6 +591,7 @@ def main():
         "mixed": mixed,
         "served": served,
         "warm_start": warm,
+        "warm_fd_jacobian": warm_fd,
         "hills": hills,
         "block_size": batch.last_timing()["block_size"],
         "arena_in_lds": batch.last_timing()["arena_in_lds"],
