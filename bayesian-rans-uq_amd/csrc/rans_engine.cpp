@@ -1698,8 +1698,9 @@ rq_server *rq_server_create(const rq_serve_options *opt, char *err, size_t errca
         std::memset(S->ctl, 0, sizeof(ServeControl));
         std::memset(S->ring, 0, sizeof(ServeRequest) * S->ring_size);
         std::memset(S->outs, 0, sizeof(ServeOutput) * S->ring_size);
-        HIPCHECK(hipMalloc(&S->d_head, sizeof(unsigned long long)));
-        HIPCHECK(hipMemset(S->d_head, 0, sizeof(unsigned long long)));
+        // [0] tickets claimed, [1] the device-side mirror of the ring tail
+        HIPCHECK(hipMalloc(&S->d_head, 2 * sizeof(unsigned long long)));
+        HIPCHECK(hipMemset(S->d_head, 0, 2 * sizeof(unsigned long long)));
         HIPCHECK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
         S->slot_seq.assign(S->ring_size, -1);
         S->seq_ticket.assign(S->ring_size, -1);

@@ -2748,14 +2748,24 @@ __global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict_
     __shared__ double coeffs[11];
     for (;;) {
         if (threadIdx.x == 0) {
+            // Idle workgroups poll a device-memory mirror of the ring tail
+            // (head[1], an L2 hit) and read the host's tail only every
+            // host_every polls, publishing what they see to the mirror.
+            // Reads of host memory from every CU at full rate saturate the
+            // path to the host and slowed the busy workgroups 5-7x. Polls
+            // are relaxed; the acquire fences run only when something new is
+            // seen: before publishing a host tail (the host's entries
+            // happen-before it) and after claiming a ticket.
+            unsigned long long *mirror = head + 1;
+            const int host_every = gridDim.x > 16 ? (int)(gridDim.x / 16) : 1;
             long long got = -1, idle = 0;
-            for (;;) {
-                const long long tail =
-                    __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            for (int poll = (int)blockIdx.x;; ++poll) {
+                const long long tail = (long long)__hip_atomic_load(mirror, __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT);
                 unsigned long long h =
                     __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if ((long long)h < tail) {
-                    if (__hip_atomic_compare_exchange_strong(head, &h, h + 1, __ATOMIC_ACQ_REL,
+                    if (__hip_atomic_compare_exchange_strong(head, &h, h + 1, __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT)) {
                         got = (long long)h;
@@ -2763,11 +2773,23 @@ __global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict_
                     }
                     continue;
                 }
-                if (__hip_atomic_load(&ctl->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM))
-                    break;
+                if (poll % host_every == 0) {
+                    const long long ht =
+                        __hip_atomic_load(&ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (ht > tail) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+                        __hip_atomic_fetch_max(mirror, (unsigned long long)ht, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                        continue;
+                    }
+                    if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                        break;
+                }
                 if (++idle > spin_limit) break;
                 __builtin_amdgcn_s_sleep(100);
             }
+            // the request's entries happen-before the mirror value it read
+            if (got >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
             claimed = got;
         }
         __syncthreads();
