@@ -285,7 +285,7 @@ def hills_leg(full, cores, cpu):
         if cpu and key == "hill_120x64":
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle_py
-            ccap = 60
+            ccap = cap  # the same capped window as the GPU rate
             prob.settings.max_iterations = ccap
             O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
                                         prob.init_vector())
