@@ -84,7 +84,13 @@ class RqBatchOut(C.Structure):
                 ("fields", _dp), ("krylov_iterations", _i64p), ("phase_cycles", _i64p)]
 
 
-EXPORTS = ["rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
+class RqTuneOut(C.Structure):
+    _fields_ = [("n_candidates", C.c_int32), ("block_size", C.c_int32 * 16),
+                ("grid_blocks", C.c_int32 * 16), ("kernel_ms", C.c_double * 16),
+                ("chosen_block_size", C.c_int32), ("chosen_grid_blocks", C.c_int32)]
+
+
+EXPORTS = ["rq_engine_autotune", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
            "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
            "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info",
            "rq_server_create", "rq_server_add_problem", "rq_server_submit", "rq_server_poll",
@@ -129,6 +135,9 @@ def load_library(path: str = LIB_PATH):
     lib.rq_engine_last_timing.restype = C.c_int
     lib.rq_engine_destroy.argtypes = [C.c_void_p]
     lib.rq_engine_destroy.restype = None
+    lib.rq_engine_autotune.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int32, C.c_int32,
+                                       C.POINTER(RqTuneOut), C.c_char_p, C.c_size_t]
+    lib.rq_engine_autotune.restype = C.c_int
     lib.rq_build_info.argtypes = []
     lib.rq_build_info.restype = C.c_char_p
     _lib = lib
@@ -429,6 +438,24 @@ class ForwardBatch:
                                        C.byref(nbytes), C.byref(gb))
         return {"kernel_ms": ms.value, "block_size": bs.value, "arena_in_lds": bool(lds.value),
                 "arena_bytes": nbytes.value, "grid_blocks": gb.value}
+
+    def autotune(self, theta, iterations: int = 200) -> dict:
+        """Times every candidate launch shape on these rows (each solve capped
+        at `iterations` outer iterations) and keeps the fastest for later
+        evaluations (rq_engine_autotune)."""
+        theta = np.ascontiguousarray(np.atleast_2d(np.asarray(theta, dtype=np.float64)))
+        if theta.shape[1] != self.n_parameters():
+            raise ValueError("theta must have shape (n_rows, n_parameters)")
+        out = RqTuneOut()
+        err = C.create_string_buffer(512)
+        rc = self.lib.rq_engine_autotune(self._h, theta.ctypes.data_as(_dp), theta.shape[0],
+                                         self.n_parameters(), int(iterations), C.byref(out),
+                                         err, 512)
+        BatchBuffers.check(rc, err)
+        cands = [{"block_size": out.block_size[i], "grid_blocks": out.grid_blocks[i],
+                  "kernel_ms": out.kernel_ms[i]} for i in range(out.n_candidates)]
+        return {"candidates": cands, "block_size": out.chosen_block_size,
+                "grid_blocks": out.chosen_grid_blocks}
 
     def evaluate(self, theta, workers: int = 1, fields: bool = False) -> ForwardBatchResult:
         theta = np.asarray(theta, dtype=np.float64)

@@ -1456,6 +1456,83 @@ int guarded(char *err, size_t errcap, F &&f) {
 }  // namespace
 
 // ----------------------------------------------------------------------------
+// Launch-shape autotuner: every candidate workgroup size (and, for arenas in
+// HBM, workgroups per draw) runs the caller's rows for a capped number of
+// outer iterations; the fastest kernel time is kept for later evaluations.
+// The shape changes only the order of parallel reductions.
+// ----------------------------------------------------------------------------
+namespace {
+
+int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
+                  int32_t iterations, rq_tune_out *out) {
+    if (!E || !out || (n_rows > 0 && !theta)) throw Fail{RQ_EINVAL, "null argument"};
+    if (n_rows <= 0) throw Fail{RQ_EINVAL, "the autotuner needs at least one coefficient row"};
+    if (n_params != E->n_active)
+        throw Fail{RQ_EINVAL, "batch rows must all have n_parameters entries"};
+    if (iterations <= 0) throw Fail{RQ_EINVAL, "the autotuner needs a positive iteration cap"};
+    struct Cand {
+        int block, grid;
+    };
+    std::vector<Cand> cands;
+    const bool mixed = E->settings.mixed_precision != 0;
+    for (int b : {64, 128, 256, 512})
+        if (!(mixed && b == 512)) cands.push_back({b, 1});
+    if (!E->lds && !mixed && E->nc >= 1024)
+        for (int g : {2, 4, 8, 16, 32, 64})
+            if (g * 128 <= E->nc) cands.push_back({kGridBlock, g});
+    const int saved_block = E->block, saved_grid = E->settings.grid_blocks;
+    const bool saved_user = E->block_user;
+    std::vector<RowOpts> opts(n_rows);
+    for (auto &o : opts) o.max_iter = iterations;
+    std::vector<int32_t> st(n_rows), it(n_rows);
+    std::vector<double> ll(n_rows), w(n_rows), rate(n_rows),
+        pred((size_t)n_rows * std::max(E->nobs, 1));
+    rq_batch_out o = one_row_out(st.data(), ll.data(), pred.data(), it.data(), w.data(),
+                                 rate.data());
+    const int nmax = (int)(sizeof(out->block_size) / sizeof(out->block_size[0]));
+    out->n_candidates = 0;
+    double best = std::numeric_limits<double>::infinity();
+    int best_block = saved_block, best_grid = saved_grid;
+    for (const Cand &c : cands) {
+        if (out->n_candidates >= nmax) break;
+        E->block = c.block;
+        E->block_user = true;
+        E->settings.grid_blocks = c.grid;
+        double ms = std::numeric_limits<double>::quiet_NaN();
+        try {
+            evaluate_impl(E, theta, n_rows, n_params, 1, &o, nullptr, nullptr, opts.data());
+            // a grid request the device cannot hold falls back to one workgroup
+            if (c.grid == 1 || E->last_nblk == c.grid) ms = E->last_ms;
+        } catch (const Fail &) {
+            // a shape this device cannot launch is skipped
+        }
+        const int k = out->n_candidates++;
+        out->block_size[k] = c.block;
+        out->grid_blocks[k] = c.grid;
+        out->kernel_ms[k] = ms;
+        if (std::isfinite(ms) && ms < best) {
+            best = ms;
+            best_block = c.block;
+            best_grid = c.grid;
+        }
+    }
+    if (std::isfinite(best)) {
+        E->block = best_block;
+        E->block_user = true;
+        E->settings.grid_blocks = best_grid;
+    } else {
+        E->block = saved_block;
+        E->block_user = saved_user;
+        E->settings.grid_blocks = saved_grid;
+    }
+    out->chosen_block_size = E->block;
+    out->chosen_grid_blocks = E->settings.grid_blocks;
+    return RQ_OK;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
 // Serving: registered problems, a persistent grid of workgroups (one slot
 // each, as many as the HBM budget and the device's co-resident capacity
 // allow), a request ring and a result ring in pinned host memory.
@@ -1643,6 +1720,13 @@ int rq_engine_evaluate(rq_engine *engine, const double *theta, int64_t n_rows, i
         set_err(err, errcap, e.what());
         return RQ_EDEVICE;
     }
+}
+
+int rq_engine_autotune(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
+                       int32_t iterations, rq_tune_out *out, char *err, size_t errcap) {
+    return guarded(err, errcap, [&]() {
+        return autotune_impl(engine, theta, n_rows, n_params, iterations, out);
+    });
 }
 
 int32_t rq_engine_n_observations(const rq_engine *e) { return e ? e->nobs : -1; }

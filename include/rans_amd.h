@@ -151,6 +151,22 @@ int rq_engine_last_timing(const rq_engine *engine, double *kernel_ms, int32_t *b
 
 void rq_engine_destroy(rq_engine *engine);
 
+/* No counterpart in the reference: the launch-shape autotuner. Every
+ * candidate workgroup size (64, 128, 256, 512; not 512 with mixed precision)
+ * and, for arenas in HBM of at least 1024 cells, every workgroups-per-draw
+ * count 2..64 that the device can hold runs the given rows with each solve
+ * capped at `iterations` outer iterations; the fastest kernel time becomes
+ * the engine's launch shape for later evaluations. kernel_ms is NaN for a
+ * shape that could not run. */
+typedef struct rq_tune_out {
+    int32_t n_candidates;
+    int32_t block_size[16], grid_blocks[16];
+    double kernel_ms[16];
+    int32_t chosen_block_size, chosen_grid_blocks;
+} rq_tune_out;
+int rq_engine_autotune(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
+                       int32_t iterations, rq_tune_out *out, char *err, size_t errcap);
+
 /* Library identity: compiler, target and build flags. */
 const char *rq_build_info(void);
 

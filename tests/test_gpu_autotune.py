@@ -1,0 +1,78 @@
+"""The launch-shape autotuner (rq_engine_autotune, no reference counterpart):
+every candidate is timed, the fastest shape is kept, and evaluations after
+tuning agree with an untuned engine to the parity bar (the shape only changes
+the order of parallel reductions)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cfdr_amd import cases, engine
+from fixtures import max_rel, max_scaled
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(prob, **kw):
+    return engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
+                               prob.u_init, prob.p_init, prob.k_init, prob.omega_init, **kw)
+
+
+def _check_report(rep, blocks, grids):
+    shapes = [(c["block_size"], c["grid_blocks"]) for c in rep["candidates"]]
+    assert shapes == [(b, 1) for b in blocks] + [(256, g) for g in grids]
+    times = [c["kernel_ms"] for c in rep["candidates"]]
+    assert all(t > 0 for t in times if not math.isnan(t))
+    finite = [(t, s) for t, s in zip(times, shapes) if not math.isnan(t)]
+    assert finite, "no candidate ran"
+    assert (rep["block_size"], rep["grid_blocks"]) == min(finite)[1]
+
+
+def test_channel_tuned_rows_match_untuned():
+    prob = cases.channel_problem()
+    theta = cases.channel_draws(8)
+    tuned = _batch(prob)
+    rep = tuned.autotune(theta, iterations=300)
+    print("channel 1x128 candidates", rep, flush=True)
+    _check_report(rep, [64, 128, 256, 512], [])
+    t = tuned.last_timing()
+    assert t["arena_in_lds"]
+    a = tuned.evaluate(theta)
+    assert tuned.last_timing()["block_size"] == rep["block_size"]
+    b = _batch(prob).evaluate(theta)
+    assert list(a.statuses) == list(b.statuses)
+    assert all(s == engine.EvaluationStatus.Converged for s in a.statuses)
+    assert max_rel(a.predictions, b.predictions) <= 1e-6
+    np.testing.assert_allclose(a.log_likelihoods, b.log_likelihoods, rtol=1e-6)
+
+
+def test_hill_grid_candidates_and_capped_fields_agree():
+    prob = cases.hill_problem(max_iterations=60)
+    theta = cases.hill_draws(2)
+    tuned = _batch(prob)
+    rep = tuned.autotune(theta, iterations=20)
+    print("hill 120x64 candidates", rep, flush=True)
+    _check_report(rep, [64, 128, 256, 512], [2, 4, 8, 16, 32])
+    a = tuned.evaluate(theta, fields=True)
+    t = tuned.last_timing()
+    assert t["grid_blocks"] == rep["grid_blocks"]
+    b = _batch(prob, block_size=64, grid_blocks=1).evaluate(theta, fields=True)
+    assert list(a.statuses) == list(b.statuses)
+    assert list(a.iteration_counts) == list(b.iteration_counts)
+    nc = prob.mesh.n_cells
+    fa, fb = a.fields.reshape(len(theta), -1), b.fields.reshape(len(theta), -1)
+    for i in range(len(theta)):
+        for v, name in enumerate(("Ux", "Uy", "p", "k", "omega")):
+            sl = slice(v * nc, (v + 1) * nc)
+            assert max_scaled(fa[i, sl], fb[i, sl]) <= 1e-6, (i, name)
+
+
+def test_autotune_rejects_bad_arguments():
+    prob = cases.channel_problem(ny=32, n_probes=4)
+    fb = _batch(prob)
+    with pytest.raises(ValueError):
+        fb.autotune(np.zeros((0, 2)))
+    with pytest.raises(ValueError):
+        fb.autotune(cases.channel_draws(2), iterations=0)
