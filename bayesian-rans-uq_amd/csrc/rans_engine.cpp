@@ -29,7 +29,7 @@ hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                         unsigned long long *head, double *arenas, int64_t arena_stride,
                         double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
                         hipStream_t stream);
-int grid_co_resident(size_t lds_bytes);
+int grid_co_resident(size_t lds_bytes, bool mixed);
 
 // meshes from this size up run multi-workgroup draws when the batch leaves CUs idle
 constexpr int kGridAutoCells = 4096;
@@ -163,13 +163,9 @@ void validate_settings(const rq_settings &s) {
         throw Fail{RQ_EINVAL, "convergence window exceeds the engine limit of 256"};
     if (s.mixed_precision != 0 && s.mixed_precision != 1)
         throw Fail{RQ_EINVAL, "mixed_precision must be 0 or 1"};
-    if (s.mixed_precision && s.block_size == 512)
-        throw Fail{RQ_EINVAL, "mixed precision runs with block sizes up to 256"};
     if (s.profile != 0 && s.profile != 1) throw Fail{RQ_EINVAL, "profile must be 0 or 1"};
     if (s.grid_blocks < 0 || s.grid_blocks > kMaxGridBlocks)
         throw Fail{RQ_EINVAL, "grid_blocks must be in [0, 64]"};
-    if (s.grid_blocks > 1 && s.mixed_precision)
-        throw Fail{RQ_EINVAL, "mixed precision runs with one workgroup per draw"};
     if (s.block_size != 0 && s.block_size != 64 && s.block_size != 128 && s.block_size != 256 &&
         s.block_size != 512)
         throw Fail{RQ_EINVAL, "block_size must be 0, 64, 128, 256 or 512"};
@@ -850,8 +846,6 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.profile = s.profile;
         S.grid_blocks = s.grid_blocks;
         HIPCHECK(hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device));
-        if (s.mixed_precision && nc > 256)
-            throw Fail{RQ_EINVAL, "mixed precision is available for meshes of up to 256 cells"};
         E->block_user = s.block_size != 0;
         if (s.block_size) E->block = s.block_size;
         else E->block = nc <= 64 ? 64 : nc <= 128 ? 128 : nc <= 256 ? 256 : 512;
@@ -1049,7 +1043,7 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
     if (nblk > 1) {
         // every workgroup of the padded grid (draws spread over the XCDs) must
         // be co-resident
-        const int co = grid_co_resident(grid_lds);
+        const int co = grid_co_resident(grid_lds, E->settings.mixed_precision != 0);
         const int per_xcd = (ns + kXcds - 1) / kXcds;
         const int fit = co / (kXcds * per_xcd);
         nblk = std::min(nblk, fit);
@@ -1474,10 +1468,8 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         int block, grid;
     };
     std::vector<Cand> cands;
-    const bool mixed = E->settings.mixed_precision != 0;
-    for (int b : {64, 128, 256, 512})
-        if (!(mixed && b == 512)) cands.push_back({b, 1});
-    if (!E->lds && !mixed && E->nc >= 1024)
+    for (int b : {64, 128, 256, 512}) cands.push_back({b, 1});
+    if (!E->lds && E->nc >= 1024)
         for (int g : {2, 4, 8, 16, 32, 64})
             if (g * 128 <= E->nc) cands.push_back({kGridBlock, g});
     const int saved_block = E->block, saved_grid = E->settings.grid_blocks;

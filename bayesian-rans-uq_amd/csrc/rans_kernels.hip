@@ -20,6 +20,13 @@
 
 #include "rans_device.h"
 
+// The file is compiled twice: part 0 holds the FP64 batch, multi-workgroup
+// and serving kernels with every launch entry point, part 1 the
+// mixed-precision instantiations of the batch kernels.
+#ifndef RQ_KERNEL_PART
+#define RQ_KERNEL_PART 0
+#endif
+
 namespace rq {
 
 #define DEV __device__ __forceinline__
@@ -172,6 +179,20 @@ struct GPtr {
     GlobalDouble *p;
     DEV GRef operator[](int i) const { return GRef{p + i}; }
 };
+// The same for the FP32 Krylov vectors of a multi-workgroup draw.
+typedef __attribute__((address_space(1))) float GlobalFloat;
+struct GRefF {
+    GlobalFloat *p;
+    DEV operator float() const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    DEV const GRefF &operator=(float v) const {
+        *p = v;
+        return *this;
+    }
+    DEV const GRefF &operator=(const GRefF &o) const { return *this = float(o); }
+    DEV const GRefF &operator+=(float v) const { return *this = float(*this) + v; }
+};
 
 template <int BS>
 struct Reducer {
@@ -239,6 +260,7 @@ struct Space;
 template <>
 struct Space<true> {
     typedef __attribute__((address_space(3))) double D;
+    typedef __attribute__((address_space(3))) float F;
     typedef __attribute__((address_space(3))) const int I;
     typedef __attribute__((address_space(3))) const v4i I4;
     typedef __attribute__((address_space(3))) const v2i I2;
@@ -246,6 +268,7 @@ struct Space<true> {
 template <>
 struct Space<false> {
     typedef __attribute__((address_space(1))) double D;
+    typedef __attribute__((address_space(1))) float F;
     typedef __attribute__((address_space(1))) const int I;
     typedef __attribute__((address_space(1))) const v4i I4;
     typedef __attribute__((address_space(1))) const v2i I2;
@@ -270,7 +293,7 @@ struct Krylov {
     double initial;
 };
 
-template <int BS, bool kLDS, bool kGrid = false>
+template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false>
 struct Draw {
     const DevMesh &M;
     const DevSettings &S;
@@ -1678,10 +1701,20 @@ struct Draw {
             for (int i = i0; i < L.n; i += is) r[i] = b[i] - coarse_row(L, i, x);
         }
         g ? sync() : bsync();
+        const WPtr rf = wp(L.r);
+        amg_restrict(l, g, [&](int m) { return l == 0 ? double(C(m, V_ks)) : double(rf[m]); });
+    }
+
+    // Restriction of level l's residual (res(m) of member m) into the next
+    // level's right-hand side: aggregate sums in member order.
+    template <class F>
+    DEV void amg_restrict(int l, bool g, F res) {
+        const AmgMesh &G = M.amg;
+        const AmgLevel &L = G.lev[l];
+        const int i0 = g ? gt : tid, is = g ? gs : BS;
         const AmgLevel &Cl = G.lev[l + 1];
-        const WPtr bc = wp(Cl.b), rf = wp(L.r);
+        const WPtr bc = wp(Cl.b);
         // member lists are padded to a multiple of 4 with -1
-        auto res = [&](int m) { return l == 0 ? double(C(m, V_ks)) : double(rf[m]); };
         for (int I = i0; I < Cl.n; I += is) {
             double a = 0.0;
             for (int q = L.mptr[I]; q < L.mptr[I + 1]; q += 4) {
@@ -1741,7 +1774,23 @@ struct Draw {
         }
     }
 
-    // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks
+    // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks (FP32:
+    // the float vectors F_R, F_Z and F_S; the coarser levels stay FP64)
+    template <bool F32>
+    DEV void amg_down_any(int l, bool g) {
+        if constexpr (F32) {
+            if (l == 0) return amg_down0_32(g);
+        }
+        amg_down(l, g);
+    }
+    template <bool F32>
+    DEV void amg_up_any(int l, bool g) {
+        if constexpr (F32) {
+            if (l == 0) return amg_up0_32(g);
+        }
+        amg_up(l, g);
+    }
+    template <bool F32 = false>
     DEV void amg_vcycle() {
         const AmgMesh &G = M.amg;
         const int nl = G.levels;
@@ -1750,8 +1799,13 @@ struct Draw {
             const WPtr X = wp(G.inv + (int64_t)n * n);
             for (int i = gt; i < n; i += gs) {
                 double a = 0.0;
-                for (int j = 0; j < n; ++j) a += X[i * n + j] * C(j, V_kr);
-                C(i, V_krh) = a;
+                if constexpr (F32) {
+                    for (int j = 0; j < n; ++j) a += X[i * n + j] * (double)float(Fv(j, F_R));
+                    Fv(i, F_Z) = (float)a;
+                } else {
+                    for (int j = 0; j < n; ++j) a += X[i * n + j] * C(j, V_kr);
+                    C(i, V_krh) = a;
+                }
             }
             sync();
             return;
@@ -1760,9 +1814,9 @@ struct Draw {
         int ls = 1;
         if constexpr (kGrid)
             while (ls < nl - 1 && G.lev[ls].n > kAmgBlockRows) ++ls;
-        for (int l = 0; l < ls; ++l) amg_down(l, kGrid);
+        for (int l = 0; l < ls; ++l) amg_down_any<F32>(l, kGrid);
         if (!kGrid || rank == 0) {
-            for (int l = ls; l < nl - 1; ++l) amg_down(l, false);
+            for (int l = ls; l < nl - 1; ++l) amg_down_any<F32>(l, false);
             // coarsest: x = A^-1 b, with b staged in LDS; the inverse was
             // written by this workgroup, so its rows are read with plain loads
             const int n = G.coarse_n;
@@ -1777,10 +1831,10 @@ struct Draw {
                 x[i] = a;
             }
             bsync();
-            for (int l = nl - 2; l >= ls; --l) amg_up(l, false);
+            for (int l = nl - 2; l >= ls; --l) amg_up_any<F32>(l, false);
         }
         if constexpr (kGrid) sync();  // rank 0's levels are visible to every workgroup
-        for (int l = ls - 1; l >= 0; --l) amg_up(l, kGrid);
+        for (int l = ls - 1; l >= 0; --l) amg_up_any<F32>(l, kGrid);
     }
 
     // AMGSolver::solve from x = 0 (LinearSolver.hpp:908-970): kr holds b.
@@ -1839,11 +1893,308 @@ struct Draw {
         return res;
     }
 
+    // -------------------------------------- mixed precision (block)
+    // FP32 Krylov for meshes beyond the wave-resident path (arenas in HBM,
+    // multi-workgroup draws, and LDS arenas of more than 256 cells). As in
+    // the wave-resident form, the FP32 iteration solves the defect equation
+    // A d = r0 / |r0| from d = 0, with r0 = b - A x0 the FP64 residual in kr,
+    // and x = x0 + |r0| d is formed in FP64; the stopping rule is the
+    // solver's own (|r| / |r0| < tol). Matrix (rounded from the FP64
+    // assembly), vectors and per-thread dot products are FP32; the partial
+    // sums meet in the FP64 reduction. Storage: each of the seven double
+    // Krylov slots after kr holds two float vectors (in LDS records, the two
+    // halves of the slot's double; in HBM, the slot's nc doubles read as
+    // 2 nc floats). Multigrid levels below level 0 stay FP64.
+    enum F32Vec {
+        F_DG = 0, F_O0 = 1,  // matrix: diagonal, slot off-diagonals (1..4)
+        F_R = 5, F_RH = 6, F_P = 7, F_V = 8, F_T = 9, F_PH = 10, F_SH = 11, F_X = 12,
+        F_Z = F_RH, F_AP = F_V, F_S = F_T, F_T2 = F_PH
+    };
+    using FD = typename Space<kLDS>::F;
+    using RefF = typename std::conditional<kGrid, GRefF, FD &>::type;
+    DEV RefF Fv(int i, int k) const {
+        // float k of the record is float 2 V_krh + k of the record; in HBM the
+        // float vectors after V_krh's doubles are consecutive nc-long arrays
+        FD *p;
+        if constexpr (kLDS) p = (FD *)(Cb + V_krh) + (__mul24(i, 2 * kNA) + k);
+        else p = (FD *)(Cb + (int64_t)V_krh * nc) + (k * nc + i);
+        if constexpr (kGrid) return GRefF{(GlobalFloat *)p};
+        else return *p;
+    }
+    DEV static float jacf32(float r, float d) { return (fabsf(d) > 1e-30f) ? r / d : r; }
+
+    // every workgroup converts its own rows
+    DEV void f32_matrix() {
+        for (int ci = gt; ci < nc; ci += gs) {
+            const v4i nb4 = slot4[2 * ci + 1];
+            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            Fv(ci, F_DG) = (float)C(ci, V_diag);
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s)
+                Fv(ci, F_O0 + s) = nbv[s] >= 0 ? (float)off(ci, s) : 0.0f;
+        }
+    }
+    template <class F>
+    DEV float spmv32_f(int ci, F x) const {
+        const v4i nb4 = slot4[2 * ci + 1];
+        const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        float y = float(Fv(ci, F_DG)) * x(ci);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if (nbv[s] >= 0) y += float(Fv(ci, F_O0 + s)) * x(nbv[s]);
+        return y;
+    }
+    DEV float spmv32(int ci, int xv) const {
+        return spmv32_f(ci, [&](int j) { return float(Fv(j, xv)); });
+    }
+    // x = x0 + |r0| d, then a barrier (callers read their own rows of x)
+    DEV void defect32(double r0) {
+        for (int ci = gt; ci < nc; ci += gs)
+            C(ci, V_kx) = C(ci, V_kx) + r0 * (double)float(Fv(ci, F_X));
+        sync();
+    }
+    // r = r0 / |r0| into F_R (and F_RH when rh), d = 0; returns r.r
+    DEV double f32_start(double inv, bool rh) {
+        float a = 0.0f;
+        for (int ci = gt; ci < nc; ci += gs) {
+            const float r = (float)(C(ci, V_kr) * inv);
+            Fv(ci, F_R) = r;
+            if (rh) Fv(ci, F_RH) = r;
+            Fv(ci, F_X) = 0.0f;
+            a += r * r;
+        }
+        return (double)a;
+    }
+
+    DEV Krylov bicgstab32(double rr, int maxIter, double tol64) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        const float tol = (float)tol64;
+        f32_matrix();
+        double a0[1] = {f32_start(1.0 / r0, true)};
+        for (int ci = gt; ci < nc; ci += gs) {
+            Fv(ci, F_P) = 0.0f;
+            Fv(ci, F_V) = 0.0f;
+        }
+        red.sum(a0);
+        float rho = 1, alpha = 1, omega = 1;
+        float rhoN = (float)a0[0];
+        for (int it = 0; it < maxIter; ++it) {
+            if (fabsf(rhoN) < 1e-30f) break;
+            const float beta = (rhoN / (rho + 1e-30f)) * (alpha / (omega + 1e-30f));
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float pv = float(Fv(ci, F_R)) +
+                                 beta * (float(Fv(ci, F_P)) - omega * float(Fv(ci, F_V)));
+                Fv(ci, F_P) = pv;
+                Fv(ci, F_PH) = jacf32(pv, Fv(ci, F_DG));
+            }
+            sync();
+            float s1 = 0.0f;
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float vv = spmv32(ci, F_PH);
+                Fv(ci, F_V) = vv;
+                s1 += float(Fv(ci, F_RH)) * vv;
+            }
+            double a1[1] = {(double)s1};
+            red.sum(a1);
+            const float rv = (float)a1[0];
+            if (fabsf(rv) < 1e-30f) break;
+            alpha = rhoN / rv;
+            // s overwrites r (r is not read again before r = s - omega t)
+            float s2 = 0.0f;
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float sv = float(Fv(ci, F_R)) - alpha * float(Fv(ci, F_V));
+                Fv(ci, F_R) = sv;
+                Fv(ci, F_SH) = jacf32(sv, Fv(ci, F_DG));
+                s2 += sv * sv;
+            }
+            double a2[1] = {(double)s2};
+            red.sum(a2);
+            const float sn = sqrtf((float)a2[0]);
+            if (sn < tol) {
+                for (int ci = gt; ci < nc; ci += gs)
+                    Fv(ci, F_X) = float(Fv(ci, F_X)) + alpha * float(Fv(ci, F_PH));
+                defect32(r0);
+                res.iterations = it + 1;
+                res.converged = 1;
+                return res;
+            }
+            float t0 = 0.0f, t1 = 0.0f;
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float tv = spmv32(ci, F_SH);
+                Fv(ci, F_T) = tv;
+                t0 += tv * tv;
+                t1 += tv * float(Fv(ci, F_R));
+            }
+            double a3[2] = {(double)t0, (double)t1};
+            red.sum(a3);
+            const float tt = (float)a3[0];
+            omega = (tt > 1e-30f) ? (float)a3[1] / tt : 0.0f;
+            float u0 = 0.0f, u1 = 0.0f;
+            for (int ci = gt; ci < nc; ci += gs) {
+                Fv(ci, F_X) = float(Fv(ci, F_X)) + alpha * float(Fv(ci, F_PH)) +
+                              omega * float(Fv(ci, F_SH));
+                const float rv2 = float(Fv(ci, F_R)) - omega * float(Fv(ci, F_T));
+                Fv(ci, F_R) = rv2;
+                u0 += rv2 * rv2;
+                u1 += float(Fv(ci, F_RH)) * rv2;
+            }
+            double a4[2] = {(double)u0, (double)u1};
+            red.sum(a4);
+            const float rn = sqrtf((float)a4[0]);
+            res.iterations = it + 1;
+            if (rn < tol) {
+                defect32(r0);
+                res.converged = 1;
+                return res;
+            }
+            if (fabsf(omega) < 1e-30f) break;
+            rho = rhoN;
+            rhoN = (float)a4[1];
+        }
+        defect32(r0);
+        return res;
+    }
+
+    // Jacobi PCG from x = 0 (kr holds b); AMG-preconditioned when amg
+    template <bool kAmg>
+    DEV Krylov pcg32(double rr, int maxIter, double tol64) {
+        Krylov res{0, 0, 0.0};
+        const double r0 = sqrt(rr);
+        res.initial = r0;
+        if (r0 < 1e-30) {
+            res.converged = 1;
+            return res;
+        }
+        const float tol = (float)tol64;
+        if constexpr (kAmg) amg_refresh();
+        f32_matrix();
+        const double rs = f32_start(1.0 / r0, false);
+        (void)rs;
+        float s0 = 0.0f;
+        if constexpr (kAmg) {
+            sync();  // the V-cycle reads neighbours' rows of r and the matrix
+            amg_vcycle<true>();
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float z = Fv(ci, F_Z);
+                Fv(ci, F_P) = z;
+                s0 += float(Fv(ci, F_R)) * z;
+            }
+        } else {
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float r = Fv(ci, F_R);
+                const float z = jacf32(r, Fv(ci, F_DG));
+                Fv(ci, F_Z) = z;
+                Fv(ci, F_P) = z;
+                s0 += r * z;
+            }
+        }
+        double a0[1] = {(double)s0};
+        red.sum(a0);
+        float rz = (float)a0[0];
+        for (int it = 0; it < maxIter; ++it) {
+            float s1 = 0.0f;
+            for (int ci = gt; ci < nc; ci += gs) {
+                const float av = spmv32(ci, F_P);
+                Fv(ci, F_AP) = av;
+                s1 += float(Fv(ci, F_P)) * av;
+            }
+            double a1[1] = {(double)s1};
+            red.sum(a1);
+            const float pAp = (float)a1[0];
+            if (fabsf(pAp) < 1e-30f) break;
+            const float alpha = rz / pAp;
+            float s2 = 0.0f, s3 = 0.0f;
+            for (int ci = gt; ci < nc; ci += gs) {
+                Fv(ci, F_X) = float(Fv(ci, F_X)) + alpha * float(Fv(ci, F_P));
+                const float rv = float(Fv(ci, F_R)) - alpha * float(Fv(ci, F_AP));
+                Fv(ci, F_R) = rv;
+                s2 += rv * rv;
+                if constexpr (!kAmg) {
+                    const float zv = jacf32(rv, Fv(ci, F_DG));
+                    Fv(ci, F_Z) = zv;
+                    s3 += rv * zv;
+                }
+            }
+            double a2[2] = {(double)s2, (double)s3};
+            red.sum(a2);
+            const float rn = sqrtf((float)a2[0]);
+            res.iterations = it + 1;
+            if (rn < tol) {
+                defect32(r0);
+                res.converged = 1;
+                return res;
+            }
+            float rzNew = (float)a2[1];
+            if constexpr (kAmg) {
+                amg_vcycle<true>();
+                float s4 = 0.0f;
+                for (int ci = gt; ci < nc; ci += gs) s4 += float(Fv(ci, F_R)) * float(Fv(ci, F_Z));
+                double a3[1] = {(double)s4};
+                red.sum(a3);
+                rzNew = (float)a3[0];
+            }
+            const float beta = rzNew / (rz + 1e-30f);
+            for (int ci = gt; ci < nc; ci += gs)
+                Fv(ci, F_P) = float(Fv(ci, F_Z)) + beta * float(Fv(ci, F_P));
+            sync();
+            rz = rzNew;
+        }
+        defect32(r0);
+        return res;
+    }
+
+    // level 0 of the FP32 V-cycle: amg_down / amg_up on the float vectors
+    // (b = F_R, x = F_Z, residual F_S, second sweep buffer F_T2)
+    DEV void amg_down0_32(bool g) {
+        const float w = (float)kJacobiOmega;
+        const int i0 = g ? gt : tid, is = g ? gs : BS;
+        auto x0 = [&](int j) {
+            const float d = Fv(j, F_DG);
+            return fabsf(d) > 1e-30f ? w * float(Fv(j, F_R)) / d : 0.0f;
+        };
+        for (int ci = i0; ci < nc; ci += is) {
+            const float d = Fv(ci, F_DG), xi = x0(ci);
+            const float r0 = float(Fv(ci, F_R)) - spmv32_f(ci, x0);
+            Fv(ci, F_Z) = fabsf(d) > 1e-30f ? xi + w * r0 / d : xi;
+        }
+        g ? sync() : bsync();
+        for (int ci = i0; ci < nc; ci += is) Fv(ci, F_S) = float(Fv(ci, F_R)) - spmv32(ci, F_Z);
+        g ? sync() : bsync();
+        amg_restrict(0, g, [&](int m) { return (double)float(Fv(m, F_S)); });
+    }
+    DEV void amg_up0_32(bool g) {
+        const AmgMesh &G = M.amg;
+        const AmgLevel &L = G.lev[0];
+        const float w = (float)kJacobiOmega;
+        const int i0 = g ? gt : tid, is = g ? gs : BS;
+        const WPtr xc = wp(G.lev[1].x);
+        auto xp = [&](int j) { return float(Fv(j, F_Z)) + (float)double(xc[L.agg[j]]); };
+        for (int ci = i0; ci < nc; ci += is) {
+            const float d = Fv(ci, F_DG), xi = xp(ci);
+            const float r = float(Fv(ci, F_R)) - spmv32_f(ci, xp);
+            Fv(ci, F_T2) = fabsf(d) > 1e-30f ? xi + w * r / d : xi;
+        }
+        g ? sync() : bsync();
+        for (int ci = i0; ci < nc; ci += is) {
+            const float d = Fv(ci, F_DG), xi = Fv(ci, F_T2);
+            const float r = float(Fv(ci, F_R)) - spmv32(ci, F_T2);
+            Fv(ci, F_Z) = fabsf(d) > 1e-30f ? xi + w * r / d : xi;
+        }
+    }
+
     // The wave-resident path is compiled into the launch shapes that small
     // meshes use (up to 256 threads).
     DEV Krylov solve(int kind, double rr, bool zeroStart) {
-        if (kind == SOLVER_AMG && zeroStart && M.amg.levels > 0)
-            return amg_pcg(rr, S.inner_iterations, S.inner_tolerance);
+        if (kind == SOLVER_AMG && zeroStart && M.amg.levels > 0) {
+            if constexpr (kF32) return pcg32<true>(rr, S.inner_iterations, S.inner_tolerance);
+            else return amg_pcg(rr, S.inner_iterations, S.inner_tolerance);
+        }
         if constexpr (BS <= 256 && !kGrid) {
             if (nc <= 256) {
                 if (tid < 64) {
@@ -1856,6 +2207,11 @@ struct Draw {
                 __syncthreads();
                 return *kres;
             }
+        }
+        if constexpr (kF32) {
+            if (kind == SOLVER_PCG && zeroStart)
+                return pcg32<false>(rr, S.inner_iterations, S.inner_tolerance);
+            return bicgstab32(rr, S.inner_iterations, S.inner_tolerance);
         }
         if (kind == SOLVER_PCG && zeroStart) return pcg(rr, S.inner_iterations, S.inner_tolerance);
         return bicgstab(rr, S.inner_iterations, S.inner_tolerance);
@@ -2563,7 +2919,7 @@ struct BlockShared {
 // rank/nblk/grid/aux describe a multi-workgroup draw (kGrid): its workgroups
 // share the arena and meet at grid barriers; aux is this workgroup's LDS for
 // its bookkeeping replica (contraction rings and monitor values).
-template <int BS, bool kLDS, bool kGrid = false>
+template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false>
 DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &settings,
                     const double *coeffSrc, double *arena, double *amg, DevResult *res,
                     double *pred, double *fields, GridState *grid = nullptr, int rank = 0,
@@ -2604,7 +2960,7 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
     } else {
         __syncthreads();
     }
-    Draw<BS, kLDS, kGrid> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
+    Draw<BS, kLDS, kGrid, kF32> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
                             aux);
     d.cbuf = sh.coarse_b;
     d.warm = warm;
@@ -2681,7 +3037,7 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
 
 // The batch kernel: blockIdx.x -> one slot (draw); in LDS mode the arena is
 // the workgroup's dynamic LDS.
-template <int BS, bool kLDS>
+template <int BS, bool kLDS, bool kF32>
 __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restrict__ slots,
                                                           DevSettings settings,
                                                           DevResult *__restrict__ results,
@@ -2692,7 +3048,8 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
     __shared__ BlockShared<BS> sh;
     const DevSlot &slot = slots[blockIdx.x];
     const int row = slot.row;
-    solve_draw<BS, kLDS>(sh, *slot.mesh, settings, slot.coeffs, kLDS ? smem : slot.arena,
+    solve_draw<BS, kLDS, false, kF32>(sh, *slot.mesh, settings, slot.coeffs,
+                                      kLDS ? smem : slot.arena,
                          slot.amg, results + row, predictions + (int64_t)row * pred_stride,
                          fields ? fields + row * field_stride : nullptr, nullptr, 0, 1, nullptr,
                          slot.warm, slot.save, slot.tol, slot.max_iter, slot.flags);
@@ -2700,7 +3057,7 @@ __global__ void __launch_bounds__(BS) simple_solve_kernel(const DevSlot *__restr
 
 // The multi-workgroup batch kernel: nblk co-resident workgroups per draw
 // (launched cooperatively), rank = blockIdx.x % nblk.
-template <int BS>
+template <int BS, bool kF32>
 __global__ void __launch_bounds__(BS) grid_solve_kernel(const DevSlot *__restrict__ slots,
                                                         DevSettings settings,
                                                         DevResult *__restrict__ results,
@@ -2719,7 +3076,7 @@ __global__ void __launch_bounds__(BS) grid_solve_kernel(const DevSlot *__restric
     if (draw >= nslots) return;  // padding workgroup of an XCD with fewer draws
     const DevSlot &slot = slots[draw];
     const int row = slot.row;
-    solve_draw<BS, false, true>(sh, *slot.mesh, settings, slot.coeffs, slot.arena, slot.amg,
+    solve_draw<BS, false, true, kF32>(sh, *slot.mesh, settings, slot.coeffs, slot.arena, slot.amg,
                                 results + row, predictions + (int64_t)row * pred_stride,
                                 fields ? fields + row * field_stride : nullptr, states + draw,
                                 rank, nblk, smem, slot.warm, slot.save, slot.tol,
@@ -2826,11 +3183,11 @@ __global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict_
 // ----------------------------------------------------------------------------
 namespace rq {
 
-template <int BS, bool kLDS>
+template <int BS, bool kLDS, bool kF32>
 static hipError_t launch_t(const DevSlot *slots, int nslots, const DevSettings &s,
                            DevResult *results, double *pred, int pred_stride, double *fields,
                            int64_t field_stride, size_t lds_bytes, hipStream_t stream) {
-    auto kern = simple_solve_kernel<BS, kLDS>;
+    auto kern = simple_solve_kernel<BS, kLDS, kF32>;
     if (kLDS) {
         hipError_t e = hipFuncSetAttribute((const void *)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2842,16 +3199,17 @@ static hipError_t launch_t(const DevSlot *slots, int nslots, const DevSettings &
     return hipGetLastError();
 }
 
-hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int nslots,
-                               const DevSettings &s, DevResult *results, double *pred,
-                               int pred_stride, double *fields, int64_t field_stride,
-                               size_t lds_bytes, hipStream_t stream) {
-#define RQ_CASE(B)                                                                          \
-    case B:                                                                                 \
-        return lds ? launch_t<B, true>(slots, nslots, s, results, pred, pred_stride, fields, \
-                                       field_stride, lds_bytes, stream)                      \
-                   : launch_t<B, false>(slots, nslots, s, results, pred, pred_stride, fields,\
-                                        field_stride, lds_bytes, stream);
+template <bool kF32>
+static hipError_t launch_simple_t(int block, bool lds, const DevSlot *slots, int nslots,
+                                  const DevSettings &s, DevResult *results, double *pred,
+                                  int pred_stride, double *fields, int64_t field_stride,
+                                  size_t lds_bytes, hipStream_t stream) {
+#define RQ_CASE(B)                                                                           \
+    case B:                                                                                  \
+        return lds ? launch_t<B, true, kF32>(slots, nslots, s, results, pred, pred_stride,   \
+                                             fields, field_stride, lds_bytes, stream)        \
+                   : launch_t<B, false, kF32>(slots, nslots, s, results, pred, pred_stride,  \
+                                              fields, field_stride, lds_bytes, stream);
     switch (block) {
         RQ_CASE(64)
         RQ_CASE(128)
@@ -2863,13 +3221,13 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
 #undef RQ_CASE
 }
 
-
-int grid_co_resident(size_t lds_bytes) {
+template <bool kF32>
+static int grid_co_resident_t(size_t lds_bytes) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    auto kern = grid_solve_kernel<kGridBlock>;
+    auto kern = grid_solve_kernel<kGridBlock, kF32>;
     if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds_bytes) != hipSuccess)
         return 0;
@@ -2879,11 +3237,12 @@ int grid_co_resident(size_t lds_bytes) {
     return per * cus;
 }
 
-hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,
-                             DevResult *results, double *pred, int pred_stride, double *fields,
-                             int64_t field_stride, GridState *states, size_t lds_bytes,
-                             hipStream_t stream) {
-    auto kern = grid_solve_kernel<kGridBlock>;
+template <bool kF32>
+static hipError_t launch_grid_t(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,
+                                DevResult *results, double *pred, int pred_stride,
+                                double *fields, int64_t field_stride, GridState *states,
+                                size_t lds_bytes, hipStream_t stream) {
+    auto kern = grid_solve_kernel<kGridBlock, kF32>;
     hipError_t e = hipFuncSetAttribute((const void *)kern,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
@@ -2894,6 +3253,62 @@ hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const D
     const int per_xcd = (nslots + kXcds - 1) / kXcds;  // draws on the fullest XCD
     return hipLaunchCooperativeKernel((const void *)kern, dim3(kXcds * per_xcd * nblk),
                                       dim3(kGridBlock), args, (unsigned)lds_bytes, stream);
+}
+
+// Mixed precision runs in kernels of its own, compiled in a translation unit
+// of its own (RQ_KERNEL_PART 1), so the FP32 block Krylov code neither shares
+// register allocation with the FP64 kernels nor lengthens their build.
+#if RQ_KERNEL_PART == 1
+hipError_t launch_simple_solve_f32(int block, bool lds, const DevSlot *slots, int nslots,
+                                   const DevSettings &s, DevResult *results, double *pred,
+                                   int pred_stride, double *fields, int64_t field_stride,
+                                   size_t lds_bytes, hipStream_t stream) {
+    return launch_simple_t<true>(block, lds, slots, nslots, s, results, pred, pred_stride, fields,
+                                 field_stride, lds_bytes, stream);
+}
+int grid_co_resident_f32(size_t lds_bytes) { return grid_co_resident_t<true>(lds_bytes); }
+hipError_t launch_grid_solve_f32(const DevSlot *slots, int nslots, int nblk,
+                                 const DevSettings &s, DevResult *results, double *pred,
+                                 int pred_stride, double *fields, int64_t field_stride,
+                                 GridState *states, size_t lds_bytes, hipStream_t stream) {
+    return launch_grid_t<true>(slots, nslots, nblk, s, results, pred, pred_stride, fields,
+                               field_stride, states, lds_bytes, stream);
+}
+#else
+hipError_t launch_simple_solve_f32(int block, bool lds, const DevSlot *slots, int nslots,
+                                   const DevSettings &s, DevResult *results, double *pred,
+                                   int pred_stride, double *fields, int64_t field_stride,
+                                   size_t lds_bytes, hipStream_t stream);
+int grid_co_resident_f32(size_t lds_bytes);
+hipError_t launch_grid_solve_f32(const DevSlot *slots, int nslots, int nblk,
+                                 const DevSettings &s, DevResult *results, double *pred,
+                                 int pred_stride, double *fields, int64_t field_stride,
+                                 GridState *states, size_t lds_bytes, hipStream_t stream);
+
+hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int nslots,
+                               const DevSettings &s, DevResult *results, double *pred,
+                               int pred_stride, double *fields, int64_t field_stride,
+                               size_t lds_bytes, hipStream_t stream) {
+    if (s.mixed_precision)
+        return launch_simple_solve_f32(block, lds, slots, nslots, s, results, pred, pred_stride,
+                                       fields, field_stride, lds_bytes, stream);
+    return launch_simple_t<false>(block, lds, slots, nslots, s, results, pred, pred_stride,
+                                  fields, field_stride, lds_bytes, stream);
+}
+
+int grid_co_resident(size_t lds_bytes, bool mixed) {
+    return mixed ? grid_co_resident_f32(lds_bytes) : grid_co_resident_t<false>(lds_bytes);
+}
+
+hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,
+                             DevResult *results, double *pred, int pred_stride, double *fields,
+                             int64_t field_stride, GridState *states, size_t lds_bytes,
+                             hipStream_t stream) {
+    if (s.mixed_precision)
+        return launch_grid_solve_f32(slots, nslots, nblk, s, results, pred, pred_stride, fields,
+                                     field_stride, states, lds_bytes, stream);
+    return launch_grid_t<false>(slots, nslots, nblk, s, results, pred, pred_stride, fields,
+                                field_stride, states, lds_bytes, stream);
 }
 
 hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
@@ -2909,5 +3324,6 @@ hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
                        outs, ring_size, head, arenas, arena_stride, amgs, amg_stride, spin_limit);
     return hipGetLastError();
 }
+#endif
 
 }  // namespace rq

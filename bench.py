@@ -18,7 +18,8 @@ run in one persistent-kernel launch. The line printed is
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 
-Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
+The sweep runs on the launch shape the autotuner picks (--no-autotune keeps the
+default). Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
 and FP64 defect correction), served (Poisson arrivals of single requests on
 64-512 cell channels against the plain one-launch-per-request path), hills
 (the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
@@ -256,32 +257,57 @@ def served_leg(n_requests, rate):
                     "256- and 512-cell channels end Unconverged at the cap on every path"}
 
 
+def _mixed_vs_fp64(rm, r):
+    """FP32-Krylov rows against the FP64 rows of the same batch: status
+    agreement and the largest relative probe difference of rows that both
+    report predictions for."""
+    ok = np.isin(r.status_codes, (0, 1)) & (rm.status_codes == r.status_codes)
+    err = (float(np.max(np.abs(rm.predictions[ok] - r.predictions[ok]) /
+                        np.abs(r.predictions[ok]))) if ok.any() else None)
+    return {"statuses_match_fp64": bool(np.array_equal(rm.status_codes, r.status_codes)),
+            "converged_flags_match_fp64": bool(np.array_equal(rm.convergence_mask,
+                                                              r.convergence_mask)),
+            "max_rel_probe_diff_vs_fp64": err}
+
+
 def hills_leg(full, cores, cpu):
     """Hill meshes: the 16-draw 120x64 batch (AMG pressure, tol 1e-4) and the
     8-draw 480x256 batch, each timed per outer iteration on solves capped at
-    a few hundred (resp. ten) iterations; the engine picks the workgroups per
-    draw. The CPU oracle runs the 120x64 draws one per thread with the same
-    cap. With full=True the 16-draw 120x64 batch also runs to convergence
-    (cap 25000; minutes)."""
+    a few hundred (resp. ten) iterations, on the launch shape the autotuner
+    picks (timed on a shorter capped run first), in FP64 and with FP32 Krylov
+    and FP64 defect correction. The CPU oracle runs the 120x64 draws one per
+    thread with the same cap. With full=True the 16-draw 120x64 batch also
+    runs to convergence in both precisions (cap 25000; minutes)."""
     from cfdr_amd import cases, engine
     out = {}
-    for key, nx, ny, nd, cap in (("hill_120x64", 120, 64, 16, 200),
-                                 ("hill_480x256", 480, 256, 8, 10)):
+    for key, nx, ny, nd, cap, tune_its in (("hill_120x64", 120, 64, 16, 200, 20),
+                                           ("hill_480x256", 480, 256, 8, 10, 2)):
         prob = cases.hill_problem(nx=nx, ny=ny, max_iterations=cap)
         th = cases.hill_draws(nd)
-        b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
-                                prob.settings, prob.u_init, prob.p_init, prob.k_init,
-                                prob.omega_init)
-        r = b.evaluate(th)
-        t = b.last_timing()
+        legs = {}
+        for prec in ("fp64", "mixed"):
+            b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
+                                    prob.settings, prob.u_init, prob.p_init, prob.k_init,
+                                    prob.omega_init, mixed_precision=prec == "mixed")
+            tune = b.autotune(th, iterations=tune_its)
+            legs[prec] = (b.evaluate(th), b.last_timing(), tune)
+            print(f"# {key} {prec}: {legs[prec][0].kernel_ms / cap:.2f} ms per outer iteration "
+                  f"(tuned block {tune['block_size']}, {tune['grid_blocks']} per draw)",
+                  file=sys.stderr, flush=True)
+        r, t, tune = legs["fp64"]
+        rm, tm, tunem = legs["mixed"]
         kry = np.asarray(r.krylov_iterations, dtype=float).mean(axis=0) / cap
         out[key] = {"draws": nd, "cells": prob.mesh.n_cells, "capped_iterations": cap,
                     "ms_per_outer_iteration": round(r.kernel_ms / cap, 3),
                     "draw_iterations_per_s": round(nd * cap / (r.kernel_ms / 1e3), 1),
                     "grid_blocks": t["grid_blocks"], "block_size": t["block_size"],
-                    "krylov_per_outer_iteration": [round(float(k), 1) for k in kry]}
-        print(f"# {key}: {r.kernel_ms / cap:.2f} ms per outer iteration", file=sys.stderr,
-              flush=True)
+                    "autotune": {"iterations": tune_its, "candidates": tune["candidates"]},
+                    "krylov_per_outer_iteration": [round(float(k), 1) for k in kry],
+                    "mixed": dict({"ms_per_outer_iteration": round(rm.kernel_ms / cap, 3),
+                                   "grid_blocks": tm["grid_blocks"],
+                                   "block_size": tm["block_size"],
+                                   "at": f"the {cap}-iteration cap"},
+                                  **_mixed_vs_fp64(rm, r))}
         if cpu and key == "hill_120x64":
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle_py
@@ -302,20 +328,26 @@ def hills_leg(full, cores, cpu):
     if full:
         prob = cases.hill_problem(nx=120, ny=64)
         th = cases.hill_draws(16)
-        b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
-                                prob.settings, prob.u_init, prob.p_init, prob.k_init,
-                                prob.omega_init)
-        print("# hill_120x64 to convergence (cap 25000) ...", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        r = b.evaluate(th)
-        dt = time.perf_counter() - t0
-        out["hill_120x64_full"] = {
-            "draws": 16, "seconds": round(dt, 1),
-            "converged_per_s": round(int(np.sum(r.convergence_mask)) / dt, 4),
-            "completed_per_s": round(16 / dt, 4),
-            "statuses": {s.name: int(sum(1 for x in r.statuses if x == s))
-                         for s in set(r.statuses)},
-            "iterations": [int(i) for i in r.iteration_counts]}
+        full_rows = {}
+        for prec in ("fp64", "mixed"):
+            b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
+                                    prob.settings, prob.u_init, prob.p_init, prob.k_init,
+                                    prob.omega_init, mixed_precision=prec == "mixed")
+            print(f"# hill_120x64 {prec} to convergence (cap 25000) ...", file=sys.stderr,
+                  flush=True)
+            t0 = time.perf_counter()
+            r = b.evaluate(th)
+            dt = time.perf_counter() - t0
+            full_rows[prec] = r
+            out["hill_120x64_full" + ("" if prec == "fp64" else "_mixed")] = {
+                "draws": 16, "seconds": round(dt, 1),
+                "converged_per_s": round(int(np.sum(r.convergence_mask)) / dt, 4),
+                "completed_per_s": round(16 / dt, 4),
+                "statuses": {s.name: int(sum(1 for x in r.statuses if x == s))
+                             for s in set(r.statuses)},
+                "iterations": [int(i) for i in r.iteration_counts]}
+        out["hill_120x64_full_mixed"].update(_mixed_vs_fp64(full_rows["mixed"],
+                                                            full_rows["fp64"]))
     return out
 
 
@@ -418,6 +450,8 @@ def main():
     ap.add_argument("--draws", type=int, default=64)
     ap.add_argument("--plain-sample", type=int, default=4)
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the engine's default launch shape for the sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -456,6 +490,13 @@ def main():
                                 prob.omega_init, block_size=args.block_size)
     print(f"# workload: {args.draws} draws, {N} cells, build {engine.build_info()}",
           file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune and not args.block_size:
+        # launch shape from the autotuner, timed on the sweep capped at 2000
+        # outer iterations (untimed: tuning happens once per problem)
+        tune = batch.autotune(theta, iterations=2000)
+        print(f"# autotune: block {tune['block_size']}, {tune['grid_blocks']} workgroup(s) per "
+              f"draw", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         batch.evaluate(theta)
@@ -594,6 +635,7 @@ def main():
         "warm_fd_jacobian": warm_fd,
         "hills": hills,
         "block_size": batch.last_timing()["block_size"],
+        "autotune": tune,
         "arena_in_lds": batch.last_timing()["arena_in_lds"],
     }
     print(json.dumps(line), flush=True)

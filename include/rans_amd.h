@@ -152,7 +152,7 @@ int rq_engine_last_timing(const rq_engine *engine, double *kernel_ms, int32_t *b
 void rq_engine_destroy(rq_engine *engine);
 
 /* No counterpart in the reference: the launch-shape autotuner. Every
- * candidate workgroup size (64, 128, 256, 512; not 512 with mixed precision)
+ * candidate workgroup size (64, 128, 256, 512)
  * and, for arenas in HBM of at least 1024 cells, every workgroups-per-draw
  * count 2..64 that the device can hold runs the given rows with each solve
  * capped at `iterations` outer iterations; the fastest kernel time becomes

@@ -212,13 +212,47 @@ def test_mixed_precision_matches_fp64_reference():
             assert max_rel(res.predictions[i], row["predictions"]) <= REL_OBS
 
 
-def test_mixed_precision_limits():
-    prob = small_channel()
-    with pytest.raises(ValueError, match="block sizes up to 256"):
-        _batch(prob, mixed_precision=True, block_size=512)
-    prob = inout_channel("upwind", "upwind")
-    with pytest.raises(ValueError, match="up to 256 cells"):
-        _batch(prob, mixed_precision=True)
+def _rows_of(d):
+    return (np.asarray(d["thetas"]), [r["status"] for r in d["rows"]],
+            [r["iters"] for r in d["rows"]], [r["predictions"] for r in d["rows"]])
+
+
+def test_mixed_precision_block_paths_match_fp64_reference():
+    """FP32 block Krylov with FP64 defect correction, beyond the
+    wave-resident path: 512-thread workgroups on the 128-cell channel (arena
+    in LDS) and the 288-cell inlet/outlet channel (arena in HBM, one and four
+    workgroups per draw), against the FP64 reference at REL_OBS."""
+    th, st, its, pred = _rows_of(load("ref_channel_c1.json"))
+    res = _batch(cases.channel_problem(), mixed_precision=True, block_size=512).evaluate(th)
+    for i in range(len(th)):
+        assert res.statuses[i] == st[i] and _iters_close(res.iteration_counts[i], its[i])
+        assert max_rel(res.predictions[i], pred[i]) <= REL_OBS
+    d = load("ref_channel_inout.json")["schemes"]["limitedLinearUpwind"]
+    prob = inout_channel(d["momentum_convection"], d["turbulence_convection"])
+    th, st, its, pred = _rows_of(d)
+    for g in (1, 4):
+        b = _batch(prob, mixed_precision=True, grid_blocks=g)
+        res = b.evaluate(th)
+        assert b.last_timing()["grid_blocks"] == g
+        for i in range(len(th)):
+            assert res.statuses[i] == st[i] and _iters_close(res.iteration_counts[i], its[i]), g
+            assert max_rel(res.predictions[i], pred[i]) <= REL_OBS, g
+
+
+def test_mixed_precision_amg_hill_matches_fp64_reference():
+    """FP32 AMG-preconditioned CG (level-0 smoothing and Krylov vectors in
+    FP32, coarser levels FP64) on the converged 30x16 hills, one and eight
+    workgroups per draw: the reference's statuses, and probes within the 1e-4
+    that the FP64 device AMG meets against the reference's AMG."""
+    d = load("ref_hill_small_converged.json")
+    th, st, its, pred = _rows_of(d)
+    prob = cases.hill_problem(nx=30, ny=16)
+    for g in (1, 8):
+        res = _batch(prob, mixed_precision=True, grid_blocks=g).evaluate(th)
+        for i in range(len(th)):
+            assert res.statuses[i] == st[i] == 0
+            assert abs(int(res.iteration_counts[i]) - its[i]) <= max(50, int(0.02 * its[i]))
+            assert max_rel(res.predictions[i], pred[i]) <= 1e-4
 
 
 def test_amg_matches_pcg_at_tight_tolerance():
