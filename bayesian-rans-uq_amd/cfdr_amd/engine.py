@@ -90,7 +90,7 @@ class RqTuneOut(C.Structure):
                 ("chosen_block_size", C.c_int32), ("chosen_grid_blocks", C.c_int32)]
 
 
-EXPORTS = ["rq_engine_autotune", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
+EXPORTS = ["rq_engine_autotune", "rq_engine_set_graph_replay", "rq_engine_graph_launches", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
            "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
            "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info",
            "rq_server_create", "rq_server_add_problem", "rq_server_submit", "rq_server_poll",
@@ -138,6 +138,10 @@ def load_library(path: str = LIB_PATH):
     lib.rq_engine_autotune.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int32, C.c_int32,
                                        C.POINTER(RqTuneOut), C.c_char_p, C.c_size_t]
     lib.rq_engine_autotune.restype = C.c_int
+    lib.rq_engine_set_graph_replay.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_size_t]
+    lib.rq_engine_set_graph_replay.restype = C.c_int
+    lib.rq_engine_graph_launches.argtypes = [C.c_void_p]
+    lib.rq_engine_graph_launches.restype = C.c_int64
     lib.rq_build_info.argtypes = []
     lib.rq_build_info.restype = C.c_char_p
     _lib = lib
@@ -456,6 +460,16 @@ class ForwardBatch:
                   "kernel_ms": out.kernel_ms[i]} for i in range(out.n_candidates)]
         return {"candidates": cands, "block_size": out.chosen_block_size,
                 "grid_blocks": out.chosen_grid_blocks}
+
+    def set_graph_replay(self, enable: bool = True) -> None:
+        """Capture the batch launch into a HIP graph once and replay it while
+        the row count and launch shape stay the same (rq_engine_set_graph_replay)."""
+        err = C.create_string_buffer(512)
+        BatchBuffers.check(self.lib.rq_engine_set_graph_replay(self._h, 1 if enable else 0, err,
+                                                               512), err)
+
+    def graph_launches(self) -> int:
+        return int(self.lib.rq_engine_graph_launches(self._h))
 
     def evaluate(self, theta, workers: int = 1, fields: bool = False) -> ForwardBatchResult:
         theta = np.asarray(theta, dtype=np.float64)

@@ -109,8 +109,48 @@ struct rq_engine {
     size_t cap_slots = 0, cap_results = 0, cap_pred = 0, cap_fields = 0, cap_arena = 0;
     size_t cap_amg = 0;
     double last_ms = 0.0;
+    // hipGraph replay of one-workgroup-per-draw batch launches: the launch
+    // (slot upload, solve kernel, result download through pinned staging
+    // buffers) is captured once and replayed while its key stays the same
+    struct GraphKey {
+        int ns = 0, block = 0, nobs = 0;
+        bool lds = false, fields = false;
+        size_t lds_bytes = 0;
+        const void *ptr[8] = {};
+        DevSettings set{};  // kernel argument baked into the graph
+        bool operator==(const GraphKey &o) const {
+            if (ns != o.ns || block != o.block || nobs != o.nobs || lds != o.lds ||
+                fields != o.fields || lds_bytes != o.lds_bytes ||
+                std::memcmp(&set, &o.set, sizeof(DevSettings)) != 0)
+                return false;
+            for (int i = 0; i < 8; ++i)
+                if (ptr[i] != o.ptr[i]) return false;
+            return true;
+        }
+    };
+    bool graph_on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey;
+    DevSlot *h_slots = nullptr;
+    DevResult *h_results = nullptr;
+    double *h_pred = nullptr, *h_fields = nullptr;
+    size_t hcap_slots = 0, hcap_results = 0, hcap_pred = 0, hcap_fields = 0;
+    int64_t graph_launches = 0;
+
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
 
     ~rq_engine() {
+        drop_graph();
+        if (h_slots) (void)hipHostFree(h_slots);
+        if (h_results) (void)hipHostFree(h_results);
+        if (h_pred) (void)hipHostFree(h_pred);
+        if (h_fields) (void)hipHostFree(h_fields);
         if (d_blob) (void)hipFree(d_blob);
         if (d_mesh) (void)hipFree(d_mesh);
         if (d_slots) (void)hipFree(d_slots);
@@ -876,6 +916,21 @@ void grow(T *&ptr, size_t &cap, size_t n) {
     cap = n;
 }
 
+// pinned host staging buffer of at least n elements (a new buffer has a new
+// address, which invalidates a captured graph through its key)
+template <class T>
+void grow_pinned(T *&ptr, size_t &cap, size_t n) {
+    if (n <= cap) return;
+    if (ptr) HIPCHECK(hipHostFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+    if (hipHostMalloc((void **)&ptr, n * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+        ptr = nullptr;
+        throw Fail{RQ_ENOMEM, "pinned host allocation failed"};
+    }
+    cap = n;
+}
+
 void write_row_error(const rq_batch_out *out, int64_t row, const std::string &msg) {
     if (out->row_errors && out->row_error_stride)
         std::snprintf(out->row_errors + row * out->row_error_stride, out->row_error_stride, "%s",
@@ -1055,48 +1110,117 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         for (int i = 0; i < ns; ++i) slots[i].arena = E->d_arena + (size_t)i * E->lay.stride;
     }
     E->last_nblk = nblk;
-    HIPCHECK(hipMemcpyAsync(E->d_slots, slots.data(), ns * sizeof(DevSlot), hipMemcpyHostToDevice,
-                            E->stream));
-    if (nblk > 1) {
-        grow(E->d_grid, E->cap_grid, ns);
-        HIPCHECK(hipMemsetAsync(E->d_grid, 0, ns * sizeof(GridState), E->stream));
-    }
-    HIPCHECK(hipEventRecord(E->ev0, E->stream));
-    if (nblk > 1)
-        HIPCHECK(launch_grid_solve(E->d_slots, ns, nblk, E->settings, E->d_results, E->d_pred,
-                                   std::max(nobs, 1), out->fields ? E->d_fields : nullptr, fstride,
-                                   E->d_grid, grid_lds, E->stream));
-    else
-        HIPCHECK(launch_simple_solve(E->block, E->lds, E->d_slots, ns, E->settings, E->d_results,
-                                     E->d_pred, std::max(nobs, 1),
-                                     out->fields ? E->d_fields : nullptr, fstride, E->lds_bytes,
-                                     E->stream));
-    HIPCHECK(hipEventRecord(E->ev1, E->stream));
-    std::vector<DevResult> res(ns);
-    std::vector<double> pred((size_t)ns * std::max(nobs, 1));
-    HIPCHECK(hipMemcpyAsync(res.data(), E->d_results, ns * sizeof(DevResult),
-                            hipMemcpyDeviceToHost, E->stream));
-    HIPCHECK(hipMemcpyAsync(pred.data(), E->d_pred, pred.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, E->stream));
-    std::vector<double> fields;
-    if (out->fields) {
-        fields.resize((size_t)ns * fstride);
-        HIPCHECK(hipMemcpyAsync(fields.data(), E->d_fields, fields.size() * sizeof(double),
+    const size_t npred = (size_t)ns * std::max(nobs, 1);
+    std::vector<DevResult> res;
+    std::vector<double> pred, fields;
+    const DevResult *resp = nullptr;
+    const double *predp = nullptr, *fieldsp = nullptr;
+    if (E->graph_on && nblk == 1) {
+        // graph replay: this call's slots go through the pinned staging buffer
+        // the captured upload reads; results come back through pinned buffers
+        grow_pinned(E->h_slots, E->hcap_slots, ns);
+        grow_pinned(E->h_results, E->hcap_results, ns);
+        grow_pinned(E->h_pred, E->hcap_pred, npred);
+        if (out->fields) grow_pinned(E->h_fields, E->hcap_fields, (size_t)ns * fstride);
+        std::memcpy(E->h_slots, slots.data(), ns * sizeof(DevSlot));
+        rq_engine::GraphKey k;
+        k.ns = ns;
+        k.block = E->block;
+        k.nobs = nobs;
+        k.lds = E->lds;
+        k.fields = out->fields != nullptr;
+        k.lds_bytes = E->lds_bytes;
+        std::memcpy(&k.set, &E->settings, sizeof(DevSettings));
+        const void *ptrs[8] = {E->d_slots, E->d_results, E->d_pred, E->d_fields,
+                               E->h_slots,  E->h_results, E->h_pred, E->h_fields};
+        for (int i = 0; i < 8; ++i) k.ptr[i] = ptrs[i];
+        if (!E->gexec || !(k == E->gkey)) {
+            E->drop_graph();
+            HIPCHECK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipMemcpyAsync(E->d_slots, E->h_slots, ns * sizeof(DevSlot),
+                                          hipMemcpyHostToDevice, E->stream);
+            if (e == hipSuccess)
+                e = launch_simple_solve(E->block, E->lds, E->d_slots, ns, E->settings,
+                                        E->d_results, E->d_pred, std::max(nobs, 1),
+                                        out->fields ? E->d_fields : nullptr, fstride,
+                                        E->lds_bytes, E->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(E->h_results, E->d_results, ns * sizeof(DevResult),
+                                   hipMemcpyDeviceToHost, E->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(E->h_pred, E->d_pred, npred * sizeof(double),
+                                   hipMemcpyDeviceToHost, E->stream);
+            if (e == hipSuccess && out->fields)
+                e = hipMemcpyAsync(E->h_fields, E->d_fields, (size_t)ns * fstride * sizeof(double),
+                                   hipMemcpyDeviceToHost, E->stream);
+            // the capture always ends, also after a failed step
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(E->stream, &g);
+            E->graph = g;
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess) e = hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                E->drop_graph();
+                throw Fail{RQ_EDEVICE, std::string("graph capture of the batch launch: ") +
+                                           hipGetErrorString(e)};
+            }
+            E->gkey = k;
+        }
+        HIPCHECK(hipEventRecord(E->ev0, E->stream));
+        HIPCHECK(hipGraphLaunch(E->gexec, E->stream));
+        HIPCHECK(hipEventRecord(E->ev1, E->stream));
+        HIPCHECK(hipStreamSynchronize(E->stream));
+        E->graph_launches++;
+        resp = E->h_results;
+        predp = E->h_pred;
+        fieldsp = E->h_fields;
+    } else {
+        HIPCHECK(hipMemcpyAsync(E->d_slots, slots.data(), ns * sizeof(DevSlot),
+                                hipMemcpyHostToDevice, E->stream));
+        if (nblk > 1) {
+            grow(E->d_grid, E->cap_grid, ns);
+            HIPCHECK(hipMemsetAsync(E->d_grid, 0, ns * sizeof(GridState), E->stream));
+        }
+        HIPCHECK(hipEventRecord(E->ev0, E->stream));
+        if (nblk > 1)
+            HIPCHECK(launch_grid_solve(E->d_slots, ns, nblk, E->settings, E->d_results,
+                                       E->d_pred, std::max(nobs, 1),
+                                       out->fields ? E->d_fields : nullptr, fstride, E->d_grid,
+                                       grid_lds, E->stream));
+        else
+            HIPCHECK(launch_simple_solve(E->block, E->lds, E->d_slots, ns, E->settings,
+                                         E->d_results, E->d_pred, std::max(nobs, 1),
+                                         out->fields ? E->d_fields : nullptr, fstride,
+                                         E->lds_bytes, E->stream));
+        HIPCHECK(hipEventRecord(E->ev1, E->stream));
+        res.resize(ns);
+        pred.resize(npred);
+        HIPCHECK(hipMemcpyAsync(res.data(), E->d_results, ns * sizeof(DevResult),
                                 hipMemcpyDeviceToHost, E->stream));
+        HIPCHECK(hipMemcpyAsync(pred.data(), E->d_pred, pred.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, E->stream));
+        if (out->fields) {
+            fields.resize((size_t)ns * fstride);
+            HIPCHECK(hipMemcpyAsync(fields.data(), E->d_fields, fields.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, E->stream));
+        }
+        HIPCHECK(hipStreamSynchronize(E->stream));
+        resp = res.data();
+        predp = pred.data();
+        fieldsp = fields.data();
     }
-    HIPCHECK(hipStreamSynchronize(E->stream));
     float ms = 0.f;
     HIPCHECK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
     E->last_ms = ms;
     for (int i = 0; i < ns; ++i) {
         const int64_t r = valid[i];
-        const DevResult &d = res[i];
-        const RowFinal fin = finalize_row(E, d, pred.data() + (size_t)i * std::max(nobs, 1));
+        const DevResult &d = resp[i];
+        const RowFinal fin = finalize_row(E, d, predp + (size_t)i * std::max(nobs, 1));
         const int status = fin.status;
         const std::string &msg = fin.msg;
         if (fin.keep_predictions && out->predictions)
             for (int j = 0; j < nobs; ++j)
-                out->predictions[r * nobs + j] = pred[(size_t)i * nobs + j];
+                out->predictions[r * nobs + j] = predp[(size_t)i * nobs + j];
         out->status[r] = status;
         if (out->loglik) out->loglik[r] = fin.loglik;
         if (out->iterations) out->iterations[r] = d.iters;
@@ -1110,7 +1234,7 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
             for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = d.cycles[e];
         if (!msg.empty()) write_row_error(out, r, msg);
         if (out->fields)
-            std::memcpy(out->fields + r * fstride, fields.data() + (size_t)i * fstride,
+            std::memcpy(out->fields + r * fstride, fieldsp + (size_t)i * fstride,
                         fstride * sizeof(double));
     }
     return RQ_OK;
@@ -1720,6 +1844,23 @@ int rq_engine_autotune(rq_engine *engine, const double *theta, int64_t n_rows, i
         return autotune_impl(engine, theta, n_rows, n_params, iterations, out);
     });
 }
+
+int rq_engine_set_graph_replay(rq_engine *e, int32_t enable, char *err, size_t errcap) {
+    try {
+        if (!e) throw Fail{RQ_EINVAL, "null engine"};
+        if (enable != 0 && enable != 1) throw Fail{RQ_EINVAL, "enable must be 0 or 1"};
+        HIPCHECK(hipSetDevice(e->device));
+        e->graph_on = enable != 0;
+        if (!e->graph_on) e->drop_graph();
+        set_err(err, errcap, "");
+        return RQ_OK;
+    } catch (const Fail &f) {
+        set_err(err, errcap, f.msg);
+        return f.code;
+    }
+}
+
+int64_t rq_engine_graph_launches(const rq_engine *e) { return e ? e->graph_launches : -1; }
 
 int32_t rq_engine_n_observations(const rq_engine *e) { return e ? e->nobs : -1; }
 int32_t rq_engine_n_parameters(const rq_engine *e) { return e ? e->n_active : -1; }

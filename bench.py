@@ -19,7 +19,8 @@ run in one persistent-kernel launch. The line printed is
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 
 The sweep runs on the launch shape the autotuner picks (--no-autotune keeps the
-default). Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
+default), and every step replays the HIP graph captured on the first
+evaluate (--no-graph launches directly). Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
 and FP64 defect correction), served (Poisson arrivals of single requests on
 64-512 cell channels against the plain one-launch-per-request path), hills
 (the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
@@ -452,6 +453,8 @@ def main():
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the engine's default launch shape for the sweep")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the sweep directly instead of replaying a captured HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -498,6 +501,10 @@ def main():
         print(f"# autotune: block {tune['block_size']}, {tune['grid_blocks']} workgroup(s) per "
               f"draw", file=sys.stderr, flush=True)
 
+    if not args.no_graph:
+        # the sweep's launch is captured on its first evaluate (a warmup step)
+        # and replayed by every later one
+        batch.set_graph_replay(True)
     for w in range(args.warmup):
         batch.evaluate(theta)
         print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -556,6 +563,8 @@ def main():
         print(f"# mixed precision: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
     plain = None
+    graph_launches = batch.graph_launches()
+    batch.set_graph_replay(False)  # the plain path launches each solve directly
     if not args.no_plain and args.plain_sample > 0:
         lat = []
         conv_plain = 0
@@ -636,6 +645,7 @@ def main():
         "hills": hills,
         "block_size": batch.last_timing()["block_size"],
         "autotune": tune,
+        "graph_replay": {"enabled": not args.no_graph, "graph_launches": graph_launches},
         "arena_in_lds": batch.last_timing()["arena_in_lds"],
     }
     print(json.dumps(line), flush=True)

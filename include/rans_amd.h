@@ -167,6 +167,17 @@ typedef struct rq_tune_out {
 int rq_engine_autotune(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
                        int32_t iterations, rq_tune_out *out, char *err, size_t errcap);
 
+/* No counterpart in the reference: HIP graph replay of the batch launch.
+ * With enable = 1, an evaluation whose draws run one workgroup each captures
+ * its launch (slot upload, solve kernel, result download through pinned host
+ * buffers) into a HIP graph the first time and replays that graph while the
+ * row count, launch shape, settings and buffers stay the same; draws spread
+ * over several workgroups (cooperative launches) are launched directly.
+ * kernel_ms of rq_engine_last_timing then covers the whole replayed graph. */
+int rq_engine_set_graph_replay(rq_engine *engine, int32_t enable, char *err, size_t errcap);
+/* Graph launches so far (each capture's first launch included). */
+int64_t rq_engine_graph_launches(const rq_engine *engine);
+
 /* Library identity: compiler, target and build flags. */
 const char *rq_build_info(void);
 
