@@ -177,17 +177,33 @@ def copy_calibration(mib):
     return 0
 
 
-def served_leg(n_requests, rate):
+HILL_MESHES = ((60, 32), (120, 64), (240, 128))
+
+
+def served_leg(n_requests, rate, hill_cap=0):
     """The served stream: seeded Poisson arrivals of single-solve requests on
     1D channels of 64/128/256/512 cells (one first-cell height), submitted at
     their arrival times to the device request loop; latency = completion -
     arrival. The plain path serves the same arrivals one launch at a time, in
-    order, with each mesh's solve time measured alone on the same GPU."""
+    order, with each mesh's solve time measured alone on the same GPU. With
+    hill_cap > 0 the meshes also include the 60x32, 120x64 and 240x128 hills,
+    whose solves are capped at hill_cap outer iterations."""
     from cfdr_amd import cases, engine
     from cfdr_amd.serve import Server
-    stream = cases.served_stream(n_requests, rate)
+    keys = list(cases.SERVED_CHANNELS)
     probs = {n: cases.channel_problem(ny=n, ratio=cases.channel_ratio(n))
              for n in cases.SERVED_CHANNELS}
+    if hill_cap > 0:
+        for nx, ny in HILL_MESHES:
+            key = f"hill{nx}x{ny}"
+            keys.append(key)
+            probs[key] = cases.hill_problem(nx=nx, ny=ny, max_iterations=hill_cap)
+    stream = cases.served_stream(n_requests, rate, meshes=tuple(keys))
+
+    def row(key, th):
+        # hill problems infer (a1, beta*, sigma_k1, sigma_w1): the stream's
+        # (a1, beta*) with sigma_k1, sigma_w1 at the centre of the hill draws
+        return np.asarray([th[0], th[1], 0.85, 0.5]) if isinstance(key, str) else th
     done = {}
     with Server(max_slots=256, idle_exit_seconds=5.0) as srv:
         pid = {n: srv.add_problem(p.mesh, p.params, p.obs, p.bcs, p.nu, p.settings, p.u_init,
@@ -207,7 +223,7 @@ def served_leg(n_requests, rate):
             while time.perf_counter() - t0 < ta:
                 collect()
                 time.sleep(2e-4)
-            tickets.append((ta, srv.submit(pid[n], th), n))
+            tickets.append((ta, srv.submit(pid[n], row(n, th)), n))
         print(f"# served: {len(tickets)} requests submitted", file=sys.stderr, flush=True)
         deadline = time.perf_counter() + 600
         while len(done) < len(tickets) and time.perf_counter() < deadline:
@@ -226,7 +242,7 @@ def served_leg(n_requests, rate):
                                 p.p_init, p.k_init, p.omega_init)
         first = next(th for ta, m, th in stream if m == n)
         t1 = time.perf_counter()
-        b.evaluate(np.asarray([first]))
+        b.evaluate(np.asarray([row(n, first)]))
         svc[n] = time.perf_counter() - t1
     end = 0.0
     plat = []
@@ -245,8 +261,11 @@ def served_leg(n_requests, rate):
             "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 1),
                            "p99": round(1e3 * float(np.percentile(lat, 99)), 1)},
             "slots": stats["slots"],
-            "meshes": "1D channels of 64/128/256/512 cells, uniform; theta uniform a1 in "
-                      "[0.28, 0.34], beta* in [0.084, 0.096]; seed 4242",
+            "meshes": "1D channels of 64/128/256/512 cells" +
+                      (f" and 60x32/120x64/240x128 hills (capped at {hill_cap} outer "
+                       "iterations)" if hill_cap > 0 else "") +
+                      ", uniform; theta uniform a1 in [0.28, 0.34], beta* in [0.084, 0.096] "
+                      "(hills: sigma_k1, sigma_w1 at the hill defaults); seed 4242",
             "plain": {"value": round(len(stream) / pspan, 4), "unit": "completed solves/s",
                       "latency_ms": {"p50": round(1e3 * float(np.percentile(plat, 50)), 1),
                                      "p99": round(1e3 * float(np.percentile(plat, 99)), 1)},
@@ -461,6 +480,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=16.0)
+    ap.add_argument("--serve-hills", type=int, default=0, metavar="CAP",
+                    help="add the 60x32/120x64/240x128 hills to the served stream, their solves "
+                         "capped at CAP outer iterations")
     ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-hills", action="store_true")
@@ -595,7 +617,7 @@ def main():
 
     served = None
     if not args.no_serve:
-        served = served_leg(args.serve_requests, args.serve_rate)
+        served = served_leg(args.serve_requests, args.serve_rate, args.serve_hills)
 
     warm = None
     if not args.no_warm:

@@ -100,3 +100,30 @@ def test_invalid_rows_settle_on_host_and_idle_relaunch():
             s.submit(pid, [0.31])
         with pytest.raises(ValueError, match="before the first submit"):
             _add(s, prob)
+
+
+def test_served_hill_in_a_channel_mix_matches_batch_path():
+    """A 60x32 periodic hill (AMG pressure, arena and multigrid workspace in
+    HBM; solves capped at 120 outer iterations) served among channel
+    requests: each hill row equals the one-workgroup batch path's row."""
+    hill = cases.hill_problem(nx=60, ny=32, max_iterations=120)
+    small = small_channel()
+    th_h = cases.hill_draws(3)
+    th_c = cases.channel_draws(3)
+    batch = engine.ForwardBatch(hill.mesh, hill.params, hill.obs, hill.bcs, hill.nu,
+                                hill.settings, hill.u_init, hill.p_init, hill.k_init,
+                                hill.omega_init, block_size=256, grid_blocks=1)
+    ref = batch.evaluate(th_h)
+    with Server(max_slots=16, idle_exit_seconds=1.0) as s:
+        ph, pc = _add(s, hill), _add(s, small)
+        tickets = []
+        for i in range(3):
+            tickets.append(("h", i, s.submit(ph, th_h[i])))
+            tickets.append(("c", i, s.submit(pc, th_c[i])))
+        rows = [(kind, i, s.wait(t)) for kind, i, t in tickets]
+    for kind, i, r in rows:
+        if kind != "h":
+            continue
+        assert r.status == ref.statuses[i] == engine.EvaluationStatus.Unconverged
+        assert r.simple_iters == ref.iteration_counts[i] == 120
+        assert max_rel(r.predictions, ref.predictions[i]) <= REL_OBS
