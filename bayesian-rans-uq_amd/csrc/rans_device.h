@@ -178,6 +178,7 @@ struct DevResult {
 
 // ---------------------------------------------------------------- serving
 constexpr int kServeBlock = 256;      // threads per serving workgroup
+constexpr int kServeBlockLarge = 512; // ... when a registered mesh has more than 256 cells
 constexpr int kMaxServeObs = 64;      // observations per served problem
 
 // A registered problem: its mesh tables, settings and arena placement.

@@ -24,7 +24,7 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
                                const DevSettings &s, DevResult *results, double *pred,
                                int pred_stride, double *fields, int64_t field_stride,
                                size_t lds_bytes, hipStream_t stream);
-hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
+hipError_t launch_serve(int block, int grid, const DevProblem *problems, ServeControl *ctl,
                         const ServeRequest *ring, ServeOutput *outs, int ring_size,
                         unsigned long long *head, double *arenas, int64_t arena_stride,
                         double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
@@ -1668,6 +1668,7 @@ struct rq_server {
     double *d_amg = nullptr;
     int64_t amg_stride = 0;             // doubles per slot multigrid workspace
     int slots = 0, max_slots = 0;
+    int block = kServeBlock;            // threads per serving workgroup
     int64_t budget = 0;
     size_t lds_bytes = 0;
     long long spin_limit = 0;
@@ -1724,6 +1725,12 @@ void server_prepare(rq_server *S) {
     std::vector<DevProblem> table(S->problems.size());
     int64_t stride = 2, amgw = 0;
     size_t lds = 0;
+    // 512 threads when a mesh has more than 256 cells (the batch path's
+    // default for such meshes); smaller meshes then run the block Krylov
+    // path instead of the one-wave path
+    S->block = kServeBlock;
+    for (const rq_engine *E : S->problems)
+        if (E->nc > 256) S->block = kServeBlockLarge;
     for (size_t i = 0; i < S->problems.size(); ++i) {
         const rq_engine *E = S->problems[i];
         const size_t bytes = (size_t)E->lay.stride * sizeof(double);
@@ -1755,7 +1762,7 @@ void server_prepare(rq_server *S) {
 void server_ensure_running(rq_server *S) {
     if (server_running(S)) return;
     HIPCHECK(hipSetDevice(S->device));
-    HIPCHECK(launch_serve(S->slots, S->d_problems, S->ctl, S->ring, S->outs, S->ring_size,
+    HIPCHECK(launch_serve(S->block, S->slots, S->d_problems, S->ctl, S->ring, S->outs, S->ring_size,
                           S->d_head, S->d_arenas, S->arena_stride, S->d_amg, S->amg_stride,
                           S->spin_limit, S->lds_bytes, S->stream));
 }

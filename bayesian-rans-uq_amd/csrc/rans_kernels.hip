@@ -21,8 +21,9 @@
 #include "rans_device.h"
 
 // The file is compiled twice: part 0 holds the FP64 batch, multi-workgroup
-// and serving kernels with every launch entry point, part 1 the
-// mixed-precision instantiations of the batch kernels.
+// and 256-thread serving kernels with every launch entry point, part 1 the
+// mixed-precision instantiations of the batch kernels and the 512-thread
+// serving kernel.
 #ifndef RQ_KERNEL_PART
 #define RQ_KERNEL_PART 0
 #endif
@@ -3255,10 +3256,35 @@ static hipError_t launch_grid_t(const DevSlot *slots, int nslots, int nblk, cons
                                       dim3(kGridBlock), args, (unsigned)lds_bytes, stream);
 }
 
+template <int BS>
+static hipError_t launch_serve_t(int grid, const DevProblem *problems, ServeControl *ctl,
+                                 const ServeRequest *ring, ServeOutput *outs, int ring_size,
+                                 unsigned long long *head, double *arenas, int64_t arena_stride,
+                                 double *amgs, int64_t amg_stride, long long spin_limit,
+                                 size_t lds_bytes, hipStream_t stream) {
+    auto kern = serve_kernel<BS>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(BS), lds_bytes, stream, problems, ctl, ring, outs,
+                       ring_size, head, arenas, arena_stride, amgs, amg_stride, spin_limit);
+    return hipGetLastError();
+}
+
 // Mixed precision runs in kernels of its own, compiled in a translation unit
 // of its own (RQ_KERNEL_PART 1), so the FP32 block Krylov code neither shares
-// register allocation with the FP64 kernels nor lengthens their build.
+// register allocation with the FP64 kernels nor lengthens their build; the
+// 512-thread serving kernel is built there too, to balance the two parts.
 #if RQ_KERNEL_PART == 1
+hipError_t launch_serve_large(int grid, const DevProblem *problems, ServeControl *ctl,
+                              const ServeRequest *ring, ServeOutput *outs, int ring_size,
+                              unsigned long long *head, double *arenas, int64_t arena_stride,
+                              double *amgs, int64_t amg_stride, long long spin_limit,
+                              size_t lds_bytes, hipStream_t stream) {
+    return launch_serve_t<kServeBlockLarge>(grid, problems, ctl, ring, outs, ring_size, head,
+                                            arenas, arena_stride, amgs, amg_stride, spin_limit,
+                                            lds_bytes, stream);
+}
 hipError_t launch_simple_solve_f32(int block, bool lds, const DevSlot *slots, int nslots,
                                    const DevSettings &s, DevResult *results, double *pred,
                                    int pred_stride, double *fields, int64_t field_stride,
@@ -3275,6 +3301,11 @@ hipError_t launch_grid_solve_f32(const DevSlot *slots, int nslots, int nblk,
                                field_stride, states, lds_bytes, stream);
 }
 #else
+hipError_t launch_serve_large(int grid, const DevProblem *problems, ServeControl *ctl,
+                              const ServeRequest *ring, ServeOutput *outs, int ring_size,
+                              unsigned long long *head, double *arenas, int64_t arena_stride,
+                              double *amgs, int64_t amg_stride, long long spin_limit,
+                              size_t lds_bytes, hipStream_t stream);
 hipError_t launch_simple_solve_f32(int block, bool lds, const DevSlot *slots, int nslots,
                                    const DevSettings &s, DevResult *results, double *pred,
                                    int pred_stride, double *fields, int64_t field_stride,
@@ -3311,18 +3342,18 @@ hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const D
                                 field_stride, states, lds_bytes, stream);
 }
 
-hipError_t launch_serve(int grid, const DevProblem *problems, ServeControl *ctl,
+hipError_t launch_serve(int block, int grid, const DevProblem *problems, ServeControl *ctl,
                         const ServeRequest *ring, ServeOutput *outs, int ring_size,
                         unsigned long long *head, double *arenas, int64_t arena_stride,
                         double *amgs, int64_t amg_stride, long long spin_limit, size_t lds_bytes,
                         hipStream_t stream) {
-    auto kern = serve_kernel<kServeBlock>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kServeBlock), lds_bytes, stream, problems, ctl, ring,
-                       outs, ring_size, head, arenas, arena_stride, amgs, amg_stride, spin_limit);
-    return hipGetLastError();
+    if (block == kServeBlockLarge)
+        return launch_serve_large(grid, problems, ctl, ring, outs, ring_size, head, arenas,
+                                  arena_stride, amgs, amg_stride, spin_limit, lds_bytes, stream);
+    if (block != kServeBlock) return hipErrorInvalidValue;
+    return launch_serve_t<kServeBlock>(grid, problems, ctl, ring, outs, ring_size, head, arenas,
+                                       arena_stride, amgs, amg_stride, spin_limit, lds_bytes,
+                                       stream);
 }
 #endif
 
