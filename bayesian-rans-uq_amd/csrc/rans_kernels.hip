@@ -1058,20 +1058,18 @@ struct Draw {
         }
         red.sum(a0);
         double rz = a0[0];
+        int pc = pv, pn = V_kph;  // p, and the buffer the next p is written to
+        double beta = 0.0;
         for (int it = 0; it < maxIter; ++it) {
             double a1[1] = {0.0};
-            for (int ci = gt; ci < nc; ci += gs) {
-                const double av = spmv(ci, pv);
-                C(ci, Ap) = av;
-                a1[0] += C(ci, pv) * av;
-            }
+            pcg_product(it, beta, pc, pn, z, a1);
             red.sum(a1);
             const double pAp = a1[0];
             if (fabs(pAp) < 1e-30) break;
             const double alpha = rz / pAp;
             double a2[2] = {0.0, 0.0};
             for (int ci = gt; ci < nc; ci += gs) {
-                C(ci, V_kx) += alpha * C(ci, pv);
+                C(ci, V_kx) += alpha * C(ci, pc);
                 const double rv = C(ci, V_kr) - alpha * C(ci, Ap);
                 C(ci, V_kr) = rv;
                 const double zv = jac(rv, C(ci, V_diag));
@@ -1087,9 +1085,7 @@ struct Draw {
                 return res;
             }
             const double rzNew = a2[1];
-            const double beta = rzNew / (rz + 1e-30);
-            for (int ci = gt; ci < nc; ci += gs) C(ci, pv) = C(ci, z) + beta * C(ci, pv);
-            sync();
+            beta = rzNew / (rz + 1e-30);
             rz = rzNew;
         }
         return res;
@@ -1701,6 +1697,9 @@ struct Draw {
             g ? sync() : bsync();
             for (int i = i0; i < L.n; i += is) r[i] = b[i] - coarse_row(L, i, x);
         }
+        // the residual is a pass of its own: formed inside the restriction,
+        // each thread would run several members' products one after another
+        // (measured slower than the extra barrier)
         g ? sync() : bsync();
         const WPtr rf = wp(L.r);
         amg_restrict(l, g, [&](int m) { return l == 0 ? double(C(m, V_ks)) : double(rf[m]); });
@@ -1775,8 +1774,9 @@ struct Draw {
         }
     }
 
-    // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks (FP32:
-    // the float vectors F_R, F_Z and F_S; the coarser levels stay FP64)
+    // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks, the
+    // second sweep's buffer V_kt (FP32: the float vectors F_R, F_Z, F_S and
+    // F_T2; the coarser levels stay FP64)
     template <bool F32>
     DEV void amg_down_any(int l, bool g) {
         if constexpr (F32) {
@@ -1856,20 +1856,20 @@ struct Draw {
         }
         red.sum(a0);
         double rz = a0[0];
+        // p, and the buffer the next p is written to (the V-cycle uses krh, ks
+        // and kt on level 0)
+        int pc = V_kp, pn = V_kph;
+        double beta = 0.0;
         for (int it = 0; it < maxIter; ++it) {
             double a1[1] = {0.0};
-            for (int ci = gt; ci < nc; ci += gs) {
-                const double av = spmv(ci, V_kp);
-                C(ci, V_kv) = av;
-                a1[0] += C(ci, V_kp) * av;
-            }
+            pcg_product(it, beta, pc, pn, V_krh, a1);
             red.sum(a1);
             const double pAp = a1[0];
             if (fabs(pAp) < 1e-30) break;
             const double alpha = rz / pAp;
             double a2[1] = {0.0};
             for (int ci = gt; ci < nc; ci += gs) {
-                C(ci, V_kx) += alpha * C(ci, V_kp);
+                C(ci, V_kx) += alpha * C(ci, pc);
                 const double rv = C(ci, V_kr) - alpha * C(ci, V_kv);
                 C(ci, V_kr) = rv;
                 a2[0] += rv * rv;
@@ -1886,12 +1886,38 @@ struct Draw {
             for (int ci = gt; ci < nc; ci += gs) a3[0] += C(ci, V_kr) * C(ci, V_krh);
             red.sum(a3);
             const double rzNew = a3[0];
-            const double beta = rzNew / (rz + 1e-30);
-            for (int ci = gt; ci < nc; ci += gs) C(ci, V_kp) = C(ci, V_krh) + beta * C(ci, V_kp);
-            sync();
+            beta = rzNew / (rz + 1e-30);
             rz = rzNew;
         }
         return res;
+    }
+
+    // Ap of a CG iteration into kv, p.Ap into a[0]. From the second
+    // iteration on, p = z + beta p_old is formed where it is read (own row
+    // and neighbours, the same expression) and written to the other buffer,
+    // so the update needs no pass and no barrier of its own; the buffers
+    // then swap. Every value has the bits of a stored update.
+    DEV void pcg_product(int it, double beta, int &pc, int &pn, int z, double (&a)[1]) {
+        if (it == 0) {
+            for (int ci = gt; ci < nc; ci += gs) {
+                const double av = spmv(ci, pc);
+                C(ci, V_kv) = av;
+                a[0] += C(ci, pc) * av;
+            }
+            return;
+        }
+        const int po = pc;
+        auto pnew = [&](int j) { return C(j, z) + beta * C(j, po); };
+        for (int ci = gt; ci < nc; ci += gs) {
+            const double pv = pnew(ci);
+            C(ci, pn) = pv;
+            const double av = spmv_f(ci, pnew);
+            C(ci, V_kv) = av;
+            a[0] += pv * av;
+        }
+        const int written = pn;
+        pn = pc;
+        pc = written;
     }
 
     // -------------------------------------- mixed precision (block)
