@@ -60,7 +60,8 @@ class RqSettings(C.Structure):
                 ("omega_min", C.c_double), ("pressure_solver", C.c_int32),
                 ("momentum_solver", C.c_int32), ("turbulence_solver", C.c_int32),
                 ("sst_variant", C.c_int32), ("mixed_precision", C.c_int32),
-                ("block_size", C.c_int32), ("profile", C.c_int32), ("grid_blocks", C.c_int32)]
+                ("block_size", C.c_int32), ("profile", C.c_int32), ("grid_blocks", C.c_int32),
+                ("amg_block_rows", C.c_int32)]
 
 
 class RqParams(C.Structure):
@@ -86,8 +87,9 @@ class RqBatchOut(C.Structure):
 
 class RqTuneOut(C.Structure):
     _fields_ = [("n_candidates", C.c_int32), ("block_size", C.c_int32 * 16),
-                ("grid_blocks", C.c_int32 * 16), ("kernel_ms", C.c_double * 16),
-                ("chosen_block_size", C.c_int32), ("chosen_grid_blocks", C.c_int32)]
+                ("grid_blocks", C.c_int32 * 16), ("amg_block_rows", C.c_int32 * 16),
+                ("kernel_ms", C.c_double * 16), ("chosen_block_size", C.c_int32),
+                ("chosen_grid_blocks", C.c_int32), ("chosen_amg_block_rows", C.c_int32)]
 
 
 EXPORTS = ["rq_engine_autotune", "rq_engine_set_graph_replay", "rq_engine_graph_launches", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
@@ -164,7 +166,7 @@ class _Keep:
 
 
 def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size: int = 0,
-                    profile: bool = False, grid_blocks: int = 0):
+                    profile: bool = False, grid_blocks: int = 0, amg_block_rows: int = 0):
     r = RqSettings()
     r.max_iterations, r.convergence_tol, r.divergence_limit = s.max_iterations, \
         s.convergence_tol, s.divergence_limit
@@ -185,6 +187,7 @@ def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size
     r.block_size = block_size
     r.profile = 1 if profile else 0
     r.grid_blocks = grid_blocks
+    r.amg_block_rows = amg_block_rows
     return r
 
 
@@ -304,7 +307,7 @@ class Problem:
                  settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
                  mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
-                 grid_blocks: int = 0, owner: str = "ForwardBatch"):
+                 grid_blocks: int = 0, owner: str = "ForwardBatch", amg_block_rows: int = 0):
         settings = settings if settings is not None else SolverSettings()
         settings.validate()
         param_set.validate()
@@ -331,7 +334,8 @@ class Problem:
         p.base = keep.d(param_set.base)
         p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
         self.o = obs_struct(prepared, keep)
-        self.s = settings_struct(settings, mixed_precision, block_size, profile, grid_blocks)
+        self.s = settings_struct(settings, mixed_precision, block_size, profile, grid_blocks,
+                                 amg_block_rows)
         self.init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
         self.nu = float(nu)
 
@@ -409,10 +413,11 @@ class ForwardBatch:
                  settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
                  mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
-                 grid_blocks: int = 0):
+                 grid_blocks: int = 0, amg_block_rows: int = 0):
         self.lib = load_library()
         pr = Problem(mesh, param_set, obs_op, bcs, nu, settings, u_init, p_init, k_init,
-                     omega_init, mixed_precision, block_size, profile, grid_blocks, "ForwardBatch")
+                     omega_init, mixed_precision, block_size, profile, grid_blocks, "ForwardBatch",
+                     amg_block_rows)
         self.param_set = param_set
         self.n_obs = obs_op.n_obs()
         self.n_cells = mesh.n_cells
@@ -457,9 +462,11 @@ class ForwardBatch:
                                          err, 512)
         BatchBuffers.check(rc, err)
         cands = [{"block_size": out.block_size[i], "grid_blocks": out.grid_blocks[i],
-                  "kernel_ms": out.kernel_ms[i]} for i in range(out.n_candidates)]
+                  "amg_block_rows": out.amg_block_rows[i], "kernel_ms": out.kernel_ms[i]}
+                 for i in range(out.n_candidates)]
         return {"candidates": cands, "block_size": out.chosen_block_size,
-                "grid_blocks": out.chosen_grid_blocks}
+                "grid_blocks": out.chosen_grid_blocks,
+                "amg_block_rows": out.chosen_amg_block_rows}
 
     def set_graph_replay(self, enable: bool = True) -> None:
         """Capture the batch launch into a HIP graph once and replay it while

@@ -139,6 +139,7 @@ struct DevSettings {
     int mixed_precision;   // 1: FP32 Krylov with FP64 defect correction
     int profile;           // 1: accumulate per-phase clock counts in DevResult::cycles
     int grid_blocks;       // workgroups per draw (1: one-workgroup draws)
+    int amg_block_rows;    // multigrid levels of at most this many rows: one workgroup
 };
 
 // One draw to solve: its mesh, its arena and its coefficients.

@@ -206,6 +206,7 @@ void validate_settings(const rq_settings &s) {
     if (s.profile != 0 && s.profile != 1) throw Fail{RQ_EINVAL, "profile must be 0 or 1"};
     if (s.grid_blocks < 0 || s.grid_blocks > kMaxGridBlocks)
         throw Fail{RQ_EINVAL, "grid_blocks must be in [0, 64]"};
+    if (s.amg_block_rows < 0) throw Fail{RQ_EINVAL, "amg_block_rows must be nonnegative"};
     if (s.block_size != 0 && s.block_size != 64 && s.block_size != 128 && s.block_size != 256 &&
         s.block_size != 512)
         throw Fail{RQ_EINVAL, "block_size must be 0, 64, 128, 256 or 512"};
@@ -885,6 +886,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.mixed_precision = s.mixed_precision;
         S.profile = s.profile;
         S.grid_blocks = s.grid_blocks;
+        S.amg_block_rows = s.amg_block_rows > 0 ? s.amg_block_rows : 1024;
         HIPCHECK(hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device));
         E->block_user = s.block_size != 0;
         if (s.block_size) E->block = s.block_size;
@@ -1625,6 +1627,7 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         const int k = out->n_candidates++;
         out->block_size[k] = c.block;
         out->grid_blocks[k] = c.grid;
+        out->amg_block_rows[k] = E->settings.amg_block_rows;
         out->kernel_ms[k] = ms;
         if (std::isfinite(ms) && ms < best) {
             best = ms;
@@ -1641,8 +1644,36 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         E->block_user = saved_user;
         E->settings.grid_blocks = saved_grid;
     }
+    // multi-workgroup draws with a multigrid pressure solve: the threshold
+    // below which a level runs on one workgroup, at the chosen shape (it
+    // changes only which workgroups compute a row, not the row's arithmetic)
+    const int saved_rows = E->settings.amg_block_rows;
+    if (std::isfinite(best) && best_grid > 1 && E->mesh_host.amg.levels > 1) {
+        int best_rows = saved_rows;
+        for (int rows : {256, 512, 2048}) {
+            if (out->n_candidates >= nmax || rows == saved_rows) continue;
+            E->settings.amg_block_rows = rows;
+            double ms = std::numeric_limits<double>::quiet_NaN();
+            try {
+                evaluate_impl(E, theta, n_rows, n_params, 1, &o, nullptr, nullptr, opts.data());
+                if (E->last_nblk == best_grid) ms = E->last_ms;
+            } catch (const Fail &) {
+            }
+            const int k = out->n_candidates++;
+            out->block_size[k] = best_block;
+            out->grid_blocks[k] = best_grid;
+            out->amg_block_rows[k] = rows;
+            out->kernel_ms[k] = ms;
+            if (std::isfinite(ms) && ms < best) {
+                best = ms;
+                best_rows = rows;
+            }
+        }
+        E->settings.amg_block_rows = best_rows;
+    }
     out->chosen_block_size = E->block;
     out->chosen_grid_blocks = E->settings.grid_blocks;
+    out->chosen_amg_block_rows = E->settings.amg_block_rows;
     return RQ_OK;
 }
 

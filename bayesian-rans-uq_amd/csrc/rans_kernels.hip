@@ -1660,10 +1660,10 @@ struct Draw {
         __syncthreads();
     }
 
-    // Levels with more than kAmgBlockRows rows are worked by every workgroup
-    // of a multi-workgroup draw; smaller ones by rank 0 alone, where a
-    // workgroup barrier replaces a grid barrier.
-    static constexpr int kAmgBlockRows = 1024;
+    // Levels with more than S.amg_block_rows rows are worked by every
+    // workgroup of a multi-workgroup draw; smaller ones by rank 0 alone,
+    // where a workgroup barrier replaces a grid barrier. Either way each row
+    // is computed by the same arithmetic, so the results do not depend on it.
 
     // One level on the way down: two damped-Jacobi sweeps from zero, the
     // residual, and its restriction (aggregate sums) into the next level's
@@ -1814,7 +1814,7 @@ struct Draw {
         // levels [0, ls) are grid levels (kGrid), the rest rank 0's
         int ls = 1;
         if constexpr (kGrid)
-            while (ls < nl - 1 && G.lev[ls].n > kAmgBlockRows) ++ls;
+            while (ls < nl - 1 && G.lev[ls].n > S.amg_block_rows) ++ls;
         for (int l = 0; l < ls; ++l) amg_down_any<F32>(l, kGrid);
         if (!kGrid || rank == 0) {
             for (int l = ls; l < nl - 1; ++l) amg_down_any<F32>(l, false);

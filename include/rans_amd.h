@@ -78,6 +78,9 @@ typedef struct rq_settings {
                                 >= 4096 cells with no explicit block_size, which spread over
                                 the CUs the batch leaves idle, one per 256 cells at most);
                                 1 = one; 2..64 = that many co-resident workgroups */
+    int32_t amg_block_rows;  /* multi-workgroup draws: multigrid levels of at most this many
+                                rows run on one workgroup of the draw, larger ones on all of
+                                them (0 = 1024); the results do not depend on it */
 } rq_settings;
 
 /* InferenceParameterSet (core/include/InferenceParameters.hpp:18-188). */
@@ -156,13 +159,16 @@ void rq_engine_destroy(rq_engine *engine);
  * and, for arenas in HBM of at least 1024 cells, every workgroups-per-draw
  * count 2..64 that the device can hold runs the given rows with each solve
  * capped at `iterations` outer iterations; the fastest kernel time becomes
- * the engine's launch shape for later evaluations. kernel_ms is NaN for a
+ * the engine's launch shape for later evaluations. When the fastest shape
+ * spreads draws with an AMG pressure solve over several workgroups, the
+ * one-workgroup multigrid threshold (amg_block_rows 256, 512, 2048 against
+ * the configured one) is timed at that shape next. kernel_ms is NaN for a
  * shape that could not run. */
 typedef struct rq_tune_out {
     int32_t n_candidates;
-    int32_t block_size[16], grid_blocks[16];
+    int32_t block_size[16], grid_blocks[16], amg_block_rows[16];
     double kernel_ms[16];
-    int32_t chosen_block_size, chosen_grid_blocks;
+    int32_t chosen_block_size, chosen_grid_blocks, chosen_amg_block_rows;
 } rq_tune_out;
 int rq_engine_autotune(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
                        int32_t iterations, rq_tune_out *out, char *err, size_t errcap);

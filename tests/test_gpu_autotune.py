@@ -20,14 +20,27 @@ def _batch(prob, **kw):
                                prob.u_init, prob.p_init, prob.k_init, prob.omega_init, **kw)
 
 
-def _check_report(rep, blocks, grids):
+def _check_report(rep, blocks, grids, amg_rows=()):
     shapes = [(c["block_size"], c["grid_blocks"]) for c in rep["candidates"]]
-    assert shapes == [(b, 1) for b in blocks] + [(256, g) for g in grids]
+    n = len(blocks) + len(grids)
+    assert shapes[:n] == [(b, 1) for b in blocks] + [(256, g) for g in grids]
+    assert all(c["amg_block_rows"] == 1024 for c in rep["candidates"][:n])
     times = [c["kernel_ms"] for c in rep["candidates"]]
     assert all(t > 0 for t in times if not math.isnan(t))
-    finite = [(t, s) for t, s in zip(times, shapes) if not math.isnan(t)]
+    finite = [(t, s) for t, s in zip(times[:n], shapes[:n]) if not math.isnan(t)]
     assert finite, "no candidate ran"
-    assert (rep["block_size"], rep["grid_blocks"]) == min(finite)[1]
+    shape = min(finite)[1]
+    assert (rep["block_size"], rep["grid_blocks"]) == shape
+    # the multigrid threshold is timed at the chosen multi-workgroup shape
+    extra = rep["candidates"][n:]
+    if shape[1] > 1 and amg_rows:
+        assert [c["amg_block_rows"] for c in extra] == list(amg_rows)
+        assert all((c["block_size"], c["grid_blocks"]) == shape for c in extra)
+        every = [(c["kernel_ms"], c["amg_block_rows"]) for c in rep["candidates"]
+                 if (c["block_size"], c["grid_blocks"]) == shape and not math.isnan(c["kernel_ms"])]
+        assert rep["amg_block_rows"] == min(every)[1]
+    else:
+        assert extra == [] and rep["amg_block_rows"] == 1024
 
 
 def test_channel_tuned_rows_match_untuned():
@@ -54,7 +67,7 @@ def test_hill_grid_candidates_and_capped_fields_agree():
     tuned = _batch(prob)
     rep = tuned.autotune(theta, iterations=20)
     print("hill 120x64 candidates", rep, flush=True)
-    _check_report(rep, [64, 128, 256, 512], [2, 4, 8, 16, 32])
+    _check_report(rep, [64, 128, 256, 512], [2, 4, 8, 16, 32], (256, 512, 2048))
     a = tuned.evaluate(theta, fields=True)
     t = tuned.last_timing()
     assert t["grid_blocks"] == rep["grid_blocks"]
