@@ -89,6 +89,8 @@ def _raw_create(settings_mut=None, theta_mut=None):
     ("momentum_solver", 2, "pressure correction only"),
     ("block_size", 96, "block_size"),
     ("convergence_window", 300, "engine limit"),
+    ("amg_block_rows", -1, "amg_block_rows"),
+    ("grid_blocks", 65, "grid_blocks"),
 ])
 def test_create_rejects_invalid_settings(field, value, text):
     def mut(s):
@@ -210,3 +212,17 @@ def test_channel_problem_matches_the_single_solve_workload():
     assert p.settings.body_force.x == 1.0 and p.settings.alpha_u == 0.5
     assert math.isclose(p.nu, 1.0 / 182.088)
     assert (p.u_init.x, p.k_init, p.omega_init) == (15.0, 1.0, 10.0)
+
+
+def test_engine_extensions_reject_null_engines():
+    """The autotuner and graph-replay entry points fail with RQ_EINVAL on a
+    null engine, before any device call."""
+    lib = engine.load_library()
+    err = C.create_string_buffer(256)
+    theta = np.zeros(2)
+    out = engine.RqTuneOut()
+    assert lib.rq_engine_autotune(None, theta.ctypes.data_as(engine._dp), 1, 2, 10, C.byref(out),
+                                  err, 256) == 1
+    assert lib.rq_engine_set_graph_replay(None, 1, err, 256) == 1
+    assert "null engine" in err.value.decode()
+    assert lib.rq_engine_graph_launches(None) == -1
