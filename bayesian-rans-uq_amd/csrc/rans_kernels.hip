@@ -1557,23 +1557,60 @@ struct Draw {
         return which == 0 ? C(ci, V_diag) : off(ci, which - 1);
     }
 
+    // Multigrid workspace pointers. In a multi-workgroup draw, the levels
+    // from amg_ls() down are read and written by rank 0 alone (their
+    // Galerkin values too), so rank 0 reads them with plain loads, which may
+    // hit its L1: its own stores keep that L1 current. Everything else another
+    // workgroup writes is read with device-scope loads (WPtr).
+    template <bool kPlain>
+    using PPtr = typename std::conditional<kPlain, double *, WPtr>::type;
+    template <bool kPlain>
+    DEV PPtr<kPlain> pp(int64_t o) const {
+        if constexpr (kPlain) return amgw + o;
+        else return wp(o);
+    }
+    // first level run by rank 0 alone (levels [0, amg_ls()) are grid levels)
+    DEV int amg_ls() const {
+        const AmgMesh &G = M.amg;
+        int ls = 1;
+        if constexpr (kGrid)
+            while (ls < G.levels - 1 && G.lev[ls].n > S.amg_block_rows) ++ls;
+        return ls;
+    }
+
+    // Galerkin values and inverse diagonal of level l, rows e0, e0 + es, ...
+    template <bool kPlain>
+    DEV void refresh_level(int l, int e0, int es) {
+        const AmgMesh &G = M.amg;
+        const AmgLevel &F = G.lev[l - 1], &L = G.lev[l];
+        const PPtr<kPlain> val = pp<kPlain>(L.val), dinv = pp<kPlain>(L.dinv);
+        const WPtr fval = wp(F.val);
+        for (int e = e0; e < L.nnz; e += es) {
+            double a = 0.0;
+            for (int q = F.gptr[e]; q < F.gptr[e + 1]; ++q) {
+                const int fe = F.gidx[q];
+                a += l == 1 ? lev0_value(G.src0[fe]) : double(fval[fe]);
+            }
+            val[e] = a;
+        }
+        kPlain ? bsync() : sync();
+        for (int i = e0; i < L.n; i += es) {
+            const double d = val[L.diagpos[i]];
+            dinv[i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+        }
+    }
+
     DEV void amg_refresh() {
         const AmgMesh &G = M.amg;
+        const int ls = amg_ls();
         for (int l = 1; l < G.levels; ++l) {
-            const AmgLevel &F = G.lev[l - 1], &L = G.lev[l];
-            const WPtr val = wp(L.val), fval = wp(F.val), dinv = wp(L.dinv);
-            for (int e = gt; e < L.nnz; e += gs) {
-                double a = 0.0;
-                for (int q = F.gptr[e]; q < F.gptr[e + 1]; ++q) {
-                    const int fe = F.gidx[q];
-                    a += l == 1 ? lev0_value(G.src0[fe]) : double(fval[fe]);
-                }
-                val[e] = a;
-            }
-            sync();
-            for (int i = gt; i < L.n; i += gs) {
-                const double d = val[L.diagpos[i]];
-                dinv[i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+            if (kGrid && l >= ls) {
+                // the tail of the hierarchy: rank 0 alone
+                if (rank != 0) break;
+                refresh_level<kGrid>(l, tid, BS);
+                bsync();
+            } else {
+                refresh_level<false>(l, gt, gs);
             }
         }
         sync();
@@ -1622,10 +1659,10 @@ struct Draw {
     // y = A x on a coarse level (CSR); x(j) gives the vector's entries
     // (coarse rows are padded to a multiple of 4 entries, so a row is read
     // 4 entries at a time and the loads of a chunk are in flight together)
-    template <class F>
+    template <bool kPlain = false, class F>
     DEV double row_f(const AmgLevel &L, int i, F x) const {
         double a = 0.0;
-        const WPtr val = wp(L.val);
+        const PPtr<kPlain> val = pp<kPlain>(L.val);
         for (int q = L.rowptr[i]; q < L.rowptr[i + 1]; q += 4) {
             const v4i c = *reinterpret_cast<const v4i *>(L.col + q);
             const double v0 = val[q], v1 = val[q + 1], v2 = val[q + 2], v3 = val[q + 3];
@@ -1637,8 +1674,9 @@ struct Draw {
         }
         return a;
     }
-    DEV double coarse_row(const AmgLevel &L, int i, WPtr x) const {
-        return row_f(L, i, [&](int j) { return double(x[j]); });
+    template <bool kPlain = false>
+    DEV double coarse_row(const AmgLevel &L, int i, PPtr<kPlain> x) const {
+        return row_f<kPlain>(L, i, [&](int j) { return double(x[j]); });
     }
     // level-0 product with the vector given entry by entry
     template <class F>
@@ -1670,6 +1708,7 @@ struct Draw {
     // right-hand side. The first sweep's iterate w D^-1 b is formed where it
     // is read, so both sweeps take one pass; the arithmetic is the two
     // sweeps' own.
+    template <bool kPlain = false>
     DEV void amg_down(int l, bool g) {
         const AmgMesh &G = M.amg;
         const AmgLevel &L = G.lev[l];
@@ -1688,32 +1727,34 @@ struct Draw {
             g ? sync() : bsync();
             for (int ci = i0; ci < nc; ci += is) C(ci, V_ks) = C(ci, V_kr) - spmv(ci, V_krh);
         } else {
-            const WPtr x = wp(L.x), b = wp(L.b), r = wp(L.r), di = wp(L.dinv);
+            const PPtr<kPlain> x = pp<kPlain>(L.x), b = pp<kPlain>(L.b), r = pp<kPlain>(L.r),
+                               di = pp<kPlain>(L.dinv);
             auto x0 = [&](int j) { return w * b[j] * di[j]; };
             for (int i = i0; i < L.n; i += is) {
-                const double xi = x0(i), r0 = b[i] - row_f(L, i, x0);
+                const double xi = x0(i), r0 = b[i] - row_f<kPlain>(L, i, x0);
                 x[i] = xi + w * r0 * di[i];
             }
             g ? sync() : bsync();
-            for (int i = i0; i < L.n; i += is) r[i] = b[i] - coarse_row(L, i, x);
+            for (int i = i0; i < L.n; i += is) r[i] = b[i] - coarse_row<kPlain>(L, i, x);
         }
         // the residual is a pass of its own: formed inside the restriction,
         // each thread would run several members' products one after another
         // (measured slower than the extra barrier)
         g ? sync() : bsync();
-        const WPtr rf = wp(L.r);
-        amg_restrict(l, g, [&](int m) { return l == 0 ? double(C(m, V_ks)) : double(rf[m]); });
+        const PPtr<kPlain> rf = pp<kPlain>(L.r);
+        amg_restrict<kPlain>(l, g,
+                             [&](int m) { return l == 0 ? double(C(m, V_ks)) : double(rf[m]); });
     }
 
     // Restriction of level l's residual (res(m) of member m) into the next
     // level's right-hand side: aggregate sums in member order.
-    template <class F>
+    template <bool kPlain = false, class F>
     DEV void amg_restrict(int l, bool g, F res) {
         const AmgMesh &G = M.amg;
         const AmgLevel &L = G.lev[l];
         const int i0 = g ? gt : tid, is = g ? gs : BS;
         const AmgLevel &Cl = G.lev[l + 1];
-        const WPtr bc = wp(Cl.b);
+        const PPtr<kPlain> bc = pp<kPlain>(Cl.b);
         // member lists are padded to a multiple of 4 with -1
         for (int I = i0; I < Cl.n; I += is) {
             double a = 0.0;
@@ -1739,12 +1780,13 @@ struct Draw {
     // level's residual array, or V_kt on level 0), so each sweep is one
     // pass. The last pass of level 0 is not followed by a barrier: every
     // caller reads only its own rows next.
+    template <bool kPlain = false>
     DEV void amg_up(int l, bool g) {
         const AmgMesh &G = M.amg;
         const AmgLevel &L = G.lev[l];
         const double w = kJacobiOmega;
         const int i0 = g ? gt : tid, is = g ? gs : BS;
-        const WPtr xc = wp(G.lev[l + 1].x);
+        const PPtr<kPlain> xc = pp<kPlain>(G.lev[l + 1].x);
         if (l == 0) {
             auto xp = [&](int j) { return C(j, V_krh) + xc[L.agg[j]]; };
             for (int ci = i0; ci < nc; ci += is) {
@@ -1759,15 +1801,16 @@ struct Draw {
                 C(ci, V_krh) = fabs(d) > 1e-300 ? xi + w * r / d : xi;
             }
         } else {
-            const WPtr x = wp(L.x), b = wp(L.b), t = wp(L.r), di = wp(L.dinv);
+            const PPtr<kPlain> x = pp<kPlain>(L.x), b = pp<kPlain>(L.b), t = pp<kPlain>(L.r),
+                               di = pp<kPlain>(L.dinv);
             auto xp = [&](int j) { return x[j] + xc[L.agg[j]]; };
             for (int i = i0; i < L.n; i += is) {
-                const double xi = xp(i), r = b[i] - row_f(L, i, xp);
+                const double xi = xp(i), r = b[i] - row_f<kPlain>(L, i, xp);
                 t[i] = xi + w * r * di[i];
             }
             g ? sync() : bsync();
             for (int i = i0; i < L.n; i += is) {
-                const double xi = t[i], r = b[i] - coarse_row(L, i, t);
+                const double xi = t[i], r = b[i] - coarse_row<kPlain>(L, i, t);
                 x[i] = xi + w * r * di[i];
             }
             g ? sync() : bsync();
@@ -1777,19 +1820,19 @@ struct Draw {
     // z = M^-1 r: level-0 b is V_kr, x is V_krh, the residual V_ks, the
     // second sweep's buffer V_kt (FP32: the float vectors F_R, F_Z, F_S and
     // F_T2; the coarser levels stay FP64)
-    template <bool F32>
+    template <bool F32, bool kPlain = false>
     DEV void amg_down_any(int l, bool g) {
         if constexpr (F32) {
             if (l == 0) return amg_down0_32(g);
         }
-        amg_down(l, g);
+        amg_down<kPlain>(l, g);
     }
-    template <bool F32>
+    template <bool F32, bool kPlain = false>
     DEV void amg_up_any(int l, bool g) {
         if constexpr (F32) {
             if (l == 0) return amg_up0_32(g);
         }
-        amg_up(l, g);
+        amg_up<kPlain>(l, g);
     }
     template <bool F32 = false>
     DEV void amg_vcycle() {
@@ -1812,18 +1855,30 @@ struct Draw {
             return;
         }
         // levels [0, ls) are grid levels (kGrid), the rest rank 0's
-        int ls = 1;
-        if constexpr (kGrid)
-            while (ls < nl - 1 && G.lev[ls].n > S.amg_block_rows) ++ls;
+        const int ls = amg_ls();
         for (int l = 0; l < ls; ++l) amg_down_any<F32>(l, kGrid);
         if (!kGrid || rank == 0) {
-            for (int l = ls; l < nl - 1; ++l) amg_down_any<F32>(l, false);
+            if constexpr (kGrid) {
+                // level ls's right-hand side was restricted by every workgroup:
+                // device-scope loads give its current values, and storing them
+                // again from this workgroup keeps its L1 current for the plain
+                // loads of rank 0's levels below
+                const AmgLevel &Lt = G.lev[ls];
+                double *bt = amgw + Lt.b;
+                for (int i = tid; i < Lt.n; i += BS) {
+                    const double v =
+                        __hip_atomic_load(bt + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(bt + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                bsync();
+            }
+            for (int l = ls; l < nl - 1; ++l) amg_down_any<F32, kGrid>(l, false);
             // coarsest: x = A^-1 b, with b staged in LDS; the inverse was
             // written by this workgroup, so its rows are read with plain loads
             const int n = G.coarse_n;
             const double *X = amgw + G.inv + (int64_t)n * n;
             const AmgLevel &L = G.lev[nl - 1];
-            const WPtr x = wp(L.x), b = wp(L.b);
+            const PPtr<kGrid> x = pp<kGrid>(L.x), b = pp<kGrid>(L.b);
             for (int i = tid; i < n; i += BS) cbuf[i] = b[i];
             __syncthreads();
             for (int i = tid; i < n; i += BS) {
@@ -1832,7 +1887,7 @@ struct Draw {
                 x[i] = a;
             }
             bsync();
-            for (int l = nl - 2; l >= ls; --l) amg_up_any<F32>(l, false);
+            for (int l = nl - 2; l >= ls; --l) amg_up_any<F32, kGrid>(l, false);
         }
         if constexpr (kGrid) sync();  // rank 0's levels are visible to every workgroup
         for (int l = ls - 1; l >= 0; --l) amg_up_any<F32>(l, kGrid);
