@@ -1583,6 +1583,10 @@ int guarded(char *err, size_t errcap, F &&f) {
 // ----------------------------------------------------------------------------
 namespace {
 
+// a later candidate replaces the best so far only when it is more than 1 %
+// faster, so shapes within timing noise of each other keep the earlier one
+constexpr double kTuneMargin = 0.99;
+
 int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
                   int32_t iterations, rq_tune_out *out) {
     if (!E || !out || (n_rows > 0 && !theta)) throw Fail{RQ_EINVAL, "null argument"};
@@ -1629,7 +1633,7 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         out->grid_blocks[k] = c.grid;
         out->amg_block_rows[k] = E->settings.amg_block_rows;
         out->kernel_ms[k] = ms;
-        if (std::isfinite(ms) && ms < best) {
+        if (std::isfinite(ms) && ms < kTuneMargin * best) {
             best = ms;
             best_block = c.block;
             best_grid = c.grid;
@@ -1664,7 +1668,7 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
             out->grid_blocks[k] = best_grid;
             out->amg_block_rows[k] = rows;
             out->kernel_ms[k] = ms;
-            if (std::isfinite(ms) && ms < best) {
+            if (std::isfinite(ms) && ms < kTuneMargin * best) {
                 best = ms;
                 best_rows = rows;
             }

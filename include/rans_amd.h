@@ -158,8 +158,9 @@ void rq_engine_destroy(rq_engine *engine);
  * candidate workgroup size (64, 128, 256, 512)
  * and, for arenas in HBM of at least 1024 cells, every workgroups-per-draw
  * count 2..64 that the device can hold runs the given rows with each solve
- * capped at `iterations` outer iterations; the fastest kernel time becomes
- * the engine's launch shape for later evaluations. When the fastest shape
+ * capped at `iterations` outer iterations, in the order listed; a candidate
+ * replaces the best so far when its kernel time is more than 1 % shorter, and
+ * the result becomes the engine's launch shape for later evaluations. When the fastest shape
  * spreads draws with an AMG pressure solve over several workgroups, the
  * one-workgroup multigrid threshold (amg_block_rows 256, 512, 2048 against
  * the configured one) is timed at that shape next. kernel_ms is NaN for a
