@@ -67,3 +67,14 @@ def test_multi_workgroup_draws_launch_directly():
     one.set_graph_replay(False)
     one.evaluate(th)
     assert one.graph_launches() == 1
+
+
+def test_mixed_precision_replays_match_direct_launches():
+    prob = small_channel()
+    th = cases.channel_draws(4)
+    g = _batch(prob, mixed_precision=True)
+    g.set_graph_replay(True)
+    direct = _batch(prob, mixed_precision=True)
+    for _ in range(2):
+        _same(g.evaluate(th), direct.evaluate(th))
+    assert g.graph_launches() == 2
