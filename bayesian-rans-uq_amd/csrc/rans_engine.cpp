@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -137,6 +138,9 @@ struct rq_engine {
     double *h_pred = nullptr, *h_fields = nullptr;
     size_t hcap_slots = 0, hcap_results = 0, hcap_pred = 0, hcap_fields = 0;
     int64_t graph_launches = 0;
+    // serialises the entry points that use the engine's buffers and stream,
+    // so evaluate may be called from several threads (one call at a time)
+    mutable std::recursive_mutex mu;
 
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1016,6 +1020,7 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
                   int32_t workers, rq_batch_out *out, const double *const *warm = nullptr,
                   double *const *save = nullptr, const RowOpts *opts = nullptr) {
     if (!E || !out || !out->status) throw Fail{RQ_EINVAL, "null argument"};
+    std::lock_guard<std::recursive_mutex> lock(E->mu);
     if (workers < 1) throw Fail{RQ_EINVAL, "workers must be positive"};
     if (n_params != E->n_active)
         throw Fail{RQ_EINVAL, "batch rows must all have n_parameters entries"};
@@ -1590,6 +1595,7 @@ constexpr double kTuneMargin = 0.99;
 int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
                   int32_t iterations, rq_tune_out *out) {
     if (!E || !out || (n_rows > 0 && !theta)) throw Fail{RQ_EINVAL, "null argument"};
+    std::lock_guard<std::recursive_mutex> lock(E->mu);
     if (n_rows <= 0) throw Fail{RQ_EINVAL, "the autotuner needs at least one coefficient row"};
     if (n_params != E->n_active)
         throw Fail{RQ_EINVAL, "batch rows must all have n_parameters entries"};
@@ -1891,6 +1897,7 @@ int rq_engine_set_graph_replay(rq_engine *e, int32_t enable, char *err, size_t e
     try {
         if (!e) throw Fail{RQ_EINVAL, "null engine"};
         if (enable != 0 && enable != 1) throw Fail{RQ_EINVAL, "enable must be 0 or 1"};
+        std::lock_guard<std::recursive_mutex> lock(e->mu);
         HIPCHECK(hipSetDevice(e->device));
         e->graph_on = enable != 0;
         if (!e->graph_on) e->drop_graph();
@@ -1912,6 +1919,7 @@ int32_t rq_engine_n_faces(const rq_engine *e) { return e ? e->nf : -1; }
 int rq_engine_last_timing(const rq_engine *e, double *kernel_ms, int32_t *block_size,
                           int32_t *arena_in_lds, int64_t *arena_bytes, int32_t *grid_blocks) {
     if (!e) return RQ_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     if (kernel_ms) *kernel_ms = e->last_ms;
     if (block_size) *block_size = e->last_nblk > 1 ? kGridBlock : e->block;
     if (arena_in_lds) *arena_in_lds = (e->lds && e->last_nblk == 1) ? 1 : 0;

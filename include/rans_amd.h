@@ -137,7 +137,9 @@ rq_engine *rq_engine_create(const rq_mesh_desc *mesh, const rq_param_desc *param
 
 /* ForwardBatch::evaluate: theta is n_rows x n_params (row-major). All rows
  * are solved on the GPU together; workers must be >= 1 and only keeps the
- * reference's argument contract. */
+ * reference's argument contract. Like the reference's const evaluate, it may
+ * be called from several threads on one engine: an internal lock runs the
+ * calls (and autotune, graph-replay and timing calls) one after another. */
 int rq_engine_evaluate(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
                        int32_t workers, rq_batch_out *out, char *err, size_t errcap);
 

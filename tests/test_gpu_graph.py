@@ -78,3 +78,26 @@ def test_mixed_precision_replays_match_direct_launches():
     for _ in range(2):
         _same(g.evaluate(th), direct.evaluate(th))
     assert g.graph_launches() == 2
+
+
+def test_concurrent_evaluates_on_one_engine():
+    """Several Python threads evaluate on one engine (ctypes releases the
+    GIL during the call): each result equals its own sequential evaluate."""
+    import threading
+    prob = small_channel()
+    b = _batch(prob)
+    draws = cases.channel_draws(8)
+    sets = [draws[0:2], draws[2:4], draws[4:6], draws[6:8]]
+    want = [b.evaluate(th) for th in sets]
+    got = [None] * len(sets)
+
+    def run(k):
+        got[k] = b.evaluate(sets[k])
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(len(sets))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    for k in range(len(sets)):
+        _same(got[k], want[k])
