@@ -92,7 +92,7 @@ class RqTuneOut(C.Structure):
                 ("chosen_grid_blocks", C.c_int32), ("chosen_amg_block_rows", C.c_int32)]
 
 
-EXPORTS = ["rq_engine_autotune", "rq_engine_set_graph_replay", "rq_engine_graph_launches", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
+EXPORTS = ["rq_engine_amg_levels", "rq_engine_autotune", "rq_engine_set_graph_replay", "rq_engine_graph_launches", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
            "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
            "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info",
            "rq_server_create", "rq_server_add_problem", "rq_server_submit", "rq_server_poll",
@@ -142,6 +142,8 @@ def load_library(path: str = LIB_PATH):
     lib.rq_engine_autotune.restype = C.c_int
     lib.rq_engine_set_graph_replay.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_size_t]
     lib.rq_engine_set_graph_replay.restype = C.c_int
+    lib.rq_engine_amg_levels.argtypes = [C.c_void_p, _ip, _i64p, C.c_int32]
+    lib.rq_engine_amg_levels.restype = C.c_int32
     lib.rq_engine_graph_launches.argtypes = [C.c_void_p]
     lib.rq_engine_graph_launches.restype = C.c_int64
     lib.rq_build_info.argtypes = []
@@ -474,6 +476,15 @@ class ForwardBatch:
         err = C.create_string_buffer(512)
         BatchBuffers.check(self.lib.rq_engine_set_graph_replay(self._h, 1 if enable else 0, err,
                                                                512), err)
+
+    def amg_levels(self) -> list:
+        """(rows, CSR entries) of each level of the pressure multigrid
+        hierarchy (rq_engine_amg_levels); empty without AMG."""
+        rows = np.zeros(16, np.int32)
+        nnz = np.zeros(16, np.int64)
+        n = self.lib.rq_engine_amg_levels(self._h, rows.ctypes.data_as(_ip),
+                                          nnz.ctypes.data_as(_i64p), 16)
+        return [(int(rows[l]), int(nnz[l])) for l in range(max(n, 0))]
 
     def graph_launches(self) -> int:
         return int(self.lib.rq_engine_graph_launches(self._h))

@@ -79,6 +79,43 @@ def pcg_iteration_bytes(N, F):
     return _spmv(N, F) + 3 * N * S_VAL + 5 * N * S_VAL + 9 * N * S_VAL
 
 
+def _csr_spmv(n, nnz):
+    """A coarse-level CSR product: values and columns, row pointers, x and y."""
+    return nnz * (S_VAL + S_IDX) + (n + 1) * S_IDX + 2 * n * S_VAL
+
+
+def amg_vcycle_bytes(N, F, levels):
+    """One V-cycle: two Jacobi sweeps down and two up per level (each a
+    product), the residual (a product and b), restriction and prolongation
+    (2 n values, n aggregate indices), and the dense coarsest solve. levels
+    is [(rows, CSR entries)] from level 0 (whose product is the LDU one)."""
+    b = 0
+    for lev, (n, nnz) in enumerate(levels[:-1]):
+        sp = _spmv(N, F) if lev == 0 else _csr_spmv(n, nnz)
+        b += 4 * sp + (sp + n * S_VAL) + 2 * n * S_VAL + n * S_IDX
+    nc = levels[-1][0]
+    return b + nc * nc * S_VAL + 2 * nc * S_VAL
+
+
+def amg_pcg_iteration_bytes(N, F, levels):
+    """One AMG-preconditioned CG iteration: product, dots and updates, one
+    V-cycle (a solve runs one V-cycle per iteration, the first before it)."""
+    return _spmv(N, F) + 5 * N * S_VAL + 9 * N * S_VAL + amg_vcycle_bytes(N, F, levels)
+
+
+def amg_setup_bytes(N, F, levels):
+    """Per p' solve: the Galerkin refresh of every coarse level (fine values
+    and gather indices in, coarse values and inverse diagonal out) and the
+    dense inverse of the coarsest level."""
+    b = 0
+    for lev in range(1, len(levels)):
+        n, nnz = levels[lev]
+        fine_vals = (N + 2 * F) if lev == 1 else levels[lev - 1][1]
+        b += fine_vals * (S_VAL + S_IDX) + nnz * S_VAL + n * S_VAL
+    nc = levels[-1][0]
+    return b + 2 * nc * nc * S_VAL
+
+
 def outer_iteration_bytes(N, F, B, turbulence):
     """Compulsory traffic of one SIMPLE outer iteration as the engine runs it
     (Krylov iterations excluded): each distinct array a stage reads, counted
@@ -102,8 +139,9 @@ def outer_iteration_bytes(N, F, B, turbulence):
     return b
 
 
-def solve_bytes(N, F, B, iters, krylov, turb_start, settings):
-    """Algorithmic bytes of one solve from its executed iteration counts."""
+def solve_bytes(N, F, B, iters, krylov, turb_start, settings, levels=None):
+    """Algorithmic bytes of one solve from its executed iteration counts
+    (levels: the AMG hierarchy when the pressure solver is AMG)."""
     turb_iters = 0
     if iters > turb_start:
         turb_iters = (iters - turb_start + settings.turb_update_interval - 1) \
@@ -112,6 +150,9 @@ def solve_bytes(N, F, B, iters, krylov, turb_start, settings):
     b = plain_iters * outer_iteration_bytes(N, F, B, False)
     b += turb_iters * outer_iteration_bytes(N, F, B, True)
     per = {"BiCGSTAB": bicgstab_iteration_bytes(N, F), "PCG": pcg_iteration_bytes(N, F)}
+    if levels:
+        per["AMG"] = amg_pcg_iteration_bytes(N, F, levels)
+        b += iters * amg_setup_bytes(N, F, levels)  # one p' solve per outer iteration
     solvers = [settings.momentum_solver, settings.momentum_solver, settings.pressure_solver,
                settings.turbulence_solver, settings.turbulence_solver]
     for count, name in zip(krylov, solvers):
@@ -311,18 +352,33 @@ def hills_leg(full, cores, cpu):
                                     prob.omega_init, mixed_precision=prec == "mixed")
             tune = b.autotune(th, iterations=tune_its)
             legs[prec] = (b.evaluate(th), b.last_timing(), tune)
+            legs["levels"] = b.amg_levels()
             print(f"# {key} {prec}: {legs[prec][0].kernel_ms / cap:.2f} ms per outer iteration "
                   f"(tuned block {tune['block_size']}, {tune['grid_blocks']} per draw)",
                   file=sys.stderr, flush=True)
         r, t, tune = legs["fp64"]
         rm, tm, tunem = legs["mixed"]
         kry = np.asarray(r.krylov_iterations, dtype=float).mean(axis=0) / cap
+        m = prob.mesh
+        launch_bytes = sum(solve_bytes(m.n_cells, m.n_internal, m.n_boundary,
+                                       int(r.iteration_counts[i]), r.krylov_iterations[i],
+                                       prob.settings.turb_start_iter, prob.settings,
+                                       legs["levels"])
+                           for i in range(nd))
+        achieved = launch_bytes / (r.kernel_ms / 1e3) / 1e9
         out[key] = {"draws": nd, "cells": prob.mesh.n_cells, "capped_iterations": cap,
                     "ms_per_outer_iteration": round(r.kernel_ms / cap, 3),
                     "draw_iterations_per_s": round(nd * cap / (r.kernel_ms / 1e3), 1),
                     "grid_blocks": t["grid_blocks"], "block_size": t["block_size"],
                     "autotune": {"iterations": tune_its, "candidates": tune["candidates"]},
                     "krylov_per_outer_iteration": [round(float(k), 1) for k in kry],
+                    "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(achieved / HBM_PEAK_GBS, 5),
+                                 "kernel": "grid_solve_kernel" if t["grid_blocks"] > 1
+                                 else "simple_solve_kernel",
+                                 "algorithmic_bytes_per_launch": int(launch_bytes),
+                                 "amg_levels": legs["levels"]},
                     "mixed": dict({"ms_per_outer_iteration": round(rm.kernel_ms / cap, 3),
                                    "grid_blocks": tm["grid_blocks"],
                                    "block_size": tm["block_size"],

@@ -187,6 +187,13 @@ int rq_engine_set_graph_replay(rq_engine *engine, int32_t enable, char *err, siz
 /* Graph launches so far (each capture's first launch included). */
 int64_t rq_engine_graph_launches(const rq_engine *engine);
 
+/* No counterpart in the reference: the pressure multigrid hierarchy (for
+ * byte models). Returns the number of levels (0 without AMG, -1 for a null
+ * engine) and fills rows and CSR entries (coarse rows padded to multiples
+ * of 4) of the first cap levels. */
+int32_t rq_engine_amg_levels(const rq_engine *engine, int32_t *rows, int64_t *entries,
+                             int32_t cap);
+
 /* Library identity: compiler, target and build flags. */
 const char *rq_build_info(void);
 

@@ -74,6 +74,9 @@ def test_hill_grid_candidates_and_capped_fields_agree():
     rep = tuned.autotune(theta, iterations=20)
     print("hill 120x64 candidates", rep, flush=True)
     _check_report(rep, [64, 128, 256, 512], [2, 4, 8, 16, 32], (256, 512, 2048))
+    levels = tuned.amg_levels()
+    assert levels[0][0] == prob.mesh.n_cells and levels[-1][0] <= 128
+    assert all(a[0] > b[0] for a, b in zip(levels, levels[1:]))
     a = tuned.evaluate(theta, fields=True)
     t = tuned.last_timing()
     assert t["grid_blocks"] == rep["grid_blocks"]
