@@ -325,7 +325,10 @@ int64_t build_layout(ArenaLayout &L, int nc, int nif, int nf, int nb, int nobs, 
 // along the strong direction, and every level of a multi-workgroup V-cycle
 // costs grid barriers. Measured: 120x64 hill 7680/3845/1701/447/121/34 rows
 // -> 7680/1701/121/34, 32 -> 35 CG iterations per p' solve, 4.4 -> 3.5 ms per
-// outer iteration; 480x256 eight levels -> five, 27.7 -> 20.8 ms.
+// outer iteration; 480x256 eight levels -> five, 27.7 -> 20.8 ms. From 16384
+// cells up the finest level aggregates a third time (480x256: 20.8 -> 19.8 ms,
+// 240x128: 7.3 -> 6.3 ms; on the 120x64 hill it costs 3.5 -> 3.6 ms, as the
+// CG iterations grow from 35 to 42).
 // ---------------------------------------------------------------------------
 struct AmgHostLevel {
     int n = 0;
@@ -338,7 +341,8 @@ struct AmgHost {
     std::vector<int> src0;
 };
 
-constexpr bool kAmgAggregateTwice = true;
+// meshes from this many cells up aggregate their finest level three times
+constexpr int kAmgThirdPassRows = 16384;
 
 void amg_aggregate(const AmgHostLevel &A, double threshold, std::vector<int> &map,
                    int &ncoarse) {
@@ -429,7 +433,9 @@ void amg_build(const rq_mesh_desc &m, const int *ISL, const int *IBF, bool has_o
         int ncoarse = 0;
         amg_aggregate(F, l == 0 ? 0.25 : 0.0, map, ncoarse);
         if (ncoarse == 0 || ncoarse >= F.n) break;
-        if (kAmgAggregateTwice && ncoarse > 50) {
+        // large meshes merge their finest aggregates a third time
+        const int passes = (l == 0 && F.n >= kAmgThirdPassRows) ? 3 : 2;
+        for (int pass = 1; pass < passes && ncoarse > 50; ++pass) {
             // a second pass over the aggregates (their Galerkin graph, no
             // strength filter) merges them again: coarse rows gather about
             // four fine rows instead of two, so the hierarchy has fewer levels
@@ -454,10 +460,9 @@ void amg_build(const rq_mesh_desc &m, const int *ISL, const int *IBF, bool has_o
             std::vector<int> map2;
             int n2 = 0;
             if (diag_ok) amg_aggregate(T, 0.0, map2, n2);
-            if (n2 > 0 && n2 < ncoarse) {
-                for (int &a : map) a = map2[a];
-                ncoarse = n2;
-            }
+            if (!(n2 > 0 && n2 < ncoarse)) break;
+            for (int &a : map) a = map2[a];
+            ncoarse = n2;
         }
         // Galerkin pattern, each coarse row sorted by column
         std::vector<std::map<int, double>> rows(ncoarse);
