@@ -1,0 +1,45 @@
+"""Multi-workgroup draws against one workgroup per draw, over several
+workgroup counts, in FP64 and with FP32 Krylov: the same statuses and the
+same outer iteration counts (the convergence decision is made on the
+device by every workgroup's replica of the bookkeeping), on the converged
+inlet/outlet channel (Jacobi PCG) and the converged 30x16 hills (AMG)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cfdr_amd import cases, engine
+from fixtures import inout_channel, load, max_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(prob, **kw):
+    return engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
+                               prob.u_init, prob.p_init, prob.k_init, prob.omega_init, **kw)
+
+
+def _agree(prob, theta, mixed, counts):
+    one = _batch(prob, mixed_precision=mixed, grid_blocks=1).evaluate(theta)
+    for g in counts:
+        b = _batch(prob, mixed_precision=mixed, grid_blocks=g)
+        r = b.evaluate(theta)
+        assert b.last_timing()["grid_blocks"] == g
+        print(f"mixed={mixed} g{g}: {[int(s) for s in r.statuses]} {list(r.iteration_counts)}",
+              flush=True)
+        assert [int(s) for s in r.statuses] == [int(s) for s in one.statuses], g
+        assert list(r.iteration_counts) == list(one.iteration_counts), g
+        assert max_rel(r.predictions, one.predictions) <= 1e-9, g
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_inlet_outlet_channel_over_workgroup_counts(mixed):
+    d = load("ref_channel_inout.json")["schemes"]["limitedLinearUpwind"]
+    prob = inout_channel(d["momentum_convection"], d["turbulence_convection"])
+    _agree(prob, np.asarray(d["thetas"]), mixed, (2, 3, 8))
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_converged_hills_over_workgroup_counts(mixed):
+    d = load("ref_hill_small_converged.json")
+    _agree(cases.hill_problem(nx=30, ny=16), np.asarray(d["thetas"]), mixed, (2, 8))
