@@ -20,7 +20,9 @@ run in one persistent-kernel launch. The line printed is
 
 The sweep runs on the launch shape the autotuner picks (--no-autotune keeps the
 default), and every step replays the HIP graph captured on the first
-evaluate (--no-graph launches directly). Further legs (each can be switched off): mixed (the sweep with FP32 Krylov
+evaluate (--no-graph launches directly). Further legs (each can be switched
+off): single_solve (one FP64 solve at the default coefficients, one launch,
+with the oracle on one thread), mixed (the sweep with FP32 Krylov
 and FP64 defect correction), served (Poisson arrivals of single requests on
 64-512 cell channels against the plain one-launch-per-request path), hills
 (the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
@@ -195,6 +197,39 @@ def cpu_baseline(prob, theta, cores):
                  "kind": "port",
                  "sample": f"{len(rows)} draws of the 64-draw sweep, one per thread, "
                            f"{dt:.1f} s wall"}
+
+
+def single_solve_leg(cpu):
+    """The single FP64 solve at the default coefficients (a1 0.31, beta* 0.09)
+    on the 1x128 channel: one launch on the GPU (the plain method), and the
+    CPU oracle on one thread beside it."""
+    from cfdr_amd import cases, engine
+    prob = cases.channel_problem()
+    theta = np.asarray([[0.31, 0.09]])
+    b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
+                            prob.u_init, prob.p_init, prob.k_init, prob.omega_init)
+    b.evaluate(theta)  # first launch loads the code object
+    t0 = time.perf_counter()
+    r = b.evaluate(theta)
+    dt = time.perf_counter() - t0
+    out = {"status": r.statuses[0].name, "iterations": int(r.iteration_counts[0]),
+           "gpu_ms": round(1e3 * dt, 1), "kernel_ms": round(r.kernel_ms, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
+                                    prob.init_vector())
+        t0 = time.perf_counter()
+        o = O.evaluate(prob.coefficients(theta[0]))
+        out["cpu_baseline"] = {"value_ms": round(1e3 * (time.perf_counter() - t0), 1),
+                               "cores": 1, "kind": "port",
+                               "sample": "the same solve on one thread"}
+        if o["status"] != int(r.statuses[0]):
+            raise SystemExit("bench.py: the single solve's status differs from the oracle's")
+        out["max_rel_err_vs_oracle"] = float(np.max(np.abs(r.predictions[0] - o["predictions"]) /
+                                                    np.abs(o["predictions"])))
+    print(f"# single solve: {out['gpu_ms']} ms on the GPU", file=sys.stderr, flush=True)
+    return out
 
 
 def copy_calibration(mib):
@@ -542,6 +577,8 @@ def main():
     ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-hills", action="store_true")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single default-coefficient solve (GPU and one CPU thread)")
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-warm-fd", action="store_true")
     ap.add_argument("--hills-full", action="store_true",
@@ -671,6 +708,10 @@ def main():
                 errs.append(float(np.max(np.abs(p - q) / np.abs(q))))
         err = max(errs) if errs else None
 
+    single = None
+    if not args.no_single:
+        single = single_solve_leg(not args.no_cpu_baseline)
+
     served = None
     if not args.no_serve:
         served = served_leg(args.serve_requests, args.serve_rate, args.serve_hills)
@@ -717,6 +758,7 @@ def main():
         "cpu_baseline": cpu,
         "max_rel_err_vs_oracle": err,
         "mixed": mixed,
+        "single_solve": single,
         "served": served,
         "warm_start": warm,
         "warm_fd_jacobian": warm_fd,
