@@ -11,9 +11,11 @@ run in one persistent-kernel launch. The line printed is
                launch / kernel time (HIP events on the engine stream), against
                the 8 TB/s HBM peak; traffic is read from the committed
                rocprofv3 --pmc summary when one exists
-  cpu_baseline the CPU FP64 oracle on the host cores, a bounded sample of the
-               same draws, one draw per thread
-  max_rel_err_vs_oracle  probe error of the GPU rows against the oracle rows
+  cpu_baseline the reference's own FP64 solver (compiled from its sources into
+               oracle/_ref/ref_harness where it was built; otherwise the CPU
+               FP64 oracle) on the host cores, a bounded sample of the same
+               draws, one draw per process (thread)
+  max_rel_err_vs_cpu_baseline  probe error of the GPU rows against the CPU rows
                of that sample (statuses must match)
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
@@ -199,6 +201,52 @@ def cpu_baseline(prob, theta, cores):
                            f"{dt:.1f} s wall"}
 
 
+def reference_cpu_baseline(prob, theta, cores):
+    """The reference's own solver, compiled from its sources into
+    oracle/_ref/ref_harness (where that was built), one draw per process on
+    `cores` processes at once. Returns (rows, baseline) like cpu_baseline, or
+    None when the harness is absent."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.access(harness, os.X_OK):
+        return None
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_fixtures import problem_lines
+    tmp = tempfile.mkdtemp(prefix="rq_ref_")
+    mesh_path = os.path.join(tmp, "mesh.msh")
+    prob.mesh.save_binary(mesh_path)
+    rows = theta[:cores]
+    paths = []
+    for i, th in enumerate(rows):
+        path = os.path.join(tmp, f"case{i}.txt")
+        with open(path, "w") as fh:
+            fh.write("\n".join(problem_lines(prob, mesh_path, [th])) + "\n")
+        paths.append(path)
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([harness, p], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                              text=True) for p in paths]
+    outs = [pr.communicate()[0] for pr in procs]
+    dt = time.perf_counter() - t0
+    if any(pr.returncode != 0 for pr in procs):
+        return None
+    res = []
+    for out in outs:
+        body = out[out.index("{"):].replace("NaN", "null").replace("-Infinity", "-1e999")
+        row = json.loads(body.replace("Infinity", "1e999"))["rows"][0]
+        res.append({"status": int(row["status"]), "predictions":
+                    np.asarray([np.nan if v is None else v for v in row["predictions"]], float),
+                    "wall_seconds": row.get("wall_seconds")})
+    conv = sum(1 for r in res if r["status"] == 0)
+    solve_s = sum(r["wall_seconds"] or 0.0 for r in res)
+    return res, {"value": conv / dt, "unit": "converged solves/s", "cores": len(rows),
+                 "kind": "reference",
+                 "sample": f"{len(rows)} draws of the 64-draw sweep, one per process of the "
+                           f"reference's solver (oracle/_ref/ref_harness), {dt:.1f} s wall "
+                           f"({solve_s / max(len(rows), 1):.2f} s per solve inside the solver)"}
+
+
 def single_solve_leg(cpu):
     """The single FP64 solve at the default coefficients (a1 0.31, beta* 0.09)
     on the 1x128 channel: one launch on the GPU (the plain method), and the
@@ -215,19 +263,27 @@ def single_solve_leg(cpu):
     out = {"status": r.statuses[0].name, "iterations": int(r.iteration_counts[0]),
            "gpu_ms": round(1e3 * dt, 1), "kernel_ms": round(r.kernel_ms, 1)}
     if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py
-        O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
-                                    prob.init_vector())
-        t0 = time.perf_counter()
-        o = O.evaluate(prob.coefficients(theta[0]))
-        out["cpu_baseline"] = {"value_ms": round(1e3 * (time.perf_counter() - t0), 1),
-                               "cores": 1, "kind": "port",
-                               "sample": "the same solve on one thread"}
+        ref = reference_cpu_baseline(prob, theta, 1)
+        if ref is not None:
+            o = ref[0][0]
+            out["cpu_baseline"] = {"value_ms": round(1e3 * o["wall_seconds"], 1), "cores": 1,
+                                   "kind": "reference",
+                                   "sample": "the same solve by the reference's solver "
+                                             "(oracle/_ref/ref_harness), its own solve timer"}
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py
+            O = oracle_py.OracleProblem(prob.mesh, prob.bcs, prob.settings, prob.obs, prob.nu,
+                                        prob.init_vector())
+            t0 = time.perf_counter()
+            o = O.evaluate(prob.coefficients(theta[0]))
+            out["cpu_baseline"] = {"value_ms": round(1e3 * (time.perf_counter() - t0), 1),
+                                   "cores": 1, "kind": "port",
+                                   "sample": "the same solve on one thread"}
         if o["status"] != int(r.statuses[0]):
-            raise SystemExit("bench.py: the single solve's status differs from the oracle's")
-        out["max_rel_err_vs_oracle"] = float(np.max(np.abs(r.predictions[0] - o["predictions"]) /
-                                                    np.abs(o["predictions"])))
+            raise SystemExit("bench.py: the single solve's status differs from the CPU's")
+        out["max_rel_err_vs_cpu_baseline"] = float(
+            np.max(np.abs(r.predictions[0] - o["predictions"]) / np.abs(o["predictions"])))
     print(f"# single solve: {out['gpu_ms']} ms on the GPU", file=sys.stderr, flush=True)
     return out
 
@@ -697,7 +753,9 @@ def main():
     cpu = None
     err = None
     if not args.no_cpu_baseline:
-        rows, cpu = cpu_baseline(prob, theta, _cores())
+        # the reference's own solver where its harness was built, else the port
+        ref = reference_cpu_baseline(prob, theta, _cores())
+        rows, cpu = ref if ref is not None else cpu_baseline(prob, theta, _cores())
         errs = []
         for i, o in enumerate(rows):
             if o["status"] != last.status_codes[i]:
@@ -756,7 +814,7 @@ def main():
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "max_rel_err_vs_oracle": err,
+        "max_rel_err_vs_cpu_baseline": err,
         "mixed": mixed,
         "single_solve": single,
         "served": served,
