@@ -7,7 +7,9 @@ the reference's MSH2 format or built by the reference's own generators, and
 stores inputs and outputs as small JSON files. It needs the reference sources
 (only present where fixtures are made); tests read the JSON files only.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py              # every fixture
+    python tests/golden/make_fixtures.py warm         # warm-start fixtures only
+    python tests/golden/make_fixtures.py body_force   # body-force fixture only
 """
 from __future__ import annotations
 
@@ -101,6 +103,19 @@ def problem_lines(prob, mesh_path, thetas, bc=(1.0, 1.0, 10.0), dump=None, extra
     if dump:
         L.append(f"dump {dump}")
     return L
+
+
+def inout_lines():
+    """Case lines of the developing channel (tests/fixtures.py inout_channel)
+    after its mesh: viscosity, inlet values, settings, initial state and
+    observations."""
+    base = ["nu 0.001", "bcs_channel 1.0 0.001 5.0", "set max_iterations 6000",
+            "set convergence_tol 1e-5", "set pressure_solver PCG", "set alpha_u 0.7",
+            "set alpha_p 0.3", "init 1 0 0 0 0.001 5.0"]
+    base += ["obs_u %r %r %d %r 0.1" % o for o in INOUT_PROBES]
+    base += ["obs_p %r %r 0.0 0.1 1.0 0.0 1.0" % o for o in INOUT_TAPS]
+    base += ["obs_cf bottom 4.0 0.0 0.005 0.001 1.0", "params 9 8"]
+    return base
 
 
 def mesh_arrays(m: Mesh):
@@ -222,12 +237,7 @@ def main():
     #    mesh, kept as an MSH2 fixture), pressure taps and skin friction, with
     #    each convection scheme (streamwise-invariant channels do not exercise them)
     io_mesh = os.path.join(HERE, "ref_channel_inout_24x12.msh")
-    base = ["nu 0.001", "bcs_channel 1.0 0.001 5.0", "set max_iterations 6000",
-            "set convergence_tol 1e-5", "set pressure_solver PCG", "set alpha_u 0.7",
-            "set alpha_p 0.3", "init 1 0 0 0 0.001 5.0"]
-    base += ["obs_u %r %r %d %r 0.1" % o for o in INOUT_PROBES]
-    base += ["obs_p %r %r 0.0 0.1 1.0 0.0 1.0" % o for o in INOUT_TAPS]
-    base += ["obs_cf bottom 4.0 0.0 0.005 0.001 1.0", "params 9 8"]
+    base = inout_lines()
     out = {"case": "makeChannel2D(24, 12, 6, 1), nu 1e-3, inlet U 1, k 1e-3, omega 5, "
                    "tol 1e-5, PCG pressure, alpha_u 0.7, alpha_p 0.3",
            "mesh": os.path.basename(io_mesh), "schemes": {}}
@@ -296,11 +306,41 @@ def warm_fixture(tmp):
                                        "thetas": thetas, "rows": res["rows"]})
 
 
+def body_force_fixture(tmp):
+    """A wall-normal body force (SolverSettings::bodyForce.y, the momentum
+    source of SIMPLESolver.cpp:303-482): the small periodic channel driven by
+    (1, 0.25), where the y force is held by a wall-normal pressure gradient,
+    and the developing channel with upwind transport under (0, -0.2)."""
+    out = {}
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
+    mp = os.path.join(tmp, "ch32f.msh")
+    prob.mesh.save_binary(mp)
+    row = run_case(problem_lines(prob, mp, [[0.31, 0.09]], dump=os.path.join(tmp, "bf"),
+                                 extra=["set body_force_y 0.25"]))["rows"][0]
+    row["fields"] = read_dump(os.path.join(tmp, "bf_0.bin"))
+    out["periodic"] = {"case": "channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, "
+                               "max_iterations=40000), body force (1, 0.25)",
+                       "body_force": [1.0, 0.25], "theta": [0.31, 0.09], "row": row}
+    io_mesh = os.path.join(HERE, "ref_channel_inout_24x12.msh")
+    lines = [f"mesh_file {io_mesh}"] + inout_lines()
+    lines += ["set momentum_convection upwind", "set turbulence_convection upwind",
+              "set body_force_y -0.2", "theta 0.31 0.09", "theta 0.33 0.095",
+              f"dump {os.path.join(tmp, 'bfio')}"]
+    res = run_case(lines)
+    res["rows"][0]["fields"] = read_dump(os.path.join(tmp, "bfio_0.bin"))
+    out["inout"] = {"case": "inout_channel('upwind', 'upwind') of tests/fixtures.py, "
+                            "body force (0, -0.2)",
+                    "body_force": [0.0, -0.2], "thetas": [[0.31, 0.09], [0.33, 0.095]],
+                    "rows": res["rows"]}
+    write("ref_channel_body_force.json", out)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["warm"]:
+    if sys.argv[1:] in (["warm"], ["body_force"]):
         if not os.path.exists(HARNESS):
             raise SystemExit("build the reference harness first: make -C oracle ref")
-        warm_fixture(tempfile.mkdtemp())
+        (warm_fixture if sys.argv[1] == "warm" else body_force_fixture)(tempfile.mkdtemp())
     else:
         main()
         warm_fixture(tempfile.mkdtemp())
+        body_force_fixture(tempfile.mkdtemp())

@@ -189,6 +189,35 @@ def test_inlet_outlet_channel_matches_reference(scheme):
         assert max_scaled(f[name], d["rows"][0]["fields"][name]) <= 1e-6, name
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_wall_normal_body_force_matches_reference(mixed):
+    """A y body force: the 1x32 periodic channel (arena in LDS) under
+    (1, 0.25) and the 288-cell developing channel (arena in HBM) under
+    (0, -0.2). FP64 rows and fields follow the reference; with FP32 Krylov
+    plus FP64 defect correction the rows do (status, iterations, probes and
+    log-likelihood at REL_OBS)."""
+    d = load("ref_channel_body_force.json")
+    c = d["periodic"]
+    prob = small_channel()
+    prob.settings.body_force.y = c["body_force"][1]
+    res = _batch(prob, mixed_precision=mixed).evaluate(np.asarray([c["theta"]]), fields=True)
+    _check_row(res, 0, c["row"])
+    if not mixed:
+        f = _fields(res, 0, prob.mesh.n_cells)
+        for name in ("Ux", "Uy", "p", "k", "omega", "nuT", "flux"):
+            assert max_scaled(f[name], c["row"]["fields"][name]) <= FIELD_TOL, name
+    io = d["inout"]
+    prob = inout_channel("upwind", "upwind")
+    prob.settings.body_force.y = io["body_force"][1]
+    res = _batch(prob, mixed_precision=mixed).evaluate(np.asarray(io["thetas"]), fields=True)
+    for i, row in enumerate(io["rows"]):
+        _check_row(res, i, row)
+    if not mixed:
+        f = _fields(res, 0, prob.mesh.n_cells)
+        for name in FIELD_NAMES + ["flux"]:
+            assert max_scaled(f[name], io["rows"][0]["fields"][name]) <= 1e-6, name
+
+
 def test_symmetry_patch_matches_reference():
     d = load("ref_channel_symmetry.json")
     prob = symmetry_channel()

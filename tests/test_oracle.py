@@ -348,6 +348,37 @@ def test_symmetry_patch():
         assert np.array_equal(r[name], np.asarray(d["row"]["fields"][name])), name
 
 
+def test_wall_normal_body_force():
+    """A y body force on the periodic channel and on the developing channel
+    (the momentum source of SIMPLESolver.cpp:303-482): every row and field
+    equals the reference's, and on the periodic channel the force is held by
+    the pressure: away from the walls p rises by f_y per unit height."""
+    d = load("ref_channel_body_force.json")
+    c = d["periodic"]
+    prob = small_channel()
+    fy = c["body_force"][1]
+    prob.settings.body_force.y = fy
+    r = _oracle(prob).evaluate(prob.coefficients(c["theta"]), fields=True)
+    _check_row(r, c["row"])
+    for name in FIELD_NAMES + ["flux"]:
+        assert np.array_equal(r[name], np.asarray(c["row"]["fields"][name])), name
+    y = prob.mesh.cell_center[:, 1]
+    order = np.argsort(y)
+    dpdy = np.diff(np.asarray(r["p"])[order]) / np.diff(y[order])
+    core = dpdy[len(dpdy) // 4: -(len(dpdy) // 4)]
+    assert np.all(np.abs(core - fy) <= 5e-4 * fy), core
+    io = d["inout"]
+    prob = inout_channel("upwind", "upwind")
+    prob.settings.body_force.y = io["body_force"][1]
+    P = _oracle(prob)
+    for i, (theta, row) in enumerate(zip(io["thetas"], io["rows"])):
+        r = P.evaluate(prob.coefficients(theta), fields=i == 0)
+        _check_row(r, row)
+        if i == 0:
+            for name in FIELD_NAMES + ["flux"]:
+                assert np.array_equal(r[name], np.asarray(row["fields"][name])), name
+
+
 def test_converged_small_hill_amg():
     """Converged 30x16 hill with the reference's AMG pressure solver."""
     d = load("ref_hill_small_converged.json")
