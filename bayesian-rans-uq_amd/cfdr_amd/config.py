@@ -265,6 +265,23 @@ class FlowBoundaryConditions:
                 om.append(PatchBC())
         return FlowBoundaryConditions(vel, pre, kk, om)
 
+    @staticmethod
+    def couette_defaults(mesh: Mesh, u_wall: float, k_in: float, om_in: float):
+        """couetteDefaults (BoundaryCondition.hpp:204-222): the channel
+        defaults at half the wall speed, the patch named 'top' a wall moving
+        at (u_wall, 0, 0) (k and omega keep their wall treatment), and the
+        patch named 'inlet' zero-gradient in every field."""
+        bc = FlowBoundaryConditions.channel_defaults(mesh, 0.5 * u_wall, k_in, om_in)
+        for i, p in enumerate(mesh.patches):
+            if p.name == "top":
+                bc.velocity[i] = PatchBC(BC_WALL_MOVING, 0.0, Vec3(u_wall, 0, 0))
+            elif p.name == "inlet":
+                bc.velocity[i] = PatchBC(BC_NEUMANN)
+                bc.pressure[i] = PatchBC(BC_NEUMANN)
+                bc.k[i] = PatchBC(BC_NEUMANN)
+                bc.omega[i] = PatchBC(BC_NEUMANN)
+        return bc
+
     def validate(self, mesh: Mesh) -> None:
         """validatePatchBCs + simple_admission::turbulenceBoundary
         (BoundaryCondition.hpp:45-103, SIMPLEValidation.hpp:14-31)."""

@@ -12,9 +12,11 @@
 //   mesh_periodic_channel nx ny Lx h ReTau yplus
 //   mesh_curved nNodes x0 yb0 x1 yb1 ... yTop ny Re yplus
 //   mesh_channel nx ny Lx Ly                   inlet/outlet channel (makeChannel2D)
+//   mesh_channel_graded nx ny Lx Ly Re yplus   the same, wall-graded (makeChannel2D overload)
 //   save_mesh <path.msh>
 //   nu <value>
 //   bcs_channel Uin kIn omIn
+//   bcs_couette Uwall kIn omIn                 moving top wall (couetteDefaults)
 //   set <key> <value>                          SolverSettings field
 //   init Ux Uy Uz p k omega
 //   obs_u x y comp observed sigma
@@ -119,7 +121,7 @@ int main(int argc, char **argv) {
     SolverSettings s;
     s.verbose = false;
     FlowBoundaryConditions bcs;
-    bool haveBcs = false;
+    bool haveBcs = false, couette = false;
     double bcU = 1, bcK = 1e-4, bcW = 1;
     Vec3 U0(1, 0, 0);
     double p0 = 0, k0 = 1e-4, w0 = 1;
@@ -163,6 +165,12 @@ int main(int argc, char **argv) {
             ls >> nx >> ny >> Lx >> Ly;
             mesh = Mesh::makeChannel2D(nx, ny, Lx, Ly);
             haveMesh = true;
+        } else if (cmd == "mesh_channel_graded") {
+            int nx, ny;
+            double Lx, Ly, Re, yplus;
+            ls >> nx >> ny >> Lx >> Ly >> Re >> yplus;
+            mesh = Mesh::makeChannel2D(nx, ny, Lx, Ly, Re, yplus);
+            haveMesh = true;
         } else if (cmd == "save_mesh") {
             ls >> savePath;
         } else if (cmd == "nu") {
@@ -170,6 +178,9 @@ int main(int argc, char **argv) {
         } else if (cmd == "bcs_channel") {
             ls >> bcU >> bcK >> bcW;
             haveBcs = true;
+        } else if (cmd == "bcs_couette") {
+            ls >> bcU >> bcK >> bcW;
+            haveBcs = couette = true;
         } else if (cmd == "set") {
             std::string k, v;
             ls >> k >> v;
@@ -214,7 +225,8 @@ int main(int argc, char **argv) {
     if (!savePath.empty()) mesh.saveBinary(savePath);
     mesh.computeWallDistance();
     if (!haveBcs) bcU = 1.0;
-    bcs = FlowBoundaryConditions::channelDefaults(mesh, bcU, bcK, bcW);
+    bcs = couette ? FlowBoundaryConditions::couetteDefaults(mesh, bcU, bcK, bcW)
+                  : FlowBoundaryConditions::channelDefaults(mesh, bcU, bcK, bcW);
     InferenceParameterSet ps = InferenceParameterSet::fromIndices("harness", params.empty() ? std::vector<int>{9, 8} : params);
 
     ForwardModel model(mesh, ps, obs, bcs, nu, s, U0, p0, k0, w0);

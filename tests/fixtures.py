@@ -58,6 +58,31 @@ def inout_channel(momentum_scheme, turbulence_scheme):
                          u_init=Vec3(1.0, 0.0, 0.0), p_init=0.0, k_init=0.001, omega_init=5.0)
 
 
+def couette_problem(pressure_solver="PCG"):
+    """Moving-wall Couette flow of ref_couette.json: the reference's graded
+    makeChannel2D(4, 96, 1.5, 2) mesh (MSH2 fixture), top wall at Uwall 2,
+    couetteDefaults, skin friction on 'bottom' and 18 velocity stations at
+    x = 0.75, tol 1e-4, alpha_u 0.7, alpha_p 0.5 (the reference's Couette
+    regression, tests/tools/record_goldens.cpp:277-301)."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    d = load("ref_couette.json")
+    mesh = Mesh.load_binary(os.path.join(GOLDEN, "ref_couette_4x96.msh"))
+    mesh.compute_wall_distance()
+    bcs = FlowBoundaryConditions.couette_defaults(mesh, d["u_wall"], d["k_in"], d["omega_in"])
+    s = SolverSettings(max_iterations=12000, convergence_tol=1e-4, alpha_u=0.7, alpha_p=0.5,
+                       pressure_solver=pressure_solver)
+    obs = ObservationOperator()
+    obs.add_skin_friction("bottom", (0.75, 0.0), 6e-3, 1e-3, d["u_bulk"])
+    for y in d["stations"]:
+        obs.add_velocity_profile((0.75, y), 0, 0.0, 0.01)
+    return cases.Problem(mesh=mesh, bcs=bcs, settings=s, obs=obs,
+                         params=InferenceParameterSet.a1_betaStar(), nu=d["nu"],
+                         u_init=Vec3(d["u_bulk"], 0.0, 0.0), p_init=0.0, k_init=d["k_in"],
+                         omega_init=d["omega_in"])
+
+
 def symmetry_channel():
     """1x32 geometric channel (ratio 1.15) whose top patch is a symmetry
     plane, 5000 capped iterations at tol 1e-6."""

@@ -10,6 +10,7 @@ stores inputs and outputs as small JSON files. It needs the reference sources
     python tests/golden/make_fixtures.py              # every fixture
     python tests/golden/make_fixtures.py warm         # warm-start fixtures only
     python tests/golden/make_fixtures.py body_force   # body-force fixture only
+    python tests/golden/make_fixtures.py couette      # moving-wall Couette fixture only
 """
 from __future__ import annotations
 
@@ -335,12 +336,46 @@ def body_force_fixture(tmp):
     write("ref_channel_body_force.json", out)
 
 
+def couette_fixture(tmp):
+    """Moving-wall Couette flow (couetteDefaults, BoundaryCondition.hpp:204-222)
+    on the reference's wall-graded makeChannel2D(4, 96, 1.5, 2, Uwall/nu, 0.3)
+    mesh, set up as the reference's own Couette regression
+    (tests/tools/record_goldens.cpp:277-301: Uwall 2, Ub 1, skin friction and
+    18 velocity stations at x = 0.75), solved with Jacobi-PCG pressure so that
+    the device's Krylov path is held to the reference's. The mesh is kept as
+    an MSH2 fixture; the reference's committed AMG result of the same case is
+    ref_couette_re171_ny96.json (a copy of its golden data)."""
+    g = json.load(open(os.path.join(HERE, "ref_couette_re171_ny96.json")))
+    nu, u_wall, ub = g["nu"], 2.0, 1.0
+    k_in = 1.5 * (ub * 0.05) * (ub * 0.05)
+    om_in = k_in / (nu * 100.0)
+    msh = os.path.join(HERE, "ref_couette_4x96.msh")
+    lines = [f"mesh_channel_graded 4 96 1.5 2.0 {u_wall / nu!r} 0.3", f"save_mesh {msh}",
+             f"nu {nu!r}", f"bcs_couette {u_wall!r} {k_in!r} {om_in!r}",
+             "set max_iterations 12000", "set convergence_tol 0.0001", "set alpha_u 0.7",
+             "set alpha_p 0.5", "set pressure_solver PCG",
+             f"init {ub!r} 0 0 0 {k_in!r} {om_in!r}",
+             "obs_cf bottom 0.75 0.0 0.006 0.001 1.0"]
+    lines += [f"obs_u 0.75 {y!r} 0 0.0 0.01" for y in g["qoi"]["probe_y_over_h"]]
+    lines += ["params 9 8", "theta 0.31 0.09", "theta 0.32 0.093",
+              f"dump {os.path.join(tmp, 'cou')}"]
+    res = run_case(lines)
+    res["rows"][0]["fields"] = read_dump(os.path.join(tmp, "cou_0.bin"))
+    write("ref_couette.json", {"case": "Couette Re_tau 171 (record_goldens.cpp:277-301) on "
+                                       "ref_couette_4x96.msh with PCG pressure",
+                               "nu": nu, "u_wall": u_wall, "u_bulk": ub, "k_in": k_in,
+                               "omega_in": om_in, "stations": g["qoi"]["probe_y_over_h"],
+                               "thetas": [[0.31, 0.09], [0.32, 0.093]], "rows": res["rows"]})
+
+
+EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture}
+
 if __name__ == "__main__":
-    if sys.argv[1:] in (["warm"], ["body_force"]):
+    if len(sys.argv) == 2 and sys.argv[1] in EXTRA:
         if not os.path.exists(HARNESS):
             raise SystemExit("build the reference harness first: make -C oracle ref")
-        (warm_fixture if sys.argv[1] == "warm" else body_force_fixture)(tempfile.mkdtemp())
+        EXTRA[sys.argv[1]](tempfile.mkdtemp())
     else:
         main()
-        warm_fixture(tempfile.mkdtemp())
-        body_force_fixture(tempfile.mkdtemp())
+        for make in EXTRA.values():
+            make(tempfile.mkdtemp())

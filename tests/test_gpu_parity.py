@@ -20,8 +20,8 @@ import numpy as np
 import pytest
 
 from cfdr_amd import cases, engine
-from fixtures import (FIELD_NAMES, inout_channel, load, max_rel, max_scaled, small_channel,
-                      small_hill, symmetry_channel)
+from fixtures import (FIELD_NAMES, couette_problem, inout_channel, load, max_rel, max_scaled,
+                      small_channel, small_hill, symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -187,6 +187,44 @@ def test_inlet_outlet_channel_matches_reference(scheme):
     f = _fields(res, 0, prob.mesh.n_cells)
     for name in FIELD_NAMES + ["flux"]:
         assert max_scaled(f[name], d["rows"][0]["fields"][name]) <= 1e-6, name
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_moving_wall_couette_matches_reference(mixed):
+    """Moving top wall (WallMoving) on the graded 4x96 Couette channel (arena
+    in HBM), PCG pressure: FP64 rows and fields follow the reference's; with
+    FP32 Krylov plus FP64 defect correction the rows do. In Couette flow the
+    wall-normal velocity and the pressure are zero up to round-off (|Uy| ~
+    1e-8, |p| ~ 1e-7 against |Ux| ~ 2), so those two are scaled by the
+    velocity scale max|Ux| (resp. its square) instead of their own size."""
+    d = load("ref_couette.json")
+    prob = couette_problem()
+    res = _batch(prob, mixed_precision=mixed).evaluate(np.asarray(d["thetas"]), fields=True)
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+    if not mixed:
+        f = _fields(res, 0, prob.mesh.n_cells)
+        ref = d["rows"][0]["fields"]
+        u = float(np.max(np.abs(ref["Ux"])))
+        floor = {"Uy": u, "p": u * u}
+        for name in FIELD_NAMES + ["flux"]:
+            b = np.asarray(ref[name])
+            scale = max(float(np.max(np.abs(b))), floor.get(name, 0.0))
+            assert float(np.max(np.abs(f[name] - b))) <= 1e-6 * scale, name
+
+
+def test_reference_couette_golden_amg():
+    """The reference's committed Couette regression (its AMG pressure
+    solver): the device AMG (Jacobi-smoothed) reaches the same status, and
+    probes within the 1e-4 it meets against the reference's AMG elsewhere."""
+    g = load("ref_couette_re171_ny96.json")
+    prob = couette_problem(pressure_solver="AMG")
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    assert res.statuses[0] == 0 and not res.stalled[0]
+    its = g["result"]["final_iter"]
+    assert abs(int(res.iteration_counts[0]) - its) <= max(50, int(0.02 * its))
+    q = g["qoi"]
+    assert max_rel(res.predictions[0], [q["cf_probe"]] + q["u_probes"]) <= 1e-4
 
 
 @pytest.mark.parametrize("mixed", [False, True])

@@ -7,9 +7,10 @@ checker of the HIP engine.
 * exact solutions of small linear systems for every Krylov/AMG solver
   (core/tests/test_linear_scratch.cpp:113-119,202-229) and a random SPD
   Laplacian against numpy;
-* the reference's own committed regression golden for the periodic channel
-  (tests/golden/constrained_omega/periodic_channel_re180.json of the
-  reference, copied as data);
+* the reference's own committed regression goldens for the periodic channel
+  and the moving-wall Couette flow
+  (tests/golden/constrained_omega/periodic_channel_re180.json and
+  couette_re171_ny96.json of the reference, copied as data);
 * fixtures written by the reference solver compiled from its sources
   (tests/golden/make_fixtures.py): full solves, intermediate states after K
   iterations, status edge cases, the config-1 channel and a small hill;
@@ -28,8 +29,8 @@ import oracle_py
 from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
-from fixtures import (FIELD_NAMES, inout_channel, load, max_rel, small_channel, small_hill,
-                      symmetry_channel)
+from fixtures import (FIELD_NAMES, couette_problem, inout_channel, load, max_rel, small_channel,
+                      small_hill, symmetry_channel)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -346,6 +347,46 @@ def test_symmetry_patch():
     _check_row(r, d["row"])
     for name in FIELD_NAMES + ["flux"]:
         assert np.array_equal(r[name], np.asarray(d["row"]["fields"][name])), name
+
+
+def test_moving_wall_couette():
+    """Moving top wall (WallMoving, couetteDefaults) on the graded 4x96
+    channel with PCG pressure: rows and fields equal the reference's. The
+    two walls carry the same shear in Couette flow, so the mean wall shear
+    stresses of bottom and top agree."""
+    d = load("ref_couette.json")
+    prob = couette_problem()
+    P = _oracle(prob)
+    for i, (theta, row) in enumerate(zip(d["thetas"], d["rows"])):
+        r = P.evaluate(prob.coefficients(theta), fields=i == 0)
+        _check_row(r, row)
+        if i == 0:
+            for name in FIELD_NAMES + ["flux"]:
+                assert np.array_equal(r[name], np.asarray(row["fields"][name])), name
+
+
+def test_reference_couette_golden():
+    """The reference's committed Couette regression (couette_re171_ny96, its
+    AMG pressure solver, recorded with another compiler and libm): status,
+    iteration count and stalled flag exactly, values to the last few digits,
+    and the wall shear balance it reports (tau_w bottom = top to 1e-3)."""
+    g = load("ref_couette_re171_ny96.json")
+    prob = couette_problem(pressure_solver="AMG")
+    assert prob.nu == g["nu"] and prob.mesh.n_cells == 4 * g["mesh"]["ny"]
+    r = _oracle(prob).evaluate(prob.coefficients([0.31, 0.09]), fields=True)
+    res = g["result"]
+    assert r["status"] == 0 and r["iters"] == res["final_iter"]
+    assert r["stalled"] == res["stalled"]
+    q = g["qoi"]
+    assert max_rel(r["predictions"], [q["cf_probe"]] + q["u_probes"]) < 1e-9
+    assert r["worst"] == pytest.approx(res["final_worst_measure"], rel=1e-5)
+    assert r["rate"] == pytest.approx(res["contraction_rate_u"], rel=1e-9)
+    norms = res["field_norms"]
+    assert float(np.sum(r["Ux"] ** 2)) == pytest.approx(norms["sumsq_ux"], rel=1e-9)
+    assert float(np.sum(r["k"] ** 2)) == pytest.approx(norms["sumsq_k"], rel=1e-9)
+    assert float(np.sum(r["omega"] ** 2)) == pytest.approx(norms["sumsq_omega"], rel=1e-9)
+    assert float(np.sum(r["nuT"] ** 2)) == pytest.approx(norms["sumsq_nut"], rel=1e-9)
+    assert q["tau_w_top"] == pytest.approx(q["tau_w_bottom"], rel=1e-3)
 
 
 def test_wall_normal_body_force():
