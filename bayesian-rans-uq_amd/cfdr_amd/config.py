@@ -282,6 +282,21 @@ class FlowBoundaryConditions:
                 bc.omega[i] = PatchBC(BC_NEUMANN)
         return bc
 
+    @staticmethod
+    def flat_plate_defaults(mesh: Mesh, u_inf: float, k_in: float, om_in: float):
+        """flatPlateDefaults (BoundaryCondition.hpp:177-192): the channel
+        defaults, with the patch named 'top' a free stream: Dirichlet velocity
+        (u_inf, 0, 0), k and omega at their inlet values, zero-gradient
+        pressure."""
+        bc = FlowBoundaryConditions.channel_defaults(mesh, u_inf, k_in, om_in)
+        for i, p in enumerate(mesh.patches):
+            if p.name == "top":
+                bc.velocity[i] = PatchBC(BC_DIRICHLET, 0.0, Vec3(u_inf, 0, 0))
+                bc.pressure[i] = PatchBC(BC_NEUMANN)
+                bc.k[i] = PatchBC(BC_DIRICHLET, k_in)
+                bc.omega[i] = PatchBC(BC_DIRICHLET, om_in)
+        return bc
+
     def validate(self, mesh: Mesh) -> None:
         """validatePatchBCs + simple_admission::turbulenceBoundary
         (BoundaryCondition.hpp:45-103, SIMPLEValidation.hpp:14-31)."""

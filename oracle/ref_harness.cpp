@@ -13,10 +13,12 @@
 //   mesh_curved nNodes x0 yb0 x1 yb1 ... yTop ny Re yplus
 //   mesh_channel nx ny Lx Ly                   inlet/outlet channel (makeChannel2D)
 //   mesh_channel_graded nx ny Lx Ly Re yplus   the same, wall-graded (makeChannel2D overload)
+//   mesh_plate nx ny Lx Ly Re yplus            one-sided graded plate mesh (makePlate2D)
 //   save_mesh <path.msh>
 //   nu <value>
 //   bcs_channel Uin kIn omIn
 //   bcs_couette Uwall kIn omIn                 moving top wall (couetteDefaults)
+//   bcs_plate Uinf kIn omIn                    free-stream top (flatPlateDefaults)
 //   set <key> <value>                          SolverSettings field
 //   init Ux Uy Uz p k omega
 //   obs_u x y comp observed sigma
@@ -121,7 +123,8 @@ int main(int argc, char **argv) {
     SolverSettings s;
     s.verbose = false;
     FlowBoundaryConditions bcs;
-    bool haveBcs = false, couette = false;
+    bool haveBcs = false;
+    std::string bcKind = "bcs_channel";
     double bcU = 1, bcK = 1e-4, bcW = 1;
     Vec3 U0(1, 0, 0);
     double p0 = 0, k0 = 1e-4, w0 = 1;
@@ -171,6 +174,12 @@ int main(int argc, char **argv) {
             ls >> nx >> ny >> Lx >> Ly >> Re >> yplus;
             mesh = Mesh::makeChannel2D(nx, ny, Lx, Ly, Re, yplus);
             haveMesh = true;
+        } else if (cmd == "mesh_plate") {
+            int nx, ny;
+            double Lx, Ly, Re, yplus;
+            ls >> nx >> ny >> Lx >> Ly >> Re >> yplus;
+            mesh = Mesh::makePlate2D(nx, ny, Lx, Ly, Re, yplus);
+            haveMesh = true;
         } else if (cmd == "save_mesh") {
             ls >> savePath;
         } else if (cmd == "nu") {
@@ -178,9 +187,10 @@ int main(int argc, char **argv) {
         } else if (cmd == "bcs_channel") {
             ls >> bcU >> bcK >> bcW;
             haveBcs = true;
-        } else if (cmd == "bcs_couette") {
+        } else if (cmd == "bcs_couette" || cmd == "bcs_plate") {
             ls >> bcU >> bcK >> bcW;
-            haveBcs = couette = true;
+            haveBcs = true;
+            bcKind = cmd;
         } else if (cmd == "set") {
             std::string k, v;
             ls >> k >> v;
@@ -225,8 +235,12 @@ int main(int argc, char **argv) {
     if (!savePath.empty()) mesh.saveBinary(savePath);
     mesh.computeWallDistance();
     if (!haveBcs) bcU = 1.0;
-    bcs = couette ? FlowBoundaryConditions::couetteDefaults(mesh, bcU, bcK, bcW)
-                  : FlowBoundaryConditions::channelDefaults(mesh, bcU, bcK, bcW);
+    if (bcKind == "bcs_couette")
+        bcs = FlowBoundaryConditions::couetteDefaults(mesh, bcU, bcK, bcW);
+    else if (bcKind == "bcs_plate")
+        bcs = FlowBoundaryConditions::flatPlateDefaults(mesh, bcU, bcK, bcW);
+    else
+        bcs = FlowBoundaryConditions::channelDefaults(mesh, bcU, bcK, bcW);
     InferenceParameterSet ps = InferenceParameterSet::fromIndices("harness", params.empty() ? std::vector<int>{9, 8} : params);
 
     ForwardModel model(mesh, ps, obs, bcs, nu, s, U0, p0, k0, w0);

@@ -83,6 +83,31 @@ def couette_problem(pressure_solver="PCG"):
                          omega_init=d["omega_in"])
 
 
+def plate_problem():
+    """Flat plate of ref_plate.json: the reference's makePlate2D(24, 16, 1,
+    0.1, 1e5, 1) mesh (MSH2 fixture), flatPlateDefaults (free-stream top),
+    the plate-validation settings of rans-uq/apps/Main.cpp:137-176 with PCG
+    pressure, skin friction along the plate and a velocity profile at x 0.5."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    d = load("ref_plate.json")
+    mesh = Mesh.load_binary(os.path.join(GOLDEN, "ref_plate_24x16.msh"))
+    mesh.compute_wall_distance()
+    bcs = FlowBoundaryConditions.flat_plate_defaults(mesh, d["u_inf"], d["k_in"], d["omega_in"])
+    s = SolverSettings(max_iterations=d["max_iterations"], convergence_tol=1e-4, alpha_u=0.7,
+                       alpha_p=0.5, turb_start_iter=0, k_max=0.1, pressure_solver="PCG")
+    obs = ObservationOperator()
+    for x in d["cf_x"]:
+        obs.add_skin_friction("bottom", (x, 0.0), 0.004, 0.001, d["u_inf"])
+    for y in d["stations"]:
+        obs.add_velocity_profile((0.5, y), 0, 0.9, 0.01)
+    return cases.Problem(mesh=mesh, bcs=bcs, settings=s, obs=obs,
+                         params=InferenceParameterSet.a1_betaStar(), nu=d["nu"],
+                         u_init=Vec3(d["u_inf"], 0.0, 0.0), p_init=0.0, k_init=d["k_in"],
+                         omega_init=d["omega_in"])
+
+
 def symmetry_channel():
     """1x32 geometric channel (ratio 1.15) whose top patch is a symmetry
     plane, 5000 capped iterations at tol 1e-6."""

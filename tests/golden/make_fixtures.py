@@ -11,6 +11,7 @@ stores inputs and outputs as small JSON files. It needs the reference sources
     python tests/golden/make_fixtures.py warm         # warm-start fixtures only
     python tests/golden/make_fixtures.py body_force   # body-force fixture only
     python tests/golden/make_fixtures.py couette      # moving-wall Couette fixture only
+    python tests/golden/make_fixtures.py plate        # free-stream flat-plate fixture only
 """
 from __future__ import annotations
 
@@ -368,7 +369,43 @@ def couette_fixture(tmp):
                                "thetas": [[0.31, 0.09], [0.32, 0.093]], "rows": res["rows"]})
 
 
-EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture}
+def plate_fixture(tmp):
+    """Flat plate with a free-stream top (flatPlateDefaults,
+    BoundaryCondition.hpp:177-192: a Dirichlet velocity, k and omega on a
+    patch that is not an inlet) on the reference's one-sided graded
+    makePlate2D(24, 16, 1, 0.1, Re 1e5, y+ 1) mesh, set up like the
+    reference's plate validation (rans-uq/apps/Main.cpp:137-176: Tu 1 %,
+    omega from nuT/nu = 100, alpha_u 0.7, alpha_p 0.5, turbulence coupled
+    from the first iteration, k_max 0.1) at a smaller Reynolds number, with
+    PCG pressure, skin friction along the plate and a velocity profile. The
+    mesh is kept as an MSH2 fixture."""
+    re_l, u_inf, tu = 1e5, 1.0, 0.01
+    nu = u_inf * 1.0 / re_l
+    k_in = 1.5 * (u_inf * tu) * (u_inf * tu)
+    om_in = k_in / (nu * 100.0)
+    msh = os.path.join(HERE, "ref_plate_24x16.msh")
+    stations = [0.002, 0.005, 0.01, 0.02, 0.04, 0.07]
+    cf_x = [0.1, 0.3, 0.5, 0.7, 0.9]
+    lines = [f"mesh_plate 24 16 1.0 0.1 {re_l!r} 1.0", f"save_mesh {msh}", f"nu {nu!r}",
+             f"bcs_plate {u_inf!r} {k_in!r} {om_in!r}", "set max_iterations 3000",
+             "set convergence_tol 0.0001", "set alpha_u 0.7", "set alpha_p 0.5",
+             "set turb_start_iter 0", "set k_max 0.1", "set pressure_solver PCG",
+             f"init {u_inf!r} 0 0 0 {k_in!r} {om_in!r}"]
+    lines += [f"obs_cf bottom {x!r} 0.0 0.004 0.001 {u_inf!r}" for x in cf_x]
+    lines += [f"obs_u 0.5 {y!r} 0 0.9 0.01" for y in stations]
+    lines += ["params 9 8", "theta 0.31 0.09", "theta 0.3 0.088",
+              f"dump {os.path.join(tmp, 'plate')}"]
+    res = run_case(lines)
+    res["rows"][0]["fields"] = read_dump(os.path.join(tmp, "plate_0.bin"))
+    write("ref_plate.json", {"case": "flat plate on ref_plate_24x16.msh (makePlate2D(24, 16, 1, "
+                                     "0.1, 1e5, 1)), flatPlateDefaults, PCG pressure",
+                             "nu": nu, "u_inf": u_inf, "k_in": k_in, "omega_in": om_in,
+                             "max_iterations": 3000, "cf_x": cf_x, "stations": stations,
+                             "thetas": [[0.31, 0.09], [0.3, 0.088]], "rows": res["rows"]})
+
+
+EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
+         "plate": plate_fixture}
 
 if __name__ == "__main__":
     if len(sys.argv) == 2 and sys.argv[1] in EXTRA:

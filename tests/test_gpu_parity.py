@@ -21,7 +21,7 @@ import pytest
 
 from cfdr_amd import cases, engine
 from fixtures import (FIELD_NAMES, couette_problem, inout_channel, load, max_rel, max_scaled,
-                      small_channel, small_hill, symmetry_channel)
+                      plate_problem, small_channel, small_hill, symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -211,6 +211,22 @@ def test_moving_wall_couette_matches_reference(mixed):
             b = np.asarray(ref[name])
             scale = max(float(np.max(np.abs(b))), floor.get(name, 0.0))
             assert float(np.max(np.abs(f[name] - b))) <= 1e-6 * scale, name
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_free_stream_flat_plate_matches_reference(mixed):
+    """Dirichlet velocity, k and omega on the free-stream top of the graded
+    plate mesh (arena in HBM), PCG pressure: FP64 rows and fields follow the
+    reference's; with FP32 Krylov plus FP64 defect correction the rows do."""
+    d = load("ref_plate.json")
+    prob = plate_problem()
+    res = _batch(prob, mixed_precision=mixed).evaluate(np.asarray(d["thetas"]), fields=True)
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+    if not mixed:
+        f = _fields(res, 0, prob.mesh.n_cells)
+        for name in FIELD_NAMES + ["flux"]:
+            assert max_scaled(f[name], d["rows"][0]["fields"][name]) <= 1e-6, name
 
 
 def test_reference_couette_golden_amg():

@@ -29,8 +29,8 @@ import oracle_py
 from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
-from fixtures import (FIELD_NAMES, couette_problem, inout_channel, load, max_rel, small_channel,
-                      small_hill, symmetry_channel)
+from fixtures import (FIELD_NAMES, couette_problem, inout_channel, load, max_rel, plate_problem,
+                      small_channel, small_hill, symmetry_channel)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -363,6 +363,29 @@ def test_moving_wall_couette():
         if i == 0:
             for name in FIELD_NAMES + ["flux"]:
                 assert np.array_equal(r[name], np.asarray(row["fields"][name])), name
+
+
+def test_free_stream_flat_plate():
+    """A Dirichlet velocity, k and omega on a patch that is not an inlet (the
+    free-stream top of flatPlateDefaults) on the graded plate mesh: rows and
+    fields equal the reference's, and the top faces carry the free stream."""
+    d = load("ref_plate.json")
+    prob = plate_problem()
+    P = _oracle(prob)
+    for i, (theta, row) in enumerate(zip(d["thetas"], d["rows"])):
+        r = P.evaluate(prob.coefficients(theta), fields=i == 0)
+        _check_row(r, row)
+        if i == 0:
+            for name in FIELD_NAMES + ["flux"]:
+                assert np.array_equal(r[name], np.asarray(row["fields"][name])), name
+    f = d["rows"][0]["fields"]
+    assert len(f["Uxb"]) == prob.mesh.n_faces - prob.mesh.n_internal
+    top = next(p for p in prob.mesh.patches if p.name == "top")
+    b = np.asarray(top.faces) - prob.mesh.n_internal
+    assert len(b) == 24
+    assert np.all(np.asarray(f["Uxb"])[b] == d["u_inf"]) and np.all(np.asarray(f["Uyb"])[b] == 0)
+    assert np.all(np.asarray(f["kb"])[b] == d["k_in"])
+    assert np.all(np.asarray(f["omegab"])[b] == d["omega_in"])
 
 
 def test_reference_couette_golden():
