@@ -14,6 +14,8 @@
 //   mesh_channel nx ny Lx Ly                   inlet/outlet channel (makeChannel2D)
 //   mesh_channel_graded nx ny Lx Ly Re yplus   the same, wall-graded (makeChannel2D overload)
 //   mesh_plate nx ny Lx Ly Re yplus            one-sided graded plate mesh (makePlate2D)
+//   mesh_bfs nxu nxd nyu nyd Lu Ld h H Re yplus backward-facing step (makeBackwardFacingStep2D)
+//   patch_type <name> <type>                   Mesh::setPatchType
 //   save_mesh <path.msh>
 //   nu <value>
 //   bcs_channel Uin kIn omIn
@@ -180,6 +182,16 @@ int main(int argc, char **argv) {
             ls >> nx >> ny >> Lx >> Ly >> Re >> yplus;
             mesh = Mesh::makePlate2D(nx, ny, Lx, Ly, Re, yplus);
             haveMesh = true;
+        } else if (cmd == "mesh_bfs") {
+            int nxu, nxd, nyu, nyd;
+            double Lu, Ld, h, H, Re, yplus;
+            ls >> nxu >> nxd >> nyu >> nyd >> Lu >> Ld >> h >> H >> Re >> yplus;
+            mesh = Mesh::makeBackwardFacingStep2D(nxu, nxd, nyu, nyd, Lu, Ld, h, H, Re, yplus);
+            haveMesh = true;
+        } else if (cmd == "patch_type") {
+            std::string name, type;
+            ls >> name >> type;
+            mesh.setPatchType(name, type);
         } else if (cmd == "save_mesh") {
             ls >> savePath;
         } else if (cmd == "nu") {

@@ -108,6 +108,37 @@ def plate_problem():
                          omega_init=d["omega_in"])
 
 
+def bfs_problem():
+    """Backward-facing step of ref_bfs.json: the reference's
+    makeBackwardFacingStep2D(20, 24, 12, 9, 10, 22, 1, 6, 5100, 1) mesh with a
+    symmetry top (MSH2 fixture), bfsDefaults (the channel defaults), the
+    settings of the reference's Le-Moin-Kim regression with PCG pressure and
+    600 capped iterations, pressure taps and skin friction along the lower
+    wall behind the step, and a velocity profile at x 4."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    d = load("ref_bfs.json")
+    mesh = Mesh.load_binary(os.path.join(GOLDEN, "ref_bfs_20x24.msh"))
+    mesh.compute_wall_distance()
+    bcs = FlowBoundaryConditions.channel_defaults(mesh, d["u_bulk"], d["k_in"], d["omega_in"])
+    s = SolverSettings(max_iterations=d["max_iterations"], convergence_tol=1e-4,
+                       divergence_limit=1e8, alpha_u=0.3, alpha_p=0.2, alpha_k=0.4,
+                       alpha_omega=0.4, inner_iterations=300, inner_tolerance=1e-4,
+                       turb_start_iter=30, turb_update_interval=2, pressure_solver="PCG")
+    obs = ObservationOperator()
+    for x in d["taps"]:
+        obs.add_pressure_tap((x, 0.0), 0.0, 0.01, d["u_bulk"], 0.0, 0.0, 1.0)
+    for x in d["taps"]:
+        obs.add_skin_friction("bottom_wall_down", (x, 0.0), 0.0, 1e-3, d["u_bulk"])
+    for y in d["profile_y"]:
+        obs.add_velocity_profile((4.0, y), 0, 0.0, 0.01)
+    return cases.Problem(mesh=mesh, bcs=bcs, settings=s, obs=obs,
+                         params=InferenceParameterSet.a1_betaStar(), nu=d["nu"],
+                         u_init=Vec3(d["u_bulk"], 0.0, 0.0), p_init=0.0, k_init=d["k_in"],
+                         omega_init=d["omega_in"])
+
+
 def symmetry_channel():
     """1x32 geometric channel (ratio 1.15) whose top patch is a symmetry
     plane, 5000 capped iterations at tol 1e-6."""

@@ -12,6 +12,7 @@ stores inputs and outputs as small JSON files. It needs the reference sources
     python tests/golden/make_fixtures.py body_force   # body-force fixture only
     python tests/golden/make_fixtures.py couette      # moving-wall Couette fixture only
     python tests/golden/make_fixtures.py plate        # free-stream flat-plate fixture only
+    python tests/golden/make_fixtures.py bfs          # backward-facing-step fixture only
 """
 from __future__ import annotations
 
@@ -404,8 +405,46 @@ def plate_fixture(tmp):
                              "thetas": [[0.31, 0.09], [0.3, 0.088]], "rows": res["rows"]})
 
 
+def bfs_fixture(tmp):
+    """Backward-facing step with a slip top: the reference's
+    makeBackwardFacingStep2D mesh (an L-shaped block mesh with a step face and
+    a re-entrant corner), top_wall relabelled a symmetry plane, bfsDefaults,
+    set up as the reference's Le-Moin-Kim regression
+    (tests/tools/record_goldens.cpp:463-497: Re_h 5100, Tu 5 %, relaxation,
+    inner solves, turbulence start) on a coarser mesh, with PCG pressure,
+    capped at 600 outer iterations, pressure taps, skin friction and a
+    velocity profile behind the step. The mesh is kept as an MSH2 fixture."""
+    h, big_h, re, ub = 1.0, 6.0, 5100.0, 1.0
+    nu = ub * h / re
+    k_in = 1.5 * (ub * 0.05) * (ub * 0.05)
+    om_in = k_in / (nu * 100.0)
+    msh = os.path.join(HERE, "ref_bfs_20x24.msh")
+    taps = [1.0, 2.0, 4.0, 6.0, 8.0, 10.0, 15.0]
+    prof = [0.05, 0.25, 0.5, 1.0, 2.0, 4.0]
+    lines = [f"mesh_bfs 20 24 12 9 10.0 22.0 {h!r} {big_h!r} {re!r} 1.0",
+             "patch_type top_wall symmetry", f"save_mesh {msh}", f"nu {nu!r}",
+             f"bcs_channel {ub!r} {k_in!r} {om_in!r}", "set max_iterations 600",
+             "set convergence_tol 0.0001", "set divergence_limit 100000000.0",
+             "set alpha_u 0.3", "set alpha_p 0.2", "set alpha_k 0.4", "set alpha_omega 0.4",
+             "set inner_iterations 300", "set inner_tolerance 0.0001", "set turb_start_iter 30",
+             "set turb_update_interval 2", "set pressure_solver PCG",
+             f"init {ub!r} 0 0 0 {k_in!r} {om_in!r}"]
+    lines += [f"obs_p {x!r} 0.0 0.0 0.01 {ub!r} 0.0 1.0" for x in taps]
+    lines += [f"obs_cf bottom_wall_down {x!r} 0.0 0.0 0.001 {ub!r}" for x in taps]
+    lines += [f"obs_u 4.0 {y!r} 0 0.0 0.01" for y in prof]
+    lines += ["params 9 8", "theta 0.31 0.09", f"dump {os.path.join(tmp, 'bfs')}"]
+    res = run_case(lines)
+    res["rows"][0]["fields"] = read_dump(os.path.join(tmp, "bfs_0.bin"))
+    write("ref_bfs.json", {"case": "backward-facing step on ref_bfs_20x24.msh "
+                                   "(makeBackwardFacingStep2D(20, 24, 12, 9, 10, 22, 1, 6, 5100, 1), "
+                                   "top_wall symmetry), bfsDefaults, PCG pressure, 600 iterations",
+                           "nu": nu, "u_bulk": ub, "k_in": k_in, "omega_in": om_in,
+                           "max_iterations": 600, "taps": taps, "profile_y": prof,
+                           "theta": [0.31, 0.09], "row": res["rows"][0]})
+
+
 EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
-         "plate": plate_fixture}
+         "plate": plate_fixture, "bfs": bfs_fixture}
 
 if __name__ == "__main__":
     if len(sys.argv) == 2 and sys.argv[1] in EXTRA:

@@ -29,8 +29,8 @@ import oracle_py
 from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
-from fixtures import (FIELD_NAMES, couette_problem, inout_channel, load, max_rel, plate_problem,
-                      small_channel, small_hill, symmetry_channel)
+from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, inout_channel, load, max_rel,
+                      plate_problem, small_channel, small_hill, symmetry_channel)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -386,6 +386,19 @@ def test_free_stream_flat_plate():
     assert np.all(np.asarray(f["Uxb"])[b] == d["u_inf"]) and np.all(np.asarray(f["Uyb"])[b] == 0)
     assert np.all(np.asarray(f["kb"])[b] == d["k_in"])
     assert np.all(np.asarray(f["omegab"])[b] == d["omega_in"])
+
+
+def test_backward_facing_step():
+    """The reference's backward-facing-step mesh (step face, re-entrant
+    corner, symmetry top, inlet and outlet), 600 capped iterations: the row
+    and every field equal the reference's."""
+    d = load("ref_bfs.json")
+    prob = bfs_problem()
+    assert {p.name: p.type for p in prob.mesh.patches}["top_wall"] == "symmetry"
+    r = _oracle(prob).evaluate(prob.coefficients(d["theta"]), fields=True)
+    _check_row(r, d["row"])
+    for name in FIELD_NAMES + ["flux"]:
+        assert np.array_equal(r[name], np.asarray(d["row"]["fields"][name])), name
 
 
 def test_reference_couette_golden():
