@@ -26,6 +26,7 @@ BC_DIRICHLET, BC_NEUMANN, BC_INLET, BC_OUTLET, BC_WALL_NOSLIP, BC_WALL_KOMEGA, \
 SOLVERS = {"PCG": 0, "BiCGSTAB": 1, "AMG": 2}
 SCHEMES = {"upwind": 0, "linearUpwind": 1, "limitedLinearUpwind": 2}
 OBS_VELOCITY, OBS_PRESSURE, OBS_SKIN_FRICTION = 0, 1, 2
+OBS_SEPARATION, OBS_REATTACHMENT = 4, 5
 PATCH_TYPES_ORDER = {"wall": 0, "inlet": 1, "outlet": 2, "symmetry": 3}
 
 CONVERGED, UNCONVERGED, DIVERGED, INVALID_PARAMETERS, UNKNOWN = range(5)
@@ -364,8 +365,8 @@ class Observable:
 
 class ObservationOperator:
     """ObservationOperator (ObservationOperator.hpp:65-150): velocity probes,
-    pressure taps and skin friction, the observables of the forward-solve
-    configurations."""
+    pressure taps, skin friction, separation points and reattachment lengths
+    (drag is not supported)."""
 
     def __init__(self):
         self.observables: list[Observable] = []
@@ -412,6 +413,23 @@ class ObservationOperator:
         if not math.isfinite(ref_vel) or ref_vel <= 0:
             raise ValueError("friction reference speed must be finite and positive")
         ob = Observable(OBS_SKIN_FRICTION, cf_obs, sigma, sigma_model, loc, 0, patch, ref_vel)
+        ob.validate()
+        self.observables.append(ob)
+
+    def add_separation_point(self, patch: str, x_obs: float, sigma: float,
+                             sigma_model: float = 0.0):
+        """addSeparationPoint (ObservationOperator.hpp:251-259): the x where
+        the streamwise wall shear on `patch` first turns non-positive."""
+        ob = Observable(OBS_SEPARATION, x_obs, sigma, sigma_model, patch_name=patch)
+        ob.validate()
+        self.observables.append(ob)
+
+    def add_reattachment_length(self, patch: str, x_obs: float, sigma: float,
+                                sigma_model: float = 0.0):
+        """addReattachmentLength (ObservationOperator.hpp:261-269): the x
+        where the streamwise wall shear on `patch` first turns non-negative
+        (the last face centre when it never does)."""
+        ob = Observable(OBS_REATTACHMENT, x_obs, sigma, sigma_model, patch_name=patch)
         ob.validate()
         self.observables.append(ob)
 
@@ -578,6 +596,17 @@ class PreparedObservation:
                 self.det[i] = axx * ayy - axy * axy
             elif ob.type == OBS_SKIN_FRICTION:
                 self.face_a[i], self.face_b[i], self.t[i] = _friction_stencil(mesh, ob)
+            elif ob.type in (OBS_SEPARATION, OBS_REATTACHMENT):
+                # the patch's faces in patch order (ObservationOperator.hpp:665)
+                pat = [p for p in mesh.patches if p.name == ob.patch_name]
+                if not pat:
+                    raise ValueError(f"unknown patch '{ob.patch_name}'")
+                for fi in pat[0].faces:
+                    nb_cell.append(-1)
+                    nb_face.append(int(fi))
+                    nb_dx.append(float(mesh.face_center[fi][0]))
+                    nb_dy.append(0.0)
+                    nb_w.append(0.0)
             nb_ptr.append(len(nb_cell))
         self.nb_ptr = np.asarray(nb_ptr, np.int32)
         self.nb_cell = np.asarray(nb_cell, np.int32)

@@ -16,7 +16,13 @@ enum PatchType { PATCH_WALL = 0, PATCH_INLET, PATCH_OUTLET, PATCH_SYMMETRY, PATC
 enum SolverKind { SOLVER_PCG = 0, SOLVER_BICGSTAB = 1, SOLVER_AMG = 2 };
 enum Scheme { CONV_UPWIND = 0, CONV_LINEAR_UPWIND = 1, CONV_LIMITED = 2 };
 enum Status { ST_CONVERGED = 0, ST_UNCONVERGED, ST_DIVERGED, ST_INVALID, ST_UNKNOWN };
-enum ObsKind { OBS_VELOCITY = 0, OBS_PRESSURE = 1, OBS_SKIN_FRICTION = 2 };
+enum ObsKind {
+    OBS_VELOCITY = 0,
+    OBS_PRESSURE = 1,
+    OBS_SKIN_FRICTION = 2,
+    OBS_SEPARATION = 4,    // x where the wall shear first turns non-positive
+    OBS_REATTACHMENT = 5,  // x where the wall shear first turns non-negative
+};
 // device error codes (mapped to the reference's exception texts on the host)
 enum DevError { ERR_NONE = 0, ERR_NONFINITE_SYSTEM = 1, ERR_NONFINITE_MONITOR = 2,
                 ERR_GRID_TIMEOUT = 3 };

@@ -2376,6 +2376,37 @@ struct Draw {
             const double dynP = .5 * R[O_refrho] * refv * refv;
             return pressure ? (value - R[O_refp]) / smax(dynP, 1e-20) : value;
         }
+        // The mixed-precision multi-workgroup kernel carries no wall-crossing
+        // observations: the host runs problems that have them on one
+        // workgroup per draw in mixed precision.
+        if constexpr (!(kGrid && kF32)) if (type == OBS_SEPARATION || type == OBS_REATTACHMENT) {
+            // first sign change of the molecular streamwise wall shear along
+            // the patch, interpolated linearly between face centres
+            const bool sep = type == OBS_SEPARATION;
+            const int q0 = Q[2], q1 = Q[3];
+            double value = (sep || q1 == q0) ? 0.0 : Onb[3 * (q1 - 1)];
+            bool have = false;
+            double prevTau = sep ? 1.0 : -1.0, prevX = 0.0;
+            for (int qq = q0; qq < q1; ++qq) {
+                const int f = onbi[2 * qq + 1];
+                const int ow = fon[f].x;
+                const double nx = Fc(f, W_nx), ny = Fc(f, W_ny);
+                const double ux = C(ow, V_Ux);
+                const double un = ux * nx + C(ow, V_Uy) * ny;
+                const double tau = S.nu * (ux - nx * un) / Fc(f, W_dlt);
+                const double x = Onb[3 * qq];
+                if (have && (sep ? (prevTau > 0 && tau <= 0) : (prevTau < 0 && tau >= 0))) {
+                    const double den = sep ? prevTau - tau : tau - prevTau;
+                    const double frac = fabs(den) > 1e-30 ? (sep ? prevTau : -prevTau) / den : 0.0;
+                    value = prevX + frac * (x - prevX);
+                    break;
+                }
+                prevTau = tau;
+                prevX = x;
+                have = true;
+            }
+            return value;
+        }
         double value = 0.0;
         for (int pass = 0; pass < 2; ++pass) {
             const int f = pass == 0 ? Q[4] : Q[5];

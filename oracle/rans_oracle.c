@@ -1378,8 +1378,8 @@ static void constrain_omega(ctx_t *X) {
 }
 
 /* PreparedObservation::evaluate (ObservationOperator.hpp:678-772) for
- * velocity probes, pressure taps and skin friction. Returns nonzero if a
- * requested kind is not supported. */
+ * velocity probes, pressure taps, skin friction, separation points and
+ * reattachment lengths. */
 static void observe(const ctx_t *X, double *out) {
     const or_obs_t *ob = X->obs;
     const or_mesh_t *m = X->m;
@@ -1409,6 +1409,31 @@ static void observe(const ctx_t *X, double *out) {
             const double value = center + gxv * ob->offx[i] + gyv * ob->offy[i];
             const double dynP = .5 * ob->ref_density[i] * ob->ref_velocity[i] * ob->ref_velocity[i];
             out[i] = pressure ? (value - ob->ref_pressure[i]) / smax(dynP, 1e-20) : value;
+        } else if (type == OR_OBS_SEPARATION || type == OR_OBS_REATTACHMENT) {
+            /* computeSeparationPoint / computeReattachmentLength
+             * (ObservationOperator.hpp:485-552, prepared form :741-767) */
+            const int sep = type == OR_OBS_SEPARATION;
+            const int q0 = ob->nb_ptr[i], q1 = ob->nb_ptr[i + 1];
+            double value = (sep || q1 == q0) ? 0.0 : ob->nb_dx[q1 - 1];
+            int have = 0;
+            double prevTau = sep ? 1.0 : -1.0, prevX = 0.0;
+            for (int q = q0; q < q1; ++q) {
+                const int fi = ob->nb_face[q], ow = m->owner[fi];
+                const double nx = m->fnx[fi], ny = m->fny[fi];
+                const double un = X->Ux[ow] * nx + X->Uy[ow] * ny;
+                const double tau = X->nu * (X->Ux[ow] - nx * un) / smax(m->fdelta[fi], 1e-20);
+                const double x = ob->nb_dx[q];
+                if (have && (sep ? (prevTau > 0 && tau <= 0) : (prevTau < 0 && tau >= 0))) {
+                    const double den = sep ? prevTau - tau : tau - prevTau;
+                    const double frac = fabs(den) > 1e-30 ? (sep ? prevTau : -prevTau) / den : 0.0;
+                    value = prevX + frac * (x - prevX);
+                    break;
+                }
+                prevTau = tau;
+                prevX = x;
+                have = 1;
+            }
+            out[i] = value;
         } else {
             double value = 0;
             const int fa = ob->face_a[i], fb = ob->face_b[i];

@@ -30,7 +30,13 @@ enum { OR_CONV_UPWIND = 0, OR_CONV_LINEAR_UPWIND = 1, OR_CONV_LIMITED = 2 };
 /* Evaluation status (core/include/EvaluationTypes.hpp:6). */
 enum { OR_CONVERGED = 0, OR_UNCONVERGED = 1, OR_DIVERGED = 2, OR_INVALID = 3, OR_UNKNOWN = 4 };
 /* Observation kinds handled (ObservationOperator.hpp:33-40 subset). */
-enum { OR_OBS_VELOCITY = 0, OR_OBS_PRESSURE = 1, OR_OBS_SKIN_FRICTION = 2 };
+enum {
+    OR_OBS_VELOCITY = 0,
+    OR_OBS_PRESSURE = 1,
+    OR_OBS_SKIN_FRICTION = 2,
+    OR_OBS_SEPARATION = 4,
+    OR_OBS_REATTACHMENT = 5
+};
 
 typedef struct {
     int n_cells, n_internal, n_faces, n_patches;

@@ -26,6 +26,8 @@
 //   obs_u x y comp observed sigma
 //   obs_cf patch x y observed sigma refVel
 //   obs_p x y observed sigma refVel refP refRho
+//   obs_sep patch observed sigma                separation point
+//   obs_reatt patch observed sigma              reattachment length
 //   params idx...
 //   theta v...                                 one coefficient row (repeatable)
 //   dump <prefix>                              binary field dump per row
@@ -223,6 +225,12 @@ int main(int argc, char **argv) {
             double x, y, o, sg, rv, rp, rr;
             ls >> x >> y >> o >> sg >> rv >> rp >> rr;
             obs.addPressureTap(Vec3(x, y, 0.0), o, sg, rv, 0.0, rp, rr);
+        } else if (cmd == "obs_sep" || cmd == "obs_reatt") {
+            std::string patch;
+            double o, sg;
+            ls >> patch >> o >> sg;
+            if (cmd == "obs_sep") obs.addSeparationPoint(patch, o, sg);
+            else obs.addReattachmentLength(patch, o, sg);
         } else if (cmd == "params") {
             int v;
             while (ls >> v) params.push_back(v);

@@ -139,6 +139,39 @@ def bfs_problem():
                          omega_init=d["omega_in"])
 
 
+def crossings_problems():
+    """The two problems of ref_wall_crossings.json: the periodic cosine bump
+    (separation point, reattachment length and a velocity probe) and the
+    backward-facing step of bfs_problem() observed by a reattachment length,
+    two separation points and a velocity probe."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    d = load("ref_wall_crossings.json")
+    c = d["cosine_bump"]
+    mesh = Mesh.make_curved_channel_periodic_2d(c["x_nodes"], c["y_bottom"], c["y_top"], c["ny"],
+                                                0.0, 1.0)
+    mesh.compute_wall_distance()
+    s = SolverSettings(max_iterations=1500, convergence_tol=1e-4, alpha_u=0.4, alpha_p=0.25,
+                       body_force=Vec3(0.004, 0.0, 0.0), pressure_solver="PCG")
+    obs = ObservationOperator()
+    obs.add_separation_point("bottom_wall", 0.0, 0.1)
+    obs.add_reattachment_length("bottom_wall", 0.0, 0.1)
+    obs.add_velocity_profile((2.25, 1.5), 0, 0.5, 0.1)
+    bump = cases.Problem(mesh=mesh, bcs=FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1e-4, 10.0),
+                         settings=s, obs=obs, params=InferenceParameterSet.a1_betaStar(),
+                         nu=c["nu"], u_init=Vec3(0.3, 0.0, 0.0), p_init=0.0, k_init=1e-4,
+                         omega_init=10.0)
+    step = bfs_problem()
+    obs = ObservationOperator()
+    obs.add_reattachment_length("bottom_wall_down", 6.28, 0.5)
+    obs.add_separation_point("bottom_wall_down", 0.0, 0.5)
+    obs.add_separation_point("bottom_wall_up", 0.0, 0.5)
+    obs.add_velocity_profile((4.0, 0.05), 0, 0.0, 0.01)
+    step.obs = obs
+    return {"cosine_bump": bump, "bfs": step}
+
+
 def symmetry_channel():
     """1x32 geometric channel (ratio 1.15) whose top patch is a symmetry
     plane, 5000 capped iterations at tol 1e-6."""

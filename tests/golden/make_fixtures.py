@@ -13,6 +13,7 @@ stores inputs and outputs as small JSON files. It needs the reference sources
     python tests/golden/make_fixtures.py couette      # moving-wall Couette fixture only
     python tests/golden/make_fixtures.py plate        # free-stream flat-plate fixture only
     python tests/golden/make_fixtures.py bfs          # backward-facing-step fixture only
+    python tests/golden/make_fixtures.py crossings    # separation / reattachment fixture
 """
 from __future__ import annotations
 
@@ -443,8 +444,52 @@ def bfs_fixture(tmp):
                            "theta": [0.31, 0.09], "row": res["rows"][0]})
 
 
+def crossings_fixture(tmp):
+    """Separation-point and reattachment-length observations
+    (ObservationOperator.hpp:485-552): the periodic cosine bump of the
+    reference's regression (tests/tools/record_goldens.cpp:630-656: 36x24,
+    Lx 4.5, bump height 1, nu 3e-4, body force 4e-3, alpha_u 0.4, alpha_p
+    0.25, init U 0.3) with PCG pressure capped at 1500 iterations, and the
+    backward-facing step of ref_bfs.json; each watches its lower wall and
+    carries a velocity probe beside it."""
+    out = {}
+    nx, ny, lx, y_top, bump, nu = 36, 24, 4.5, 3.0, 1.0, 3.0e-4
+    xs = [lx * i / nx for i in range(nx + 1)]
+    ys = [0.5 * bump * (1.0 + math.cos(2.0 * math.pi * x / lx)) for x in xs]
+    cb = os.path.join(tmp, "bump.msh")
+    lines = ["mesh_curved %d " % (nx + 1) + " ".join("%r %r" % (x, y) for x, y in zip(xs, ys))
+             + f" {y_top!r} {ny} 0.0 1.0", f"save_mesh {cb}", f"nu {nu!r}",
+             "bcs_channel 1.0 0.0001 10.0", "set max_iterations 1500",
+             "set convergence_tol 0.0001", "set alpha_u 0.4", "set alpha_p 0.25",
+             "set body_force_x 0.004", "set pressure_solver PCG", "init 0.3 0 0 0 0.0001 10.0",
+             "obs_sep bottom_wall 0.0 0.1", "obs_reatt bottom_wall 0.0 0.1",
+             "obs_u 2.25 1.5 0 0.5 0.1", "params 9 8", "theta 0.31 0.09", "theta 0.33 0.095"]
+    res = run_case(lines)
+    out["cosine_bump"] = {"case": "cosine bump 36x24 (makeCurvedChannelPeriodic2D, Re 0, y+ 1), "
+                                  "PCG pressure, 1500 capped iterations",
+                          "x_nodes": xs, "y_bottom": ys, "y_top": y_top, "ny": ny, "nu": nu,
+                          "thetas": [[0.31, 0.09], [0.33, 0.095]], "rows": res["rows"]}
+    d = json.load(open(os.path.join(HERE, "ref_bfs.json")))
+    lines = [f"mesh_file {os.path.join(HERE, 'ref_bfs_20x24.msh')}", f"nu {d['nu']!r}",
+             f"bcs_channel {d['u_bulk']!r} {d['k_in']!r} {d['omega_in']!r}",
+             "set max_iterations 600", "set convergence_tol 0.0001",
+             "set divergence_limit 100000000.0", "set alpha_u 0.3", "set alpha_p 0.2",
+             "set alpha_k 0.4", "set alpha_omega 0.4", "set inner_iterations 300",
+             "set inner_tolerance 0.0001", "set turb_start_iter 30", "set turb_update_interval 2",
+             "set pressure_solver PCG",
+             f"init {d['u_bulk']!r} 0 0 0 {d['k_in']!r} {d['omega_in']!r}",
+             "obs_reatt bottom_wall_down 6.28 0.5", "obs_sep bottom_wall_down 0.0 0.5",
+             "obs_sep bottom_wall_up 0.0 0.5", "obs_u 4.0 0.05 0 0.0 0.01",
+             "params 9 8", "theta 0.31 0.09"]
+    res = run_case(lines)
+    out["bfs"] = {"case": "ref_bfs.json's problem with reattachment and separation "
+                          "observations in place of its taps and profile",
+                  "thetas": [[0.31, 0.09]], "rows": res["rows"]}
+    write("ref_wall_crossings.json", out)
+
+
 EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
-         "plate": plate_fixture, "bfs": bfs_fixture}
+         "plate": plate_fixture, "bfs": bfs_fixture, "crossings": crossings_fixture}
 
 if __name__ == "__main__":
     if len(sys.argv) == 2 and sys.argv[1] in EXTRA:

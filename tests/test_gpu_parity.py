@@ -20,8 +20,9 @@ import numpy as np
 import pytest
 
 from cfdr_amd import cases, engine
-from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, inout_channel, load, max_rel,
-                      max_scaled, plate_problem, small_channel, small_hill, symmetry_channel)
+from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, crossings_problems,
+                      inout_channel, load, max_rel, max_scaled, plate_problem, small_channel,
+                      small_hill, symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -251,6 +252,33 @@ def test_backward_facing_step_matches_reference(grid_blocks):
         assert res.statuses[0] == d["row"]["status"]
         assert res.iteration_counts[0] == d["row"]["iters"]
         assert max_scaled(res.predictions[0], d["row"]["predictions"]) <= 1e-4
+
+
+@pytest.mark.parametrize("case", ["cosine_bump", "bfs"])
+def test_separation_and_reattachment_match_reference(case):
+    """Separation-point and reattachment-length observations on the cosine
+    bump (arena in HBM) and the backward-facing step, capped runs: the rows
+    follow the reference's, these observations included."""
+    d = load("ref_wall_crossings.json")[case]
+    prob = crossings_problems()[case]
+    res = _batch(prob).evaluate(np.asarray(d["thetas"]))
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+    if case == "bfs":
+        # four workgroups per draw in FP64 evaluate the crossings in the
+        # multi-workgroup kernel; in mixed precision such a problem runs on
+        # one workgroup per draw, with the capped status and count kept and
+        # probes within 1e-4 of the FP64 reference
+        b = _batch(prob, grid_blocks=4)
+        res = b.evaluate(np.asarray(d["thetas"]))
+        assert b.last_timing()["grid_blocks"] == 4
+        _check_row(res, 0, d["rows"][0])
+        b = _batch(prob, grid_blocks=4, mixed_precision=True)
+        res = b.evaluate(np.asarray(d["thetas"]))
+        assert b.last_timing()["grid_blocks"] == 1
+        assert res.statuses[0] == d["rows"][0]["status"]
+        assert res.iteration_counts[0] == d["rows"][0]["iters"]
+        assert max_scaled(res.predictions[0], d["rows"][0]["predictions"]) <= 1e-4
 
 
 def test_reference_couette_golden_amg():

@@ -29,8 +29,9 @@ import oracle_py
 from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
-from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, inout_channel, load, max_rel,
-                      plate_problem, small_channel, small_hill, symmetry_channel)
+from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, crossings_problems,
+                      inout_channel, load, max_rel, plate_problem, small_channel, small_hill,
+                      symmetry_channel)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -399,6 +400,25 @@ def test_backward_facing_step():
     _check_row(r, d["row"])
     for name in FIELD_NAMES + ["flux"]:
         assert np.array_equal(r[name], np.asarray(d["row"]["fields"][name])), name
+
+
+@pytest.mark.parametrize("case", ["cosine_bump", "bfs"])
+def test_separation_and_reattachment(case):
+    """Separation-point and reattachment-length observations on the cosine
+    bump and the backward-facing step: rows (these observations included)
+    equal the reference's; behind the step the bubble reattaches inside the
+    domain, and the wall before the step never separates (the value 0)."""
+    d = load("ref_wall_crossings.json")[case]
+    prob = crossings_problems()[case]
+    P = _oracle(prob)
+    for i, (theta, row) in enumerate(zip(d["thetas"], d["rows"])):
+        r = P.evaluate(prob.coefficients(theta), fields=i == 0)
+        _check_row(r, row)
+    if case == "bfs":
+        # a recirculation bubble behind the step, reattaching within the
+        # domain, and no separation on the wall before the step
+        x_r = r["predictions"][0]
+        assert 1.0 < x_r < 22.0 and r["predictions"][2] == 0.0
 
 
 def test_reference_couette_golden():
