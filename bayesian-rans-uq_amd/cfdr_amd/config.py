@@ -26,7 +26,7 @@ BC_DIRICHLET, BC_NEUMANN, BC_INLET, BC_OUTLET, BC_WALL_NOSLIP, BC_WALL_KOMEGA, \
 SOLVERS = {"PCG": 0, "BiCGSTAB": 1, "AMG": 2}
 SCHEMES = {"upwind": 0, "linearUpwind": 1, "limitedLinearUpwind": 2}
 OBS_VELOCITY, OBS_PRESSURE, OBS_SKIN_FRICTION = 0, 1, 2
-OBS_SEPARATION, OBS_REATTACHMENT = 4, 5
+OBS_DRAG, OBS_SEPARATION, OBS_REATTACHMENT = 3, 4, 5
 PATCH_TYPES_ORDER = {"wall": 0, "inlet": 1, "outlet": 2, "symmetry": 3}
 
 CONVERGED, UNCONVERGED, DIVERGED, INVALID_PARAMETERS, UNKNOWN = range(5)
@@ -350,6 +350,7 @@ class Observable:
     ref_velocity: float = 1.0
     ref_pressure: float = 0.0
     ref_density: float = 1.0
+    reference_area: float = 1.0
 
     def total_variance(self) -> float:
         return self.sigma_obs * self.sigma_obs + self.sigma_model * self.sigma_model
@@ -365,8 +366,8 @@ class Observable:
 
 class ObservationOperator:
     """ObservationOperator (ObservationOperator.hpp:65-150): velocity probes,
-    pressure taps, skin friction, separation points and reattachment lengths
-    (drag is not supported)."""
+    pressure taps, skin friction, drag, separation points and reattachment
+    lengths."""
 
     def __init__(self):
         self.observables: list[Observable] = []
@@ -413,6 +414,19 @@ class ObservationOperator:
         if not math.isfinite(ref_vel) or ref_vel <= 0:
             raise ValueError("friction reference speed must be finite and positive")
         ob = Observable(OBS_SKIN_FRICTION, cf_obs, sigma, sigma_model, loc, 0, patch, ref_vel)
+        ob.validate()
+        self.observables.append(ob)
+
+    def add_drag(self, patch: str, cd_obs: float, sigma: float, ref_area: float, ref_vel: float,
+                 sigma_model: float = 0.0):
+        """addDrag (ObservationOperator.hpp:189-201): the pressure and
+        viscous x force on `patch` over 0.5 ref_vel^2 ref_area."""
+        if not math.isfinite(ref_area) or ref_area <= 0:
+            raise ValueError("drag reference area must be finite and positive")
+        if not math.isfinite(ref_vel) or ref_vel <= 0:
+            raise ValueError("drag reference speed must be finite and positive")
+        ob = Observable(OBS_DRAG, cd_obs, sigma, sigma_model, patch_name=patch,
+                        ref_velocity=ref_vel, reference_area=ref_area)
         ob.validate()
         self.observables.append(ob)
 
@@ -596,8 +610,9 @@ class PreparedObservation:
                 self.det[i] = axx * ayy - axy * axy
             elif ob.type == OBS_SKIN_FRICTION:
                 self.face_a[i], self.face_b[i], self.t[i] = _friction_stencil(mesh, ob)
-            elif ob.type in (OBS_SEPARATION, OBS_REATTACHMENT):
+            elif ob.type in (OBS_DRAG, OBS_SEPARATION, OBS_REATTACHMENT):
                 # the patch's faces in patch order (ObservationOperator.hpp:665)
+                self.offx[i] = ob.reference_area
                 pat = [p for p in mesh.patches if p.name == ob.patch_name]
                 if not pat:
                     raise ValueError(f"unknown patch '{ob.patch_name}'")

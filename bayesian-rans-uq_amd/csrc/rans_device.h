@@ -20,6 +20,7 @@ enum ObsKind {
     OBS_VELOCITY = 0,
     OBS_PRESSURE = 1,
     OBS_SKIN_FRICTION = 2,
+    OBS_DRAG = 3,          // pressure plus viscous x force on a patch, as Cd
     OBS_SEPARATION = 4,    // x where the wall shear first turns non-positive
     OBS_REATTACHMENT = 5,  // x where the wall shear first turns non-negative
 };

@@ -85,7 +85,8 @@ struct rq_engine {
     // owned problem copy
     int nc = 0, nif = 0, nf = 0, nb = 0, npatch = 0, K = 0;
     int n_active = 0, nobs = 0;
-    bool wall_crossings = false;  // separation / reattachment observations present
+    bool wall_crossings = false;  // separation or reattachment observations present
+    bool drag = false;            // drag observations present
     std::vector<int> active;
     std::vector<double> base, lower, upper, observed, variance;
     DevSettings settings{};
@@ -655,9 +656,11 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         const int nnb = no > 0 ? O.nb_ptr[no] : 0;
         for (int i = 0; i < no; ++i) {
             const int t = O.type[i];
-            if (t < 0 || t > 5 || t == 3) throw Fail{RQ_EINVAL, "unsupported observation type"};
+            if (t < 0 || t > 5) throw Fail{RQ_EINVAL, "unsupported observation type"};
+            const bool scan = t == OBS_DRAG || t == OBS_SEPARATION || t == OBS_REATTACHMENT;
+            if (t == OBS_DRAG) E->drag = true;
             if (t == OBS_SEPARATION || t == OBS_REATTACHMENT) E->wall_crossings = true;
-            if (t == OBS_SEPARATION || t == OBS_REATTACHMENT)
+            if (scan)
                 for (int q = O.nb_ptr[i]; q < O.nb_ptr[i + 1]; ++q)
                     if (O.nb_cell[q] >= 0)
                         throw Fail{RQ_EINVAL, "wall-crossing stencil must list boundary faces"};
@@ -1159,9 +1162,11 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         nblk = std::min(nblk, fit);
         if (nblk < 2) nblk = 1;
     }
-    // the mixed-precision multi-workgroup kernel does not evaluate wall-crossing
-    // observations: such problems run one workgroup per draw in mixed precision
-    if (nblk > 1 && E->settings.mixed_precision != 0 && E->wall_crossings) nblk = 1;
+    // the multi-workgroup kernels do not evaluate drag, and the mixed-precision
+    // one neither separation nor reattachment: such problems run one
+    // workgroup per draw
+    if (nblk > 1 && (E->drag || (E->settings.mixed_precision != 0 && E->wall_crossings)))
+        nblk = 1;
     if (nblk > 1 && E->lds) {
         // the grid mode keeps the arena in HBM
         grow(E->d_arena, E->cap_arena, (size_t)ns * E->lay.stride);

@@ -2376,9 +2376,24 @@ struct Draw {
             const double dynP = .5 * R[O_refrho] * refv * refv;
             return pressure ? (value - R[O_refp]) / smax(dynP, 1e-20) : value;
         }
-        // The mixed-precision multi-workgroup kernel carries no wall-crossing
-        // observations: the host runs problems that have them on one
-        // workgroup per draw in mixed precision.
+        // The multi-workgroup kernels carry no drag observation, and the
+        // mixed-precision one no wall-crossing observations either: the host
+        // runs problems that have them on one workgroup per draw.
+        if constexpr (!kGrid) if (type == OBS_DRAG) {
+            // pressure and viscous x force over the patch faces, as Cd
+            double force = 0.0;
+            for (int qq = Q[2]; qq < Q[3]; ++qq) {
+                const int f = onbi[2 * qq + 1];
+                const int ow = fon[f].x;
+                const double area = Fc(f, W_A);
+                force += C(ow, V_p) * Fc(f, W_nx) * area;
+                const double tau = S.nu * (C(ow, V_Ux) - Bd(f - nif, B_Ux)) / Fc(f, W_dlt);
+                force += tau * area;
+            }
+            const double refv = R[O_refv];
+            const double dynP = .5 * refv * refv;
+            return force / (dynP * R[O_offx]);
+        }
         if constexpr (!(kGrid && kF32)) if (type == OBS_SEPARATION || type == OBS_REATTACHMENT) {
             // first sign change of the molecular streamwise wall shear along
             // the patch, interpolated linearly between face centres

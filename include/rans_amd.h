@@ -92,10 +92,11 @@ typedef struct rq_param_desc {
 } rq_param_desc;
 
 /* PreparedObservation stencils (core/include/ObservationOperator.hpp:611-669):
- * type 0 velocity probe, 1 pressure tap, 2 skin friction, 4 separation point,
- * 5 reattachment length (3, drag, is not supported). For types 4 and 5 the
- * stencil nb_ptr[i]..nb_ptr[i+1] lists the wall patch's faces in patch order
- * (nb_cell -1, nb_face the boundary face, nb_dx its centre x). */
+ * type 0 velocity probe, 1 pressure tap, 2 skin friction, 3 drag, 4 separation
+ * point, 5 reattachment length. For types 3-5 the stencil
+ * nb_ptr[i]..nb_ptr[i+1] lists the patch's faces in patch order (nb_cell -1,
+ * nb_face the boundary face, nb_dx its centre x); drag takes its reference
+ * area in offx[i]. */
 typedef struct rq_obs_desc {
     int32_t n_obs;
     const int32_t *type, *component;

@@ -1378,7 +1378,7 @@ static void constrain_omega(ctx_t *X) {
 }
 
 /* PreparedObservation::evaluate (ObservationOperator.hpp:678-772) for
- * velocity probes, pressure taps, skin friction, separation points and
+ * velocity probes, pressure taps, skin friction, drag, separation points and
  * reattachment lengths. */
 static void observe(const ctx_t *X, double *out) {
     const or_obs_t *ob = X->obs;
@@ -1409,6 +1409,18 @@ static void observe(const ctx_t *X, double *out) {
             const double value = center + gxv * ob->offx[i] + gyv * ob->offy[i];
             const double dynP = .5 * ob->ref_density[i] * ob->ref_velocity[i] * ob->ref_velocity[i];
             out[i] = pressure ? (value - ob->ref_pressure[i]) / smax(dynP, 1e-20) : value;
+        } else if (type == OR_OBS_DRAG) {
+            /* computeDrag's prepared form (ObservationOperator.hpp:728-740) */
+            double force = 0;
+            for (int q = ob->nb_ptr[i]; q < ob->nb_ptr[i + 1]; ++q) {
+                const int fi = ob->nb_face[q], ow = m->owner[fi];
+                force += X->p[ow] * m->fnx[fi] * m->farea[fi];
+                const double delta = smax(m->fdelta[fi], 1e-20);
+                const double tau = X->nu * (X->Ux[ow] - X->Uxb[fi - X->nif]) / delta;
+                force += tau * m->farea[fi];
+            }
+            const double dynP = .5 * ob->ref_velocity[i] * ob->ref_velocity[i];
+            out[i] = force / (dynP * ob->offx[i]);
         } else if (type == OR_OBS_SEPARATION || type == OR_OBS_REATTACHMENT) {
             /* computeSeparationPoint / computeReattachmentLength
              * (ObservationOperator.hpp:485-552, prepared form :741-767) */

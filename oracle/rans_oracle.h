@@ -34,6 +34,7 @@ enum {
     OR_OBS_VELOCITY = 0,
     OR_OBS_PRESSURE = 1,
     OR_OBS_SKIN_FRICTION = 2,
+    OR_OBS_DRAG = 3,
     OR_OBS_SEPARATION = 4,
     OR_OBS_REATTACHMENT = 5
 };

@@ -26,6 +26,7 @@
 //   obs_u x y comp observed sigma
 //   obs_cf patch x y observed sigma refVel
 //   obs_p x y observed sigma refVel refP refRho
+//   obs_drag patch observed sigma refArea refVel drag coefficient
 //   obs_sep patch observed sigma                separation point
 //   obs_reatt patch observed sigma              reattachment length
 //   params idx...
@@ -225,6 +226,11 @@ int main(int argc, char **argv) {
             double x, y, o, sg, rv, rp, rr;
             ls >> x >> y >> o >> sg >> rv >> rp >> rr;
             obs.addPressureTap(Vec3(x, y, 0.0), o, sg, rv, 0.0, rp, rr);
+        } else if (cmd == "obs_drag") {
+            std::string patch;
+            double o, sg, ra, rv;
+            ls >> patch >> o >> sg >> ra >> rv;
+            obs.addDrag(patch, o, sg, ra, rv);
         } else if (cmd == "obs_sep" || cmd == "obs_reatt") {
             std::string patch;
             double o, sg;

@@ -172,6 +172,26 @@ def crossings_problems():
     return {"cosine_bump": bump, "bfs": step}
 
 
+def drag_problems():
+    """The two problems of ref_drag.json: the flat plate observed by the drag
+    of its plate and a velocity probe, and the backward-facing step observed
+    by the drag of its step face and of the lower wall behind it and a
+    velocity probe."""
+    from cfdr_amd.config import ObservationOperator
+    plate = plate_problem()
+    obs = ObservationOperator()
+    obs.add_drag("bottom", 0.004, 0.001, 1.0, 1.0)
+    obs.add_velocity_profile((0.5, 0.01), 0, 0.9, 0.01)
+    plate.obs = obs
+    step = bfs_problem()
+    obs = ObservationOperator()
+    obs.add_drag("step_face", 0.0, 0.1, 1.0, 1.0)
+    obs.add_drag("bottom_wall_down", 0.0, 0.01, 22.0, 1.0)
+    obs.add_velocity_profile((4.0, 0.05), 0, 0.0, 0.01)
+    step.obs = obs
+    return {"plate": plate, "bfs": step}
+
+
 def symmetry_channel():
     """1x32 geometric channel (ratio 1.15) whose top patch is a symmetry
     plane, 5000 capped iterations at tol 1e-6."""

@@ -14,6 +14,7 @@ stores inputs and outputs as small JSON files. It needs the reference sources
     python tests/golden/make_fixtures.py plate        # free-stream flat-plate fixture only
     python tests/golden/make_fixtures.py bfs          # backward-facing-step fixture only
     python tests/golden/make_fixtures.py crossings    # separation / reattachment fixture
+    python tests/golden/make_fixtures.py drag         # drag fixture
 """
 from __future__ import annotations
 
@@ -488,8 +489,45 @@ def crossings_fixture(tmp):
     write("ref_wall_crossings.json", out)
 
 
+def drag_fixture(tmp):
+    """Drag observations (addDrag, ObservationOperator.hpp:189-201; prepared
+    form :728-740): the flat plate of ref_plate.json with the drag of its
+    plate, and the backward-facing step of ref_bfs.json with the drag of the
+    step face (pressure only) and of the lower wall behind it."""
+    out = {}
+    d = json.load(open(os.path.join(HERE, "ref_plate.json")))
+    lines = [f"mesh_file {os.path.join(HERE, 'ref_plate_24x16.msh')}", f"nu {d['nu']!r}",
+             f"bcs_plate {d['u_inf']!r} {d['k_in']!r} {d['omega_in']!r}",
+             "set max_iterations 3000", "set convergence_tol 0.0001", "set alpha_u 0.7",
+             "set alpha_p 0.5", "set turb_start_iter 0", "set k_max 0.1",
+             "set pressure_solver PCG", f"init {d['u_inf']!r} 0 0 0 {d['k_in']!r} {d['omega_in']!r}",
+             "obs_drag bottom 0.004 0.001 1.0 1.0", "obs_u 0.5 0.01 0 0.9 0.01",
+             "params 9 8", "theta 0.31 0.09", "theta 0.3 0.088"]
+    res = run_case(lines)
+    out["plate"] = {"case": "ref_plate.json's problem observed by the drag of 'bottom' "
+                            "(area 1, speed 1) and one velocity probe",
+                    "thetas": [[0.31, 0.09], [0.3, 0.088]], "rows": res["rows"]}
+    d = json.load(open(os.path.join(HERE, "ref_bfs.json")))
+    lines = [f"mesh_file {os.path.join(HERE, 'ref_bfs_20x24.msh')}", f"nu {d['nu']!r}",
+             f"bcs_channel {d['u_bulk']!r} {d['k_in']!r} {d['omega_in']!r}",
+             "set max_iterations 600", "set convergence_tol 0.0001",
+             "set divergence_limit 100000000.0", "set alpha_u 0.3", "set alpha_p 0.2",
+             "set alpha_k 0.4", "set alpha_omega 0.4", "set inner_iterations 300",
+             "set inner_tolerance 0.0001", "set turb_start_iter 30", "set turb_update_interval 2",
+             "set pressure_solver PCG",
+             f"init {d['u_bulk']!r} 0 0 0 {d['k_in']!r} {d['omega_in']!r}",
+             "obs_drag step_face 0.0 0.1 1.0 1.0", "obs_drag bottom_wall_down 0.0 0.01 22.0 1.0",
+             "obs_u 4.0 0.05 0 0.0 0.01", "params 9 8", "theta 0.31 0.09"]
+    res = run_case(lines)
+    out["bfs"] = {"case": "ref_bfs.json's problem observed by the drag of 'step_face' (area 1) "
+                          "and of 'bottom_wall_down' (area 22) and one velocity probe",
+                  "thetas": [[0.31, 0.09]], "rows": res["rows"]}
+    write("ref_drag.json", out)
+
+
 EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
-         "plate": plate_fixture, "bfs": bfs_fixture, "crossings": crossings_fixture}
+         "plate": plate_fixture, "bfs": bfs_fixture, "crossings": crossings_fixture,
+         "drag": drag_fixture}
 
 if __name__ == "__main__":
     if len(sys.argv) == 2 and sys.argv[1] in EXTRA:

@@ -21,8 +21,8 @@ import pytest
 
 from cfdr_amd import cases, engine
 from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, crossings_problems,
-                      inout_channel, load, max_rel, max_scaled, plate_problem, small_channel,
-                      small_hill, symmetry_channel)
+                      drag_problems, inout_channel, load, max_rel, max_scaled, plate_problem,
+                      small_channel, small_hill, symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -279,6 +279,24 @@ def test_separation_and_reattachment_match_reference(case):
         assert res.statuses[0] == d["rows"][0]["status"]
         assert res.iteration_counts[0] == d["rows"][0]["iters"]
         assert max_scaled(res.predictions[0], d["rows"][0]["predictions"]) <= 1e-4
+
+
+@pytest.mark.parametrize("case", ["plate", "bfs"])
+def test_drag_matches_reference(case):
+    """Drag observations on the flat plate (converged) and on the step face
+    and lower wall of the backward-facing step (capped): the rows follow the
+    reference's, the drag included. A problem with drag observations runs
+    one workgroup per draw even when four are asked for."""
+    d = load("ref_drag.json")[case]
+    prob = drag_problems()[case]
+    res = _batch(prob).evaluate(np.asarray(d["thetas"]))
+    for i, row in enumerate(d["rows"]):
+        _check_row(res, i, row)
+    if case == "bfs":
+        b = _batch(prob, grid_blocks=4)
+        res = b.evaluate(np.asarray(d["thetas"]))
+        assert b.last_timing()["grid_blocks"] == 1
+        _check_row(res, 0, d["rows"][0])
 
 
 def test_reference_couette_golden_amg():

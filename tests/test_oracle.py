@@ -30,8 +30,8 @@ from cfdr_amd import cases
 from cfdr_amd.config import COEFF_DEFAULTS
 from cfdr_amd.mesh import Mesh
 from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, crossings_problems,
-                      inout_channel, load, max_rel, plate_problem, small_channel, small_hill,
-                      symmetry_channel)
+                      drag_problems, inout_channel, load, max_rel, plate_problem, small_channel,
+                      small_hill, symmetry_channel)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -419,6 +419,29 @@ def test_separation_and_reattachment(case):
         # domain, and no separation on the wall before the step
         x_r = r["predictions"][0]
         assert 1.0 < x_r < 22.0 and r["predictions"][2] == 0.0
+
+
+@pytest.mark.parametrize("case", ["plate", "bfs"])
+def test_drag(case):
+    """Drag observations on the flat plate and on the step face and lower
+    wall of the backward-facing step: rows equal the reference's. On the
+    plate the pressure term vanishes (the wall normal has no x part), so the
+    drag is the area-weighted mean skin friction of the converged field."""
+    d = load("ref_drag.json")[case]
+    prob = drag_problems()[case]
+    P = _oracle(prob)
+    for i, (theta, row) in enumerate(zip(d["thetas"], d["rows"])):
+        r = P.evaluate(prob.coefficients(theta), fields=i == 0)
+        _check_row(r, row)
+        if case == "plate" and i == 0:
+            m = prob.mesh
+            wall = next(p for p in m.patches if p.name == "bottom").faces
+            ow = m.owner[wall]
+            assert np.all(m.face_normal[wall][:, 0] == 0.0)
+            # a stationary no-slip wall: the boundary velocity is zero
+            tau = prob.nu * np.asarray(r["Ux"])[ow] / np.maximum(m.face_delta[wall], 1e-20)
+            cd = float(np.sum(tau * m.face_area[wall])) / 0.5
+            assert r["predictions"][0] == pytest.approx(cd, rel=1e-12)
 
 
 def test_reference_couette_golden():
