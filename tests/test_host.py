@@ -226,3 +226,32 @@ def test_engine_extensions_reject_null_engines():
     assert lib.rq_engine_set_graph_replay(None, 1, err, 256) == 1
     assert "null engine" in err.value.decode()
     assert lib.rq_engine_graph_launches(None) == -1
+
+
+def test_patch_observables_validation_and_stencils():
+    """addDrag's reference checks (ObservationOperator.hpp:189-201), an
+    unknown patch name, and the prepared stencil of the patch observables:
+    the patch's faces in patch order with their centre x, drag's reference
+    area in offx."""
+    with pytest.raises(ValueError, match="drag reference area"):
+        ObservationOperator().add_drag("bottom", 0.0, 0.1, 0.0, 1.0)
+    with pytest.raises(ValueError, match="drag reference speed"):
+        ObservationOperator().add_drag("bottom", 0.0, 0.1, 1.0, math.nan)
+    with pytest.raises(ValueError, match="Gaussian measurements"):
+        ObservationOperator().add_separation_point("bottom", 0.0, 0.0)
+    prob = cases.channel_problem(ny=8, ratio=1.2, n_probes=1)
+    mesh = prob.mesh
+    op = ObservationOperator()
+    op.add_reattachment_length("no_such_patch", 0.0, 0.1)
+    with pytest.raises(ValueError, match="unknown patch"):
+        op.prepare(mesh)
+    op = ObservationOperator()
+    op.add_drag("bottom", 0.0, 0.1, 2.5, 1.0)
+    op.add_separation_point("top", 0.0, 0.1)
+    P = op.prepare(mesh)
+    assert list(P.type) == [3, 4] and P.offx[0] == 2.5
+    for i, name in enumerate(("bottom", "top")):
+        faces = next(p for p in mesh.patches if p.name == name).faces
+        q = slice(P.nb_ptr[i], P.nb_ptr[i + 1])
+        assert list(P.nb_face[q]) == list(faces) and np.all(P.nb_cell[q] == -1)
+        assert np.array_equal(P.nb_dx[q], mesh.face_center[faces][:, 0])
