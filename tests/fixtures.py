@@ -172,6 +172,44 @@ def crossings_problems():
     return {"cosine_bump": bump, "bfs": step}
 
 
+def bfs_lemoin_problem():
+    """The reference's committed Le-Moin-Kim regression
+    (tests/tools/record_goldens.cpp:463-497, golden data in
+    ref_bfs_lemoin_uniform_inlet.json): makeBackwardFacingStep2D(40, 48, 24,
+    18, 10, 22, 1, 6, 5100, 1) with a symmetry top (MSH2 fixture written by the
+    reference's generator), bfsDefaults, Re_h 5100, Tu 5 %, AMG pressure,
+    a reattachment length, 7 pressure taps, 7 skin-friction stations and a
+    velocity profile at x 4."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    g = load("ref_bfs_lemoin_uniform_inlet.json")
+    mesh = Mesh.load_binary(os.path.join(GOLDEN, "ref_bfs_lemoin_40x48.msh"))
+    mesh.compute_wall_distance()
+    nu, ub = g["nu"], 1.0
+    k_in = 1.5 * (ub * 0.05) * (ub * 0.05)
+    om_in = k_in / (nu * 100.0)
+    st = g["settings"]
+    s = SolverSettings(max_iterations=st["max_iterations"], convergence_tol=st["convergence_tol"],
+                       divergence_limit=st["divergence_limit"], alpha_u=st["alpha_u"],
+                       alpha_p=st["alpha_p"], alpha_k=st["alpha_k"], alpha_omega=st["alpha_omega"],
+                       inner_iterations=st["inner_iterations"],
+                       inner_tolerance=st["inner_tolerance"], turb_start_iter=st["turb_start_iter"],
+                       turb_update_interval=st["turb_update_interval"])
+    q = g["qoi"]
+    obs = ObservationOperator()
+    obs.add_reattachment_length("bottom_wall_down", 6.28, 0.5)
+    for x in q["tap_x"]:
+        obs.add_pressure_tap((x, 0.0), 0.0, 0.01, ub)
+    for x in q["tap_x"]:
+        obs.add_skin_friction("bottom_wall_down", (x, 0.0), 0.0, 1e-3, ub)
+    for y in q["profile_y_at_x4"]:
+        obs.add_velocity_profile((4.0, y), 0, 0.0, 0.01)
+    return cases.Problem(mesh=mesh, bcs=FlowBoundaryConditions.channel_defaults(mesh, ub, k_in, om_in),
+                         settings=s, obs=obs, params=InferenceParameterSet.a1_betaStar(), nu=nu,
+                         u_init=Vec3(ub, 0.0, 0.0), p_init=0.0, k_init=k_in, omega_init=om_in)
+
+
 def drag_problems():
     """The two problems of ref_drag.json: the flat plate observed by the drag
     of its plate and a velocity probe, and the backward-facing step observed

@@ -437,6 +437,10 @@ def bfs_fixture(tmp):
     lines += ["params 9 8", "theta 0.31 0.09", f"dump {os.path.join(tmp, 'bfs')}"]
     res = run_case(lines)
     res["rows"][0]["fields"] = read_dump(os.path.join(tmp, "bfs_0.bin"))
+    # the reference's own Le-Moin-Kim mesh, for its committed regression
+    # (copied as data to ref_bfs_lemoin_uniform_inlet.json)
+    run_case(["mesh_bfs 40 48 24 18 10.0 22.0 1.0 6.0 5100.0 1.0", "patch_type top_wall symmetry",
+              f"save_mesh {os.path.join(HERE, 'ref_bfs_lemoin_40x48.msh')}", "nu 1", "params 9 8"])
     write("ref_bfs.json", {"case": "backward-facing step on ref_bfs_20x24.msh "
                                    "(makeBackwardFacingStep2D(20, 24, 12, 9, 10, 22, 1, 6, 5100, 1), "
                                    "top_wall symmetry), bfsDefaults, PCG pressure, 600 iterations",

@@ -21,8 +21,8 @@ import pytest
 
 from cfdr_amd import cases, engine
 from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, crossings_problems,
-                      drag_problems, inout_channel, load, max_rel, max_scaled, plate_problem,
-                      small_channel, small_hill, symmetry_channel)
+                      bfs_lemoin_problem, drag_problems, inout_channel, load, max_rel,
+                      max_scaled, plate_problem, small_channel, small_hill, symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -297,6 +297,32 @@ def test_drag_matches_reference(case):
         res = b.evaluate(np.asarray(d["thetas"]))
         assert b.last_timing()["grid_blocks"] == 1
         _check_row(res, 0, d["rows"][0])
+
+
+def test_reference_bfs_golden_amg():
+    """The reference's committed Le-Moin-Kim backward-facing-step regression
+    (2,976 cells, AMG pressure, converged in 4,060 iterations, recorded with
+    another compiler and libm): the device AMG (Jacobi-smoothed) converges
+    too, at the parity bar for converged solves, and each group of
+    observations (reattachment length, pressure taps, skin friction, the
+    velocity profile) stays within REL_OBS of the group's size (observed:
+    the same 4,060 iterations and 1e-12)."""
+    g = load("ref_bfs_lemoin_uniform_inlet.json")
+    prob = bfs_lemoin_problem()
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    its = g["result"]["final_iter"]
+    print("bfs golden", res.statuses[0], res.iteration_counts[0], its, flush=True)
+    assert res.statuses[0] == 0 and not res.stalled[0]
+    assert _iters_close(res.iteration_counts[0], its)
+    q, p = g["qoi"], np.asarray(res.predictions[0])
+    n = len(q["tap_x"])
+    groups = {"x_reattachment": (p[:1], [q["x_reattachment"]]),
+              "cp": (p[1:1 + n], q["cp_taps"]), "cf": (p[1 + n:1 + 2 * n], q["cf_taps"]),
+              "u": (p[1 + 2 * n:], q["u_profile_x4"])}
+    for name, (got, want) in groups.items():
+        err = max_scaled(got, want)
+        print(name, err, flush=True)
+        assert err <= REL_OBS, name
 
 
 def test_reference_couette_golden_amg():
