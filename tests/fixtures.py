@@ -172,6 +172,36 @@ def crossings_problems():
     return {"cosine_bump": bump, "bfs": step}
 
 
+def re180_problem():
+    """The reference's committed periodic-channel regression
+    (periodic_channel_re180.json, copied as data): makePeriodicChannel2D from
+    the golden's mesh record, channelDefaults(1, 1, 10), U+ probes at x 0.5,
+    the golden's settings with body force (1, 0, 0) and AMG pressure, init
+    U 15, k 1, omega 10."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    g = load("ref_periodic_channel_re180.json")
+    mp = g["mesh"]
+    mesh = Mesh.make_periodic_channel_2d(mp["nx"], mp["ny"], mp["Lx"], mp["h"], mp["Re_tau"],
+                                         mp["yplus_target"])
+    mesh.compute_wall_distance()
+    obs = ObservationOperator()
+    for y in g["qoi"]["probe_y_over_delta"]:
+        obs.add_velocity_profile((0.5, y, 0.0), 0, 0.0, 0.1)
+    st = g["settings"]
+    s = SolverSettings(max_iterations=st["max_iterations"], convergence_tol=st["convergence_tol"],
+                       alpha_u=st["alpha_u"], alpha_p=st["alpha_p"], alpha_k=st["alpha_k"],
+                       alpha_omega=st["alpha_omega"], inner_iterations=st["inner_iterations"],
+                       inner_tolerance=st["inner_tolerance"], turb_start_iter=st["turb_start_iter"],
+                       turb_update_interval=st["turb_update_interval"],
+                       divergence_limit=st["divergence_limit"], body_force=Vec3(1.0, 0, 0))
+    return cases.Problem(mesh=mesh, bcs=FlowBoundaryConditions.channel_defaults(mesh, 1.0, 1.0, 10.0),
+                         settings=s, obs=obs, params=InferenceParameterSet.a1_betaStar(),
+                         nu=g["nu"], u_init=Vec3(15.0, 0.0, 0.0), p_init=0.0, k_init=1.0,
+                         omega_init=10.0)
+
+
 def bfs_lemoin_problem():
     """The reference's committed Le-Moin-Kim regression
     (tests/tools/record_goldens.cpp:463-497, golden data in

@@ -22,7 +22,8 @@ import pytest
 from cfdr_amd import cases, engine
 from fixtures import (FIELD_NAMES, bfs_problem, couette_problem, crossings_problems,
                       bfs_lemoin_problem, drag_problems, inout_channel, load, max_rel,
-                      max_scaled, plate_problem, small_channel, small_hill, symmetry_channel)
+                      max_scaled, plate_problem, re180_problem, small_channel, small_hill,
+                      symmetry_channel)
 
 pytestmark = pytest.mark.gpu
 
@@ -297,6 +298,21 @@ def test_drag_matches_reference(case):
         res = b.evaluate(np.asarray(d["thetas"]))
         assert b.last_timing()["grid_blocks"] == 1
         _check_row(res, 0, d["rows"][0])
+
+
+def test_reference_periodic_channel_golden_amg():
+    """The reference's committed periodic-channel regression (4x56, AMG
+    pressure, 4,080 iterations, recorded with another compiler and libm): the
+    device AMG gives its status and stalled flag, the parity bar's iteration
+    count and U+ probes within REL_OBS."""
+    g = load("ref_periodic_channel_re180.json")
+    prob = re180_problem()
+    res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
+    r = g["result"]
+    print("re180 golden", res.statuses[0], res.iteration_counts[0], r["final_iter"], flush=True)
+    assert res.statuses[0] == 0 and bool(res.stalled[0]) == r["stalled"]
+    assert _iters_close(res.iteration_counts[0], r["final_iter"])
+    assert max_rel(res.predictions[0], g["qoi"]["u_plus_probes"]) <= REL_OBS
 
 
 def test_reference_bfs_golden_amg():
