@@ -343,16 +343,19 @@ def test_reference_bfs_golden_amg():
 
 def test_reference_couette_golden_amg():
     """The reference's committed Couette regression (its AMG pressure
-    solver): the device AMG (Jacobi-smoothed) reaches the same status, and
-    probes within the 1e-4 it meets against the reference's AMG elsewhere."""
+    solver, recorded with another compiler and libm): the device AMG
+    (Jacobi-smoothed) reaches the same status, the parity bar's iteration
+    count and probes within REL_OBS."""
     g = load("ref_couette_re171_ny96.json")
     prob = couette_problem(pressure_solver="AMG")
     res = _batch(prob).evaluate(np.asarray([[0.31, 0.09]]))
-    assert res.statuses[0] == 0 and not res.stalled[0]
     its = g["result"]["final_iter"]
-    assert abs(int(res.iteration_counts[0]) - its) <= max(50, int(0.02 * its))
     q = g["qoi"]
-    assert max_rel(res.predictions[0], [q["cf_probe"]] + q["u_probes"]) <= 1e-4
+    err = max_rel(res.predictions[0], [q["cf_probe"]] + q["u_probes"])
+    print("couette golden", res.statuses[0], res.iteration_counts[0], its, err, flush=True)
+    assert res.statuses[0] == 0 and not res.stalled[0]
+    assert _iters_close(res.iteration_counts[0], its)
+    assert err <= REL_OBS
 
 
 @pytest.mark.parametrize("mixed", [False, True])
