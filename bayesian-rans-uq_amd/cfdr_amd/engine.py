@@ -430,11 +430,23 @@ class ForwardBatch:
             raise ValueError(err.value.decode())
         self._h = h
 
-    def __del__(self):
+    def close(self) -> None:
+        """Releases the engine's device memory, stream and captured graph now
+        (rq_engine_destroy) instead of at garbage collection."""
         h = getattr(self, "_h", None)
-        if h and self.lib is not None:
-            self.lib.rq_engine_destroy(h)
+        if h and getattr(self, "lib", None) is not None:
             self._h = None
+            self.lib.rq_engine_destroy(h)
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def n_parameters(self) -> int:
         return self.param_set.n_active()

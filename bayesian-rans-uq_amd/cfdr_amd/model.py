@@ -150,11 +150,16 @@ class ForwardModel:
             raise ValueError(err.value.decode())
         self._h = h
 
-    def __del__(self):
+    def close(self) -> None:
+        """Releases the native object now (rq_model_destroy) instead of at garbage
+        collection."""
         h = getattr(self, "_h", None)
         if h and getattr(self, "lib", None) is not None:
-            self.lib.rq_model_destroy(h)
             self._h = None
+            self.lib.rq_model_destroy(h)
+
+    def __del__(self):
+        self.close()
 
     def param_set(self) -> InferenceParameterSet:
         return self._param_set
@@ -218,11 +223,16 @@ class HostWarmCache:
             raise ValueError(err.value.decode())
         self._h = h
 
-    def __del__(self):
+    def close(self) -> None:
+        """Releases the native object now (rq_warm_cache_destroy) instead of at garbage
+        collection."""
         h = getattr(self, "_h", None)
         if h and getattr(self, "lib", None) is not None:
-            self.lib.rq_warm_cache_destroy(h)
             self._h = None
+            self.lib.rq_warm_cache_destroy(h)
+
+    def __del__(self):
+        self.close()
 
     @staticmethod
     def entry_bytes(n_theta: int, n_cells: int, n_boundary: int) -> int:

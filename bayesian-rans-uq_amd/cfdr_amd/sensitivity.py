@@ -92,11 +92,16 @@ class ParameterSensitivity:
             raise ValueError(err.value.decode())
         self._h = h
 
-    def __del__(self):
+    def close(self) -> None:
+        """Releases the native object now (rq_sensitivity_destroy) instead of at garbage
+        collection."""
         h = getattr(self, "_h", None)
         if h and getattr(self, "lib", None) is not None:
-            self.lib.rq_sensitivity_destroy(h)
             self._h = None
+            self.lib.rq_sensitivity_destroy(h)
+
+    def __del__(self):
+        self.close()
 
     @staticmethod
     def _theta(theta11) -> np.ndarray:

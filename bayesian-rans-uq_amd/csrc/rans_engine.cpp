@@ -152,6 +152,8 @@ struct rq_engine {
     }
 
     ~rq_engine() {
+        // nothing of this engine may still run when its memory goes back
+        if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         if (h_slots) (void)hipHostFree(h_slots);
         if (h_results) (void)hipHostFree(h_results);

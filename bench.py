@@ -284,6 +284,7 @@ def single_solve_leg(cpu):
             raise SystemExit("bench.py: the single solve's status differs from the CPU's")
         out["max_rel_err_vs_cpu_baseline"] = float(
             np.max(np.abs(r.predictions[0] - o["predictions"]) / np.abs(o["predictions"])))
+    b.close()
     print(f"# single solve: {out['gpu_ms']} ms on the GPU", file=sys.stderr, flush=True)
     return out
 
@@ -376,6 +377,7 @@ def served_leg(n_requests, rate, hill_cap=0):
         t1 = time.perf_counter()
         b.evaluate(np.asarray([row(n, first)]))
         svc[n] = time.perf_counter() - t1
+        b.close()
     end = 0.0
     plat = []
     for ta, n, th in stream:
@@ -444,6 +446,7 @@ def hills_leg(full, cores, cpu):
             tune = b.autotune(th, iterations=tune_its)
             legs[prec] = (b.evaluate(th), b.last_timing(), tune)
             legs["levels"] = b.amg_levels()
+            b.close()
             print(f"# {key} {prec}: {legs[prec][0].kernel_ms / cap:.2f} ms per outer iteration "
                   f"(tuned block {tune['block_size']}, {tune['grid_blocks']} per draw)",
                   file=sys.stderr, flush=True)
@@ -516,6 +519,7 @@ def hills_leg(full, cores, cpu):
             t0 = time.perf_counter()
             r = b.evaluate(th)
             dt = time.perf_counter() - t0
+            b.close()
             full_rows[prec] = r
             out["hill_120x64_full" + ("" if prec == "fp64" else "_mixed")] = {
                 "draws": 16, "seconds": round(dt, 1),
@@ -560,6 +564,7 @@ def warm_leg(steps):
             t0 = time.perf_counter()
             rw, used = m.evaluate_batch(stencil)
             warm_s.append(time.perf_counter() - t0)
+            m.close()
         ok = (rc.status_codes == 0) & (rw.status_codes == 0)
         err = float(np.max(np.abs(rw.predictions[ok] - rc.predictions[ok]) /
                            np.abs(rc.predictions[ok]))) if ok.any() else None
@@ -577,6 +582,7 @@ def warm_leg(steps):
             "max_rel_diff_warm_vs_cold": err}
         print(f"# warm {label}: cold {out[label]['cold_ms']} ms, warm {out[label]['warm_ms']} ms",
               file=sys.stderr, flush=True)
+    batch.close()
     return out
 
 
@@ -597,9 +603,11 @@ def warm_fd_leg(cpu):
     t0 = time.perf_counter()
     g = s.eta_jacobian_warm_fd(theta)
     gpu_s = time.perf_counter() - t0
+    kernel_ms = s.last_kernel_ms()
+    s.close()
     out = {"solves": 22, "columns_converged": int(np.sum(g.krylov_converged)),
            "simple_iterations": int(g.n_residual_evals), "gpu_ms": round(1e3 * gpu_s, 1),
-           "kernel_ms": round(s.last_kernel_ms(), 1)}
+           "kernel_ms": round(kernel_ms, 1)}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py
@@ -742,6 +750,7 @@ def main():
                  "unit": "converged solves/s", "dtype": "f64 assembly/norms, f32 Krylov",
                  "kernel_ms": round(rm.kernel_ms, 3), "statuses_match_fp64": same,
                  "max_rel_err_vs_fp64_gpu": err_fp64}
+        mb.close()
         print(f"# mixed precision: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
     plain = None
@@ -797,6 +806,8 @@ def main():
     if not args.no_hills:
         hills = hills_leg(args.hills_full, _cores(), not args.no_cpu_baseline)
 
+    timing = batch.last_timing()
+    batch.close()  # every engine is released before the interpreter exits
     line = {
         "metric": "converged SST solves/s (served vs plain) at <=1e-6 rel err; p50/p99 latency",
         "value": round(value, 4),
@@ -832,10 +843,10 @@ def main():
         "warm_start": warm,
         "warm_fd_jacobian": warm_fd,
         "hills": hills,
-        "block_size": batch.last_timing()["block_size"],
+        "block_size": timing["block_size"],
         "autotune": tune,
         "graph_replay": {"enabled": not args.no_graph, "graph_launches": graph_launches},
-        "arena_in_lds": batch.last_timing()["arena_in_lds"],
+        "arena_in_lds": timing["arena_in_lds"],
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -529,9 +529,103 @@ def drag_fixture(tmp):
     write("ref_drag.json", out)
 
 
+def run_rows_parallel(case_lines, workers):
+    """One reference process per case (each case holds one coefficient row):
+    the rows of a batch are independent solves, so they run side by side and
+    come back in the order given."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return [r["rows"][0] for r in pool.map(run_case, case_lines)]
+
+
+def _workers():
+    return int(os.environ.get("FIXTURE_WORKERS", max(1, (os.cpu_count() or 2) - 1)))
+
+
+def field_sample(fields, n_cells, count=64):
+    """Field values at `count` evenly spaced cells (and the face fluxes of as
+    many evenly spaced faces): enough to pin a large mesh's state without
+    storing it whole."""
+    cells = np.linspace(0, n_cells - 1, count).round().astype(int).tolist()
+    nf = len(fields["flux"])
+    faces = np.linspace(0, nf - 1, count).round().astype(int).tolist()
+    out = {"cells": cells, "faces": faces,
+           "flux": [fields["flux"][f] for f in faces]}
+    for n in ("Ux", "Uy", "p", "k", "omega", "nuT"):
+        out[n] = [fields[n][c] for c in cells]
+        out["sumsq_" + n] = float(np.sum(np.square(np.asarray(fields[n]))))
+    out["sumsq_flux"] = float(np.sum(np.square(np.asarray(fields["flux"]))))
+    return out
+
+
+def hill_c3_fixture(tmp):
+    """The 16-draw periodic hill at its full size (120x64 cells, AMG pressure,
+    tol 1e-4, cap 25,000: the reference's hills case,
+    tests/tools/record_goldens.cpp:558-628, observed by 20 velocity probes),
+    every draw solved to convergence or to the cap by the reference's
+    ForwardModel. Statuses, iteration counts, worst measures, rates and
+    probes pin the batch row by row."""
+    prob = cases.hill_problem(nx=120, ny=64)
+    mp = os.path.join(tmp, "hill120.msh")
+    prob.mesh.save_binary(mp)
+    thetas = cases.hill_draws(16).tolist()
+    jobs = [problem_lines(prob, mp, [th], bc=(1.0, 1e-4, 10.0)) for th in thetas]
+    rows = run_rows_parallel(jobs, _workers())
+    write("ref_hill_c3_converged.json",
+          {"case": "hill_problem(nx=120, ny=64): tol 1e-4, cap 25000, AMG pressure",
+           "thetas": thetas, "rows": rows})
+
+
+def hill_c5_fixture(tmp, iters=40):
+    """The 8-draw 480x256 periodic hill (122,880 cells) capped at 40 outer
+    iterations with Jacobi-PCG pressure, so that the device follows the
+    reference's own Krylov path: turbulence starts at iteration 30, so the
+    last iterations run the full SST update. Predictions, field sums of
+    squares and a strided sample of every field pin the state."""
+    prob = cases.hill_problem(nx=480, ny=256, max_iterations=iters, pressure_solver="PCG")
+    mp = os.path.join(tmp, "hill480.msh")
+    prob.mesh.save_binary(mp)
+    thetas = cases.hill_draws(8).tolist()
+    jobs = [problem_lines(prob, mp, [th], bc=(1.0, 1e-4, 10.0),
+                          dump=os.path.join(tmp, f"h480_{i}")) for i, th in enumerate(thetas)]
+    rows = run_rows_parallel(jobs, min(_workers(), 8))
+    for i, row in enumerate(rows):
+        fields = read_dump(os.path.join(tmp, f"h480_{i}_0.bin"))
+        row["fields"] = field_sample(fields, prob.mesh.n_cells)
+    write("ref_hill_c5_capped.json",
+          {"case": f"hill_problem(nx=480, ny=256, max_iterations={iters}, "
+                   f"pressure_solver='PCG')", "thetas": thetas, "rows": rows})
+
+
+def served_channels_fixture(tmp):
+    """The served stream's channel meshes (64, 256 and 512 wall-normal cells
+    with the first-cell height of the 128-cell mesh, cases.channel_ratio) at
+    the calibration settings (tol 1e-8, cap 150,000), two coefficient rows
+    each from the served stream's distribution: statuses and iteration counts
+    at the cap pin what the stream reports for these meshes."""
+    rng = np.random.default_rng(cases.SERVED_SEED + 1)
+    thetas = np.column_stack([rng.uniform(0.28, 0.34, size=2),
+                              rng.uniform(0.084, 0.096, size=2)]).tolist()
+    jobs, keys = [], []
+    for n in (64, 256, 512):
+        prob = cases.channel_problem(ny=n, ratio=cases.channel_ratio(n))
+        mp = os.path.join(tmp, f"ch{n}.msh")
+        prob.mesh.save_binary(mp)
+        for th in thetas:
+            jobs.append(problem_lines(prob, mp, [th]))
+            keys.append(n)
+    rows = run_rows_parallel(jobs, _workers())
+    out = {"case": "channel_problem(ny=n, ratio=channel_ratio(n)) for n in 64, 256, 512",
+           "thetas": thetas, "meshes": {}}
+    for n in (64, 256, 512):
+        out["meshes"][str(n)] = [r for r, k in zip(rows, keys) if k == n]
+    write("ref_served_channels.json", out)
+
+
 EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
          "plate": plate_fixture, "bfs": bfs_fixture, "crossings": crossings_fixture,
-         "drag": drag_fixture}
+         "drag": drag_fixture, "hill_c3": hill_c3_fixture, "hill_c5": hill_c5_fixture,
+         "served_channels": served_channels_fixture}
 
 if __name__ == "__main__":
     if len(sys.argv) == 2 and sys.argv[1] in EXTRA:
