@@ -92,7 +92,7 @@ class RqTuneOut(C.Structure):
                 ("chosen_grid_blocks", C.c_int32), ("chosen_amg_block_rows", C.c_int32)]
 
 
-EXPORTS = ["rq_engine_amg_levels", "rq_engine_autotune", "rq_engine_set_graph_replay", "rq_engine_graph_launches", "rq_engine_create", "rq_engine_evaluate", "rq_engine_n_observations",
+EXPORTS = ["rq_engine_amg_levels", "rq_engine_autotune", "rq_engine_set_graph_replay", "rq_engine_graph_launches", "rq_engine_create", "rq_engine_evaluate", "rq_engine_evaluate_noalloc", "rq_engine_n_observations",
            "rq_engine_n_parameters", "rq_engine_n_cells", "rq_engine_n_faces",
            "rq_engine_last_timing", "rq_engine_destroy", "rq_build_info",
            "rq_server_create", "rq_server_add_problem", "rq_server_submit", "rq_server_poll",
@@ -129,6 +129,8 @@ def load_library(path: str = LIB_PATH):
     lib.rq_engine_evaluate.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int32, C.c_int32,
                                        C.POINTER(RqBatchOut), C.c_char_p, C.c_size_t]
     lib.rq_engine_evaluate.restype = C.c_int
+    lib.rq_engine_evaluate_noalloc.argtypes = lib.rq_engine_evaluate.argtypes
+    lib.rq_engine_evaluate_noalloc.restype = C.c_int
     for name in ("rq_engine_n_observations", "rq_engine_n_parameters", "rq_engine_n_cells",
                  "rq_engine_n_faces"):
         getattr(lib, name).argtypes = [C.c_void_p]
@@ -515,6 +517,24 @@ class ForwardBatch:
         rc = self.lib.rq_engine_evaluate(self._h, theta.ctypes.data_as(_dp), theta.shape[0],
                                          self.n_parameters(), int(workers), C.byref(bb.out), err,
                                          512)
+        BatchBuffers.check(rc, err)
+        return bb.result(self.last_timing()["kernel_ms"])
+
+    def evaluate_noalloc(self, theta, workers: int = 1) -> ForwardBatchResult:
+        """The same evaluation through rq_engine_evaluate_noalloc: every
+        per-call device buffer comes from the library's fixed scratch region
+        (no device allocation in the call); no fields are returned."""
+        theta = np.asarray(theta, dtype=np.float64)
+        if theta.ndim == 1 and theta.size == 0:
+            theta = theta.reshape(0, self.n_parameters())
+        if theta.ndim != 2 or theta.shape[1] != self.n_parameters():
+            raise ValueError("theta must have shape (n_rows, n_parameters)")
+        theta = np.ascontiguousarray(theta)
+        bb = BatchBuffers(theta.shape[0], self.n_obs, self.n_cells, self.n_faces, False)
+        err = C.create_string_buffer(512)
+        rc = self.lib.rq_engine_evaluate_noalloc(self._h, theta.ctypes.data_as(_dp),
+                                                 theta.shape[0], self.n_parameters(), int(workers),
+                                                 C.byref(bb.out), err, 512)
         BatchBuffers.check(rc, err)
         return bb.result(self.last_timing()["kernel_ms"])
 

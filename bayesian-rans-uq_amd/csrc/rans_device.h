@@ -31,6 +31,10 @@ enum DevError { ERR_NONE = 0, ERR_NONFINITE_SYSTEM = 1, ERR_NONFINITE_MONITOR = 
 // Multi-workgroup draws: the workgroups of one draw meet at grid barriers on
 // this per-draw block (device memory, zeroed before every launch) and
 // exchange their reduction partials through it.
+// Fixed device region of the allocation-free evaluation
+// (rq_engine_evaluate_noalloc): one per GPU, held until the process ends.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
 constexpr int kMaxGridBlocks = 64;
 constexpr int kGridBlock = 256;      // threads per workgroup of a multi-workgroup draw
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X

@@ -38,6 +38,9 @@ hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const D
                              DevResult *results, double *pred, int pred_stride, double *fields,
                              int64_t field_stride, GridState *states, size_t lds_bytes,
                              hipStream_t stream);
+hipError_t noalloc_scratch(void **ptr, size_t *bytes);
+std::mutex &noalloc_lock(int dev);
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream);
 }
 
 using namespace rq;
@@ -990,23 +993,25 @@ void grow_pinned(T *&ptr, size_t &cap, size_t n) {
     cap = n;
 }
 
-void write_row_error(const rq_batch_out *out, int64_t row, const std::string &msg) {
+void write_row_error(const rq_batch_out *out, int64_t row, const char *msg) {
     if (out->row_errors && out->row_error_stride)
         std::snprintf(out->row_errors + row * out->row_error_stride, out->row_error_stride, "%s",
-                      msg.c_str());
+                      msg);
 }
 
 // Status, log-likelihood and error text of one solved row
 // (ForwardModel.cpp:92-135, ObservationOperator.hpp:592-607).
+// msg is a string literal or null (no heap text, so the allocation-free
+// evaluation can use it too)
 struct RowFinal {
     int status;
     double loglik;
     bool keep_predictions;
-    std::string msg;
+    const char *msg;
 };
 
 RowFinal finalize_row(const rq_engine *E, const DevResult &d, const double *pred) {
-    RowFinal f{d.status, -1e6, false, std::string()};
+    RowFinal f{d.status, -1e6, false, nullptr};
     if (f.status == ST_UNKNOWN)
         f.msg = d.error == ERR_NONFINITE_MONITOR ? "convergence monitor returned a nonfinite observation"
                 : d.error == ERR_GRID_TIMEOUT   ? "device synchronisation of the draw timed out"
@@ -1026,7 +1031,7 @@ RowFinal finalize_row(const rq_engine *E, const DevResult &d, const double *pred
                 break;
             }
         }
-        if (f.msg.empty()) {
+        if (!f.msg) {
             f.loglik = acc;
             f.keep_predictions = true;
         } else {
@@ -1038,7 +1043,7 @@ RowFinal finalize_row(const rq_engine *E, const DevResult &d, const double *pred
 
 // InferenceParameterSet::inBounds and SSTCoefficients::validate for one row;
 // fills the 11 coefficients. Returns 0, or ST_INVALID / ST_UNKNOWN with text.
-int admit_row(const rq_engine *E, const double *row, double *coeffs, std::string &msg) {
+int admit_row(const rq_engine *E, const double *row, double *coeffs, const char *&msg) {
     for (int i = 0; i < E->n_active; ++i) {
         const double v = row[i];
         if (v < E->lower[i] || v > E->upper[i] || std::isnan(v) || std::isinf(v)) {
@@ -1056,6 +1061,46 @@ int admit_row(const rq_engine *E, const double *row, double *coeffs, std::string
         return ST_UNKNOWN;
     }
     return 0;
+}
+
+// Dynamic LDS of a multi-workgroup launch: each workgroup's bookkeeping
+// replica (four contraction rings of 2m samples, the current and the previous
+// monitor values).
+size_t grid_lds_bytes(const rq_engine *E) {
+    return (8 * (size_t)E->settings.convergence_window + 2 * (size_t)std::max(E->nobs, 1)) * 8;
+}
+
+// Workgroups per draw for a launch of ns admitted rows.
+// Automatic selection (grid_blocks 0, no explicit block size): arenas in
+// HBM of at least kGridAutoCells cells spread each draw over the CUs the
+// batch leaves idle, one workgroup per 256 cells at most. Below that size
+// one workgroup per draw is faster (measured: 60x32 hill 2.3 ms per outer
+// iteration on one workgroup against 2.8 on eight; 120x64 9.3 against
+// 4.7 on sixteen).
+int choose_grid_blocks(const rq_engine *E, int ns, size_t grid_lds) {
+    int nblk = 1;
+    if (E->settings.grid_blocks > 1) {
+        nblk = E->settings.grid_blocks;
+    } else if (E->settings.grid_blocks == 0 && !E->lds && !E->block_user &&
+               E->nc >= kGridAutoCells) {
+        const int per_xcd = (ns + kXcds - 1) / kXcds;
+        nblk = std::min({kMaxGridBlocks, E->cus / (kXcds * per_xcd), (E->nc + 255) / 256});
+    }
+    if (nblk > 1) {
+        // every workgroup of the padded grid (draws spread over the XCDs) must
+        // be co-resident
+        const int co = grid_co_resident(grid_lds, E->settings.mixed_precision != 0);
+        const int per_xcd = (ns + kXcds - 1) / kXcds;
+        const int fit = co / (kXcds * per_xcd);
+        nblk = std::min(nblk, fit);
+        if (nblk < 2) nblk = 1;
+    }
+    // the multi-workgroup kernels do not evaluate drag, and the mixed-precision
+    // one neither separation nor reattachment: such problems run one
+    // workgroup per draw
+    if (nblk > 1 && (E->drag || (E->settings.mixed_precision != 0 && E->wall_crossings)))
+        nblk = 1;
+    return nblk;
 }
 
 // Per-row solve options of the sensitivity path: tolerance and iteration
@@ -1101,7 +1146,7 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         // and SSTCoefficients::validate (core/include/SSTModel.hpp:29-45)
         DevSlot sl{};
         sl.mesh = E->d_mesh;
-        std::string msg;
+        const char *msg = nullptr;
         const int bad = admit_row(E, row, sl.coeffs, msg);
         if (bad) {
             out->status[r] = bad;
@@ -1139,36 +1184,8 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
     }
     // workgroups per draw: meshes whose arena lives in HBM spread each draw
     // over several co-resident workgroups when the batch leaves CUs idle
-    int nblk = 1;
-    const size_t grid_lds =
-        (8 * (size_t)E->settings.convergence_window + 2 * (size_t)std::max(nobs, 1)) * 8;
-    // Automatic selection (grid_blocks 0, no explicit block size): arenas in
-    // HBM of at least kGridAutoCells cells spread each draw over the CUs the
-    // batch leaves idle, one workgroup per 256 cells at most. Below that size
-    // one workgroup per draw is faster (measured: 60x32 hill 2.3 ms per outer
-    // iteration on one workgroup against 2.8 on eight; 120x64 9.3 against
-    // 4.7 on sixteen).
-    if (E->settings.grid_blocks > 1) {
-        nblk = E->settings.grid_blocks;
-    } else if (E->settings.grid_blocks == 0 && !E->lds && !E->block_user &&
-               E->nc >= kGridAutoCells) {
-        const int per_xcd = (ns + kXcds - 1) / kXcds;
-        nblk = std::min({kMaxGridBlocks, E->cus / (kXcds * per_xcd), (E->nc + 255) / 256});
-    }
-    if (nblk > 1) {
-        // every workgroup of the padded grid (draws spread over the XCDs) must
-        // be co-resident
-        const int co = grid_co_resident(grid_lds, E->settings.mixed_precision != 0);
-        const int per_xcd = (ns + kXcds - 1) / kXcds;
-        const int fit = co / (kXcds * per_xcd);
-        nblk = std::min(nblk, fit);
-        if (nblk < 2) nblk = 1;
-    }
-    // the multi-workgroup kernels do not evaluate drag, and the mixed-precision
-    // one neither separation nor reattachment: such problems run one
-    // workgroup per draw
-    if (nblk > 1 && (E->drag || (E->settings.mixed_precision != 0 && E->wall_crossings)))
-        nblk = 1;
+    const size_t grid_lds = grid_lds_bytes(E);
+    int nblk = choose_grid_blocks(E, ns, grid_lds);
     if (nblk > 1 && E->lds) {
         // the grid mode keeps the arena in HBM
         grow(E->d_arena, E->cap_arena, (size_t)ns * E->lay.stride);
@@ -1282,7 +1299,7 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         const DevResult &d = resp[i];
         const RowFinal fin = finalize_row(E, d, predp + (size_t)i * std::max(nobs, 1));
         const int status = fin.status;
-        const std::string &msg = fin.msg;
+        const char *msg = fin.msg;
         if (fin.keep_predictions && out->predictions)
             for (int j = 0; j < nobs; ++j)
                 out->predictions[r * nobs + j] = predp[(size_t)i * nobs + j];
@@ -1297,7 +1314,7 @@ int evaluate_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
             for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = d.krylov[e];
         if (out->phase_cycles)
             for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = d.cycles[e];
-        if (!msg.empty()) write_row_error(out, r, msg);
+        if (msg) write_row_error(out, r, msg);
         if (out->fields)
             std::memcpy(out->fields + r * fstride, fieldsp + (size_t)i * fstride,
                         fstride * sizeof(double));
@@ -1350,7 +1367,7 @@ namespace {
 // (ForwardModel.cpp:45-49 returns before it).
 bool row_admissible(const rq_engine *E, const double *row) {
     double coeffs[11];
-    std::string msg;
+    const char *msg = nullptr;
     return admit_row(E, row, coeffs, msg) == 0;
 }
 
@@ -1880,7 +1897,7 @@ void server_collect(rq_server *S) {
         const RowFinal fin = finalize_row(E, t.res, t.pred.data());
         t.status = fin.status;
         t.loglik = fin.loglik;
-        t.msg = fin.msg;
+        t.msg = fin.msg ? fin.msg : "";
         if (!fin.keep_predictions)
             std::fill(t.pred.begin(), t.pred.end(), std::numeric_limits<double>::quiet_NaN());
         t.done = true;
@@ -1977,6 +1994,210 @@ int32_t rq_engine_amg_levels(const rq_engine *e, int32_t *rows, int64_t *entries
         if (entries) entries[l] = G.lev[l].nnz;
     }
     return G.levels;
+}
+
+// ----------------------------------------------------------------------------
+// Allocation-free evaluation (no counterpart in the reference): the kernels
+// of rq_engine_evaluate, with every per-call device buffer (slot table,
+// result rows, predictions, grid-barrier state, HBM arenas, multigrid
+// workspace) carved out of the current GPU's fixed scratch region, and the
+// host side staged through small stack buffers. Errors are return codes
+// (nothing is thrown), so no path of the call allocates.
+// ----------------------------------------------------------------------------
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static void put_err(char *err, size_t cap, const char *fmt, ...) {
+    if (!err || cap == 0) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(err, cap, fmt, ap);
+    va_end(ap);
+}
+
+static_assert(sizeof(GridState) % sizeof(double) == 0, "grid state is cleared as doubles");
+static_assert(kNoAllocScratchBytes == RQ_NOALLOC_SCRATCH_BYTES, "one scratch size");
+
+int rq_engine_evaluate_noalloc(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
+                               int32_t workers, rq_batch_out *out, char *err, size_t errcap) {
+#define RQ_NA_FAIL(code, ...)                     \
+    do {                                          \
+        put_err(err, errcap, __VA_ARGS__);        \
+        return code;                              \
+    } while (0)
+#define RQ_NA_HIP(expr)                                                                    \
+    do {                                                                                   \
+        const hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) RQ_NA_FAIL(RQ_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+    if (!E || !out || !out->status) RQ_NA_FAIL(RQ_EINVAL, "null argument");
+    std::lock_guard<std::recursive_mutex> lock(E->mu);
+    if (workers < 1) RQ_NA_FAIL(RQ_EINVAL, "workers must be positive");
+    if (n_params != E->n_active)
+        RQ_NA_FAIL(RQ_EINVAL, "batch rows must all have n_parameters entries");
+    if (n_rows < 0) RQ_NA_FAIL(RQ_EINVAL, "row count must be nonnegative");
+    if (n_rows > 0 && !theta) RQ_NA_FAIL(RQ_EINVAL, "null coefficient rows");
+    if (out->fields) RQ_NA_FAIL(RQ_EINVAL, "the allocation-free evaluation returns no fields");
+    const int nobs = E->nobs, po = std::max(nobs, 1);
+    if (n_rows > 0 && nobs > 0 && !out->predictions)
+        RQ_NA_FAIL(RQ_EINVAL, "the allocation-free evaluation needs the predictions array");
+    // the region's plan for n_rows rows (every row admitted), checked before
+    // anything is written
+    const size_t cap = kNoAllocScratchBytes;
+    if (n_rows > (int64_t)(cap / sizeof(DevSlot)))
+        RQ_NA_FAIL(RQ_EINVAL,
+                   "the allocation-free evaluation takes at most %zu bytes of device scratch; "
+                   "%lld rows exceed it", cap, (long long)n_rows);
+    const bool arenas = !E->lds || E->settings.grid_blocks > 1;
+    const int64_t amgw = E->mesh_host.amg.workspace;
+    const size_t nr = (size_t)n_rows;
+    size_t off = 0;
+    const size_t o_slots = off;
+    off += round256(nr * sizeof(DevSlot));
+    const size_t o_res = off;
+    off += round256(nr * sizeof(DevResult));
+    const size_t o_pred = off;
+    off += round256(nr * po * sizeof(double));
+    const size_t o_grid = off;
+    off += arenas ? round256(nr * sizeof(GridState)) : 0;
+    const size_t o_arena = off;
+    off += arenas ? round256(nr * (size_t)E->lay.stride * sizeof(double)) : 0;
+    const size_t o_amg = off;
+    off += amgw > 0 ? round256(nr * (size_t)amgw * sizeof(double)) : 0;
+    if (off > cap)
+        RQ_NA_FAIL(RQ_EINVAL,
+                   "the allocation-free evaluation of %lld rows needs %zu bytes of device "
+                   "scratch; its fixed region holds %zu bytes", (long long)n_rows, off, cap);
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    // per-row outputs and the host admission (ForwardModel.cpp:45-49);
+    // admitted rows are marked -1 until their result arrives
+    int ns = 0;
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *row = theta + r * n_params;
+        if (out->row_errors && out->row_error_stride) out->row_errors[r * out->row_error_stride] = 0;
+        if (out->predictions)
+            for (int i = 0; i < nobs; ++i) out->predictions[r * nobs + i] = nan;
+        if (out->iterations) out->iterations[r] = 0;
+        if (out->final_worst_measure) out->final_worst_measure[r] = 0.0;
+        if (out->contraction_rate_u) out->contraction_rate_u[r] = 0.0;
+        if (out->stalled) out->stalled[r] = 0;
+        if (out->converged) out->converged[r] = 0;
+        if (out->krylov_iterations)
+            for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = 0;
+        if (out->phase_cycles)
+            for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = 0;
+        double coeffs[11];
+        const char *msg = nullptr;
+        const int bad = admit_row(E, row, coeffs, msg);
+        if (bad) {
+            out->status[r] = bad;
+            if (out->loglik) out->loglik[r] = bad == ST_INVALID ? -1e30 : -1e6;
+            if (msg) write_row_error(out, r, msg);
+        } else {
+            out->status[r] = -1;
+            ++ns;
+        }
+    }
+    if (ns == 0) {
+        put_err(err, errcap, "%s", "");
+        return RQ_OK;
+    }
+    RQ_NA_HIP(hipSetDevice(E->device));
+    void *base = nullptr;
+    size_t have = 0;
+    RQ_NA_HIP(noalloc_scratch(&base, &have));
+    if (off > have) RQ_NA_FAIL(RQ_EINVAL, "device scratch region smaller than planned");
+    // every call on this GPU shares the region: one at a time, from the first
+    // write to the last read
+    std::lock_guard<std::mutex> region(noalloc_lock(E->device));
+    unsigned char *b = static_cast<unsigned char *>(base);
+    DevSlot *d_slots = reinterpret_cast<DevSlot *>(b + o_slots);
+    DevResult *d_res = reinterpret_cast<DevResult *>(b + o_res);
+    double *d_pred = reinterpret_cast<double *>(b + o_pred);
+    GridState *d_grid = reinterpret_cast<GridState *>(b + o_grid);
+    double *d_arena = reinterpret_cast<double *>(b + o_arena);
+    double *d_amg = reinterpret_cast<double *>(b + o_amg);
+    const size_t grid_lds = grid_lds_bytes(E);
+    const int nblk = choose_grid_blocks(E, ns, grid_lds);
+    // the slot table, uploaded from the stack a few rows at a time; a row's
+    // result and predictions land at its own row index
+    constexpr int kStage = 32;
+    DevSlot stage[kStage];
+    int placed = 0, k = 0;
+    for (int64_t r = 0; r < n_rows; ++r) {
+        if (out->status[r] != -1) continue;
+        DevSlot &sl = stage[k];
+        sl = DevSlot{};
+        sl.mesh = E->d_mesh;
+        const char *msg = nullptr;
+        (void)admit_row(E, theta + r * n_params, sl.coeffs, msg);
+        sl.row = (int)r;
+        if (!E->lds || nblk > 1) sl.arena = d_arena + (size_t)placed * E->lay.stride;
+        if (amgw > 0) sl.amg = d_amg + (size_t)placed * amgw;
+        ++placed;
+        if (++k == kStage || placed == ns) {
+            RQ_NA_HIP(hipMemcpyAsync(d_slots + (placed - k), stage, k * sizeof(DevSlot),
+                                     hipMemcpyHostToDevice, E->stream));
+            RQ_NA_HIP(hipStreamSynchronize(E->stream));
+            k = 0;
+        }
+    }
+    // predictions start as NaN (rows the host settled keep them); the
+    // barrier counters of a multi-workgroup launch start at zero
+    RQ_NA_HIP(launch_fill(d_pred, (int64_t)(nr * po), nan, E->stream));
+    if (nblk > 1)
+        RQ_NA_HIP(launch_fill(reinterpret_cast<double *>(d_grid),
+                              (int64_t)(ns * sizeof(GridState) / sizeof(double)), 0.0, E->stream));
+    RQ_NA_HIP(hipEventRecord(E->ev0, E->stream));
+    if (nblk > 1)
+        RQ_NA_HIP(launch_grid_solve(d_slots, ns, nblk, E->settings, d_res, d_pred, po, nullptr, 0,
+                                    d_grid, grid_lds, E->stream));
+    else
+        RQ_NA_HIP(launch_simple_solve(E->block, E->lds, d_slots, ns, E->settings, d_res, d_pred,
+                                      po, nullptr, 0, E->lds_bytes, E->stream));
+    RQ_NA_HIP(hipEventRecord(E->ev1, E->stream));
+    if (nobs > 0)
+        RQ_NA_HIP(hipMemcpyAsync(out->predictions, d_pred, nr * nobs * sizeof(double),
+                                 hipMemcpyDeviceToHost, E->stream));
+    RQ_NA_HIP(hipStreamSynchronize(E->stream));
+    float ms = 0.f;
+    RQ_NA_HIP(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+    E->last_ms = ms;
+    E->last_nblk = nblk;
+    DevResult rstage[kStage];
+    for (int64_t r0 = 0; r0 < n_rows; r0 += kStage) {
+        const int cnt = (int)std::min<int64_t>(kStage, n_rows - r0);
+        bool any = false;
+        for (int j = 0; j < cnt; ++j) any |= out->status[r0 + j] == -1;
+        if (!any) continue;
+        RQ_NA_HIP(hipMemcpyAsync(rstage, d_res + r0, cnt * sizeof(DevResult),
+                                 hipMemcpyDeviceToHost, E->stream));
+        RQ_NA_HIP(hipStreamSynchronize(E->stream));
+        for (int j = 0; j < cnt; ++j) {
+            const int64_t r = r0 + j;
+            if (out->status[r] != -1) continue;
+            const DevResult &d = rstage[j];
+            const RowFinal fin =
+                finalize_row(E, d, nobs > 0 ? out->predictions + r * nobs : nullptr);
+            if (!fin.keep_predictions && out->predictions)
+                for (int i = 0; i < nobs; ++i) out->predictions[r * nobs + i] = nan;
+            out->status[r] = fin.status;
+            if (out->loglik) out->loglik[r] = fin.loglik;
+            if (out->iterations) out->iterations[r] = d.iters;
+            if (out->final_worst_measure) out->final_worst_measure[r] = d.worst;
+            if (out->contraction_rate_u) out->contraction_rate_u[r] = d.rate;
+            if (out->stalled) out->stalled[r] = (uint8_t)d.stalled;
+            if (out->converged) out->converged[r] = fin.status == ST_CONVERGED;
+            if (out->krylov_iterations)
+                for (int e = 0; e < 5; ++e) out->krylov_iterations[r * 5 + e] = d.krylov[e];
+            if (out->phase_cycles)
+                for (int e = 0; e < kPhases; ++e) out->phase_cycles[r * kPhases + e] = d.cycles[e];
+            if (fin.msg) write_row_error(out, r, fin.msg);
+        }
+    }
+    put_err(err, errcap, "%s", "");
+    return RQ_OK;
+#undef RQ_NA_HIP
+#undef RQ_NA_FAIL
 }
 
 int32_t rq_engine_n_observations(const rq_engine *e) { return e ? e->nobs : -1; }
@@ -2089,13 +2310,13 @@ int64_t rq_server_submit(rq_server *S, int32_t problem, const double *theta, int
         rq_server::Ticket t;
         t.problem = problem;
         double coeffs[11];
-        std::string msg;
+        const char *msg = nullptr;
         const int bad = admit_row(E, theta, coeffs, msg);
         if (bad) {  // settled on the host, like a batch row that never reaches the device
             t.done = true;
             t.status = bad;
             t.loglik = bad == ST_INVALID ? -1e30 : -1e6;
-            t.msg = msg;
+            t.msg = msg ? msg : "";
             t.pred.assign(E->nobs, std::numeric_limits<double>::quiet_NaN());
             S->tickets.push_back(std::move(t));
             set_err(err, errcap, "");

@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "rans_device.h"
@@ -3456,6 +3459,56 @@ hipError_t launch_simple_solve(int block, bool lds, const DevSlot *slots, int ns
 
 int grid_co_resident(size_t lds_bytes, bool mixed) {
     return mixed ? grid_co_resident_f32(lds_bytes) : grid_co_resident_t<false>(lds_bytes);
+}
+
+// ----------------------------------------------------------------------------
+// The allocation-free evaluation's device memory: a fixed region reserved
+// when this code object is loaded on a GPU and held until the process ends
+// (each GPU holds its own copy at its own address). Host code never hands the
+// array's name to a kernel or a copy -- on the host that name is only a
+// stand-in -- but the address hipGetSymbolAddress returns for the current
+// device, looked up once per device and kept.
+// ----------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(void **ptr, size_t *bytes) {
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    *bytes = kNoAllocScratchBytes;
+    return hipSuccess;
+}
+
+// every noalloc call on one GPU shares its region: callers hold this lock
+// (one per device) from the first write to the last read of the region
+std::mutex &noalloc_lock(int dev) {
+    static std::mutex locks[kMaxDevices];
+    return locks[dev < 0 || dev >= kMaxDevices ? 0 : dev];
+}
+
+// n doubles set to v: clears the region with a kernel (a memset node in a
+// captured graph is avoided on purpose)
+__global__ void __launch_bounds__(256) fill_kernel(double *__restrict__ p, int64_t n, double v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_grid_solve(const DevSlot *slots, int nslots, int nblk, const DevSettings &s,

@@ -147,6 +147,26 @@ rq_engine *rq_engine_create(const rq_mesh_desc *mesh, const rq_param_desc *param
 int rq_engine_evaluate(rq_engine *engine, const double *theta, int64_t n_rows, int32_t n_params,
                        int32_t workers, rq_batch_out *out, char *err, size_t errcap);
 
+/* No counterpart in the reference: rq_engine_evaluate for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same kernels, with every per-call
+ * device buffer (slot table, result rows, predictions, grid-barrier state,
+ * HBM arenas, multigrid workspace) taken from one fixed device region of
+ * RQ_NOALLOC_SCRATCH_BYTES per GPU, reserved when the library's kernels are
+ * loaded on that GPU and held until the process ends; the call itself makes
+ * no hipMalloc, hipFree or other allocation. A call that would need more
+ * than the region (about rows x (arena + multigrid workspace + result and
+ * prediction bytes)) fails with RQ_EINVAL and names the size. out->fields
+ * must be NULL and, when the problem has observations, out->predictions
+ * must be given. The region keeps the last call's contents; every call on
+ * one GPU shares it, so calls that overlap on a GPU would race on it: the
+ * library runs them one at a time with a process-wide lock per GPU. No graph
+ * replay: the launch is direct. */
+#define RQ_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int rq_engine_evaluate_noalloc(rq_engine *engine, const double *theta, int64_t n_rows,
+                               int32_t n_params, int32_t workers, rq_batch_out *out, char *err,
+                               size_t errcap);
+
 int32_t rq_engine_n_observations(const rq_engine *engine);
 int32_t rq_engine_n_parameters(const rq_engine *engine);
 int32_t rq_engine_n_cells(const rq_engine *engine);
