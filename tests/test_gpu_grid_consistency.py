@@ -43,3 +43,27 @@ def test_inlet_outlet_channel_over_workgroup_counts(mixed):
 def test_converged_hills_over_workgroup_counts(mixed):
     d = load("ref_hill_small_converged.json")
     _agree(cases.hill_problem(nx=30, ny=16), np.asarray(d["thetas"]), mixed, (2, 8))
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_capped_worst_measure_matches_one_workgroup(mixed):
+    """At a fixed cap the state, the probes, the contraction rate and the
+    final worst convergence measure of a multi-workgroup draw are those of
+    one workgroup. A build whose multi-workgroup kernel evaluates the worst
+    measure wrongly reports it infinite here with every other output right
+    (the signature of the defect that keeps drag and wall-crossing
+    observations out of these kernels)."""
+    d = load("ref_channel_inout.json")["schemes"]["limitedLinearUpwind"]
+    prob = inout_channel(d["momentum_convection"], d["turbulence_convection"])
+    prob.settings.max_iterations = 800
+    theta = np.asarray(d["thetas"])
+    one = _batch(prob, mixed_precision=mixed, grid_blocks=1).evaluate(theta)
+    assert np.all(np.isfinite(one.final_worst_measures))
+    for g in (2, 3, 8):
+        r = _batch(prob, mixed_precision=mixed, grid_blocks=g).evaluate(theta)
+        assert [int(s) for s in r.statuses] == [int(s) for s in one.statuses], g
+        assert list(r.iteration_counts) == list(one.iteration_counts), g
+        assert np.all(np.isfinite(r.final_worst_measures)), g
+        assert np.allclose(r.final_worst_measures, one.final_worst_measures, rtol=1e-6, atol=0), g
+        assert np.allclose(r.contraction_rates, one.contraction_rates, rtol=1e-9, atol=0), g
+        assert max_rel(r.predictions, one.predictions) <= 1e-9, g

@@ -238,7 +238,8 @@ def test_backward_facing_step_matches_reference(grid_blocks):
     capped iterations, one and four workgroups per draw: the row and every
     field follow the reference's. With FP32 Krylov plus FP64 defect
     correction the capped row keeps its status and count, and its probes stay
-    within 1e-4 of the FP64 reference (a capped state, not a converged one)."""
+    within REL_OBS of the FP64 reference scaled by the probes' magnitude
+    (a capped state whose skin-friction probes cross zero; measured 1.5e-7)."""
     d = load("ref_bfs.json")
     prob = bfs_problem()
     b = _batch(prob, grid_blocks=grid_blocks)
@@ -252,7 +253,7 @@ def test_backward_facing_step_matches_reference(grid_blocks):
         res = _batch(prob, mixed_precision=True).evaluate(np.asarray([d["theta"]]))
         assert res.statuses[0] == d["row"]["status"]
         assert res.iteration_counts[0] == d["row"]["iters"]
-        assert max_scaled(res.predictions[0], d["row"]["predictions"]) <= 1e-4
+        assert max_scaled(res.predictions[0], d["row"]["predictions"]) <= REL_OBS
 
 
 @pytest.mark.parametrize("case", ["cosine_bump", "bfs"])
@@ -269,7 +270,8 @@ def test_separation_and_reattachment_match_reference(case):
         # four workgroups per draw in FP64 evaluate the crossings in the
         # multi-workgroup kernel; in mixed precision such a problem runs on
         # one workgroup per draw, with the capped status and count kept and
-        # probes within 1e-4 of the FP64 reference
+        # probes within REL_OBS of the FP64 reference, scaled by their
+        # magnitude (measured 2.3e-7)
         b = _batch(prob, grid_blocks=4)
         res = b.evaluate(np.asarray(d["thetas"]))
         assert b.last_timing()["grid_blocks"] == 4
@@ -279,7 +281,7 @@ def test_separation_and_reattachment_match_reference(case):
         assert b.last_timing()["grid_blocks"] == 1
         assert res.statuses[0] == d["rows"][0]["status"]
         assert res.iteration_counts[0] == d["rows"][0]["iters"]
-        assert max_scaled(res.predictions[0], d["rows"][0]["predictions"]) <= 1e-4
+        assert max_scaled(res.predictions[0], d["rows"][0]["predictions"]) <= REL_OBS
 
 
 @pytest.mark.parametrize("case", ["plate", "bfs"])
@@ -440,8 +442,8 @@ def test_mixed_precision_block_paths_match_fp64_reference():
 def test_mixed_precision_amg_hill_matches_fp64_reference():
     """FP32 AMG-preconditioned CG (level-0 smoothing and Krylov vectors in
     FP32, coarser levels FP64) on the converged 30x16 hills, one and eight
-    workgroups per draw: the reference's statuses, and probes within the 1e-4
-    that the FP64 device AMG meets against the reference's AMG."""
+    workgroups per draw: the reference's statuses and iteration counts, and
+    probes within REL_OBS of the reference's AMG (measured 4.7e-9)."""
     d = load("ref_hill_small_converged.json")
     th, st, its, pred = _rows_of(d)
     prob = cases.hill_problem(nx=30, ny=16)
@@ -449,8 +451,8 @@ def test_mixed_precision_amg_hill_matches_fp64_reference():
         res = _batch(prob, mixed_precision=True, grid_blocks=g).evaluate(th)
         for i in range(len(th)):
             assert res.statuses[i] == st[i] == 0
-            assert abs(int(res.iteration_counts[i]) - its[i]) <= max(50, int(0.02 * its[i]))
-            assert max_rel(res.predictions[i], pred[i]) <= 1e-4
+            assert _iters_close(res.iteration_counts[i], its[i])
+            assert max_rel(res.predictions[i], pred[i]) <= REL_OBS
 
 
 def test_amg_matches_pcg_at_tight_tolerance():
@@ -469,16 +471,17 @@ def test_amg_matches_pcg_at_tight_tolerance():
 
 
 def test_converged_hill_amg_matches_reference():
-    """Converged 30x16 hills: the reference's AMG (Gauss-Seidel smoothing,
-    value-based aggregates) against the device AMG (Jacobi smoothing,
-    geometric aggregates) reach the same fixed point to the solve tolerance."""
+    """Converged 30x16 hills: the reference's AMG (Gauss-Seidel smoothing)
+    against the device AMG (damped-Jacobi smoothing, coarser aggregates):
+    both reach the same fixed point, with the reference's iteration counts and
+    probes within REL_OBS (measured 4.6e-9)."""
     d = load("ref_hill_small_converged.json")
     prob = cases.hill_problem(nx=30, ny=16)
     res = _batch(prob).evaluate(np.asarray(d["thetas"]))
     for i, row in enumerate(d["rows"]):
         assert res.statuses[i] == row["status"] == 0
-        assert abs(int(res.iteration_counts[i]) - row["iters"]) <= max(50, int(0.02 * row["iters"]))
-        assert max_rel(res.predictions[i], row["predictions"]) <= 1e-4
+        assert _iters_close(res.iteration_counts[i], row["iters"])
+        assert max_rel(res.predictions[i], row["predictions"]) <= REL_OBS
 
 
 @pytest.mark.parametrize("solver", ["PCG", "AMG"])
@@ -506,8 +509,8 @@ def test_multi_workgroup_converged_hill_matches_reference():
     res = _batch(prob, grid_blocks=8).evaluate(np.asarray(d["thetas"]))
     for i, row in enumerate(d["rows"]):
         assert res.statuses[i] == row["status"] == 0
-        assert abs(int(res.iteration_counts[i]) - row["iters"]) <= max(50, int(0.02 * row["iters"]))
-        assert max_rel(res.predictions[i], row["predictions"]) <= 1e-4
+        assert _iters_close(res.iteration_counts[i], row["iters"])
+        assert max_rel(res.predictions[i], row["predictions"]) <= REL_OBS
 
 
 def test_batch_results_independent_of_batch_composition():
