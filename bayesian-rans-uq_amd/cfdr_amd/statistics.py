@@ -1,13 +1,21 @@
-"""Status-aware statistics over GPU forward-model ensembles: the reference's
-statistics layer (rans-uq/python/cfdr/statistics.py) with the same public
-functions, arguments, results and errors, evaluated through this engine's
-``ForwardBatch`` and ``ForwardModel``.
+"""Status-aware statistics over ensembles of GPU forward solves.
 
-Failed rows are never admitted: any evaluation that is not Converged (or that
-carries an error) makes the posterior undefined, whatever its log-likelihood
-sentinel. Every random stream is keyed by (root seed, sample ID, purpose)
-through SHA-256, so results do not depend on row order, batching or worker
-assignment.
+The public functions, their arguments, results and error texts are those of
+the reference's statistics layer (rans-uq/python/cfdr/statistics.py), so a
+calibration script written against ``cfdr.statistics`` runs unchanged on
+``cfdr_amd.engine.ForwardBatch`` / ``cfdr_amd.model.ForwardModel``.
+
+Three rules shape everything here:
+
+* Admission is by status, never by value. A row counts only when it is
+  Converged with finite predictions and log-likelihood and no error text;
+  one unresolved row makes any posterior undefined, whatever its
+  likelihood sentinel (statistics.py:208-232 in the reference).
+* Randomness is addressed, not sequential. Every draw comes from a stream
+  named by (root seed, sample ID, purpose), so a result does not depend on
+  row order, batching or which worker ran what.
+* Variances are exact properties of the stated mixture, not sample
+  estimates of it.
 """
 from __future__ import annotations
 
@@ -15,76 +23,94 @@ import hashlib
 import math
 from collections.abc import Callable, Sequence
 from dataclasses import dataclass
-from typing import Any
+from typing import Any, NamedTuple
 
 import numpy as np
 
 from . import engine
 from .model import ForwardModel
 
-STATUS_NAMES = ("Converged", "Unconverged", "InvalidParameters", "DivergenceDetected", "Unknown")
-FAILED_WITHOUT_PREDICTIONS = ("InvalidParameters", "DivergenceDetected", "Unknown")
 BACKEND_MODEL = "cfdr_amd.ForwardModel"
 BACKEND_BATCH = "cfdr_amd.ForwardBatch"
 
+# Evaluation statuses, the reference's names (bindings.cpp:464-469)
+STATUS_NAMES = ("Converged", "Unconverged", "InvalidParameters", "DivergenceDetected", "Unknown")
+# statuses whose rows carry no predictions at all (their NaN placeholders are dropped)
+FAILED_WITHOUT_PREDICTIONS = ("InvalidParameters", "DivergenceDetected", "Unknown")
+_NONFINITE_SUCCESS = "nonfinite or missing successful output"
 
-def _nonnegative_int(value, what: str) -> None:
-    ok = isinstance(value, int) and not isinstance(value, bool) and value >= 0
-    if not ok:
+
+# ------------------------------------------------------------------ checks
+def _count(value, what: str) -> int:
+    """A nonnegative Python integer (bool excluded) or ValueError."""
+    if type(value) is bool or not isinstance(value, int) or value < 0:
         raise ValueError(f"{what} must be a nonnegative integer")
+    return value
 
 
-def _unique_ids(values: Sequence[int]) -> tuple:
-    ids = tuple(values)
-    for v in ids:
-        _nonnegative_int(v, "sample ID")
-    if len(ids) != len(set(ids)):
+def _distinct_ids(values: Sequence[int]) -> tuple:
+    ids = tuple(_count(v, "sample ID") for v in values)
+    if len(frozenset(ids)) < len(ids):
         raise ValueError("sample IDs must be unique")
     return ids
 
 
-def sample_rng(root_seed: int, sample_id: int, purpose: str) -> np.random.Generator:
-    """The PCG64 stream of (root seed, sample ID, purpose), statistics.py:42-58:
-    the SHA-256 digest of "cfdr-statistics-v1:<root>:<id>:<purpose>" read as
-    eight little-endian 32-bit words seeds a SeedSequence."""
-    _nonnegative_int(root_seed, "root seed")
-    _nonnegative_int(sample_id, "sample ID")
-    if not purpose:
-        raise ValueError("RNG purpose must be nonempty")
-    key = f"cfdr-statistics-v1:{root_seed}:{sample_id}:{purpose}".encode("utf-8")
-    digest = hashlib.sha256(key).digest()
-    words = [int.from_bytes(digest[4 * w:4 * w + 4], "little") for w in range(8)]
+def _finite_vector(values, what: str) -> np.ndarray:
+    a = np.asarray(values, dtype=float)
+    if a.ndim == 1 and a.size and np.isfinite(a).all():
+        return a
+    raise ValueError(f"{what} must be a finite nonempty vector")
+
+
+# ----------------------------------------------------------------- streams
+def _stream(root_seed: int, sample_id: int, purpose: str) -> np.random.Generator:
+    """The stream construction every consumer of these results relies on:
+    SHA-256 of "cfdr-statistics-v1:<root>:<id>:<purpose>" read as eight
+    little-endian 32-bit words seeds a SeedSequence for PCG64
+    (statistics.py:42-58). Callers have validated the fields."""
+    digest = hashlib.sha256(
+        f"cfdr-statistics-v1:{root_seed}:{sample_id}:{purpose}".encode("utf-8")).digest()
+    words = np.frombuffer(digest, dtype="<u4").tolist()
     return np.random.Generator(np.random.PCG64(np.random.SeedSequence(words)))
 
 
+def sample_rng(root_seed: int, sample_id: int, purpose: str) -> np.random.Generator:
+    """The PCG64 stream of (root seed, sample ID, purpose)."""
+    _count(root_seed, "root seed")
+    _count(sample_id, "sample ID")
+    if not purpose:
+        raise ValueError("RNG purpose must be nonempty")
+    return _stream(root_seed, sample_id, purpose)
+
+
+# ----------------------------------------------------------------- designs
 @dataclass(frozen=True)
 class Sample:
     sample_id: int
     theta: tuple
 
     def __post_init__(self):
-        _nonnegative_int(self.sample_id, "sample ID")
+        _count(self.sample_id, "sample ID")
 
 
 def independent_uniform_design(sample_ids: Sequence[int], lower, upper, *,
                                root_seed: int) -> tuple:
-    """One independent uniform prior draw per ID (statistics.py:69-90)."""
-    ids = _unique_ids(sample_ids)
+    """One independent uniform draw in [lower, upper) per ID, each from its
+    own "prior-design" stream (not a Latin hypercube)."""
+    ids = _distinct_ids(sample_ids)
     lo = np.asarray(lower, dtype=float)
     hi = np.asarray(upper, dtype=float)
-    valid = (lo.ndim == 1 and lo.size > 0 and hi.shape == lo.shape and
-             bool(np.all(np.isfinite(lo))) and bool(np.all(np.isfinite(hi))) and
-             bool(np.all(lo < hi)) and bool(np.all(np.isfinite(hi - lo))))
-    if not valid:
+    with np.errstate(over="ignore", invalid="ignore"):  # an infinite width is refused below
+        width = hi - lo if hi.shape == lo.shape else None
+    if not (lo.ndim == 1 and lo.size and width is not None and np.isfinite(lo).all()
+            and np.isfinite(hi).all() and np.isfinite(width).all() and (width > 0).all()):
         raise ValueError("uniform bounds must be finite nonempty vectors with lower < upper")
-    _nonnegative_int(root_seed, "root seed")
-    out = []
-    for i in ids:
-        draw = sample_rng(root_seed, i, "prior-design").uniform(lo, hi)
-        out.append(Sample(i, tuple(float(v) for v in draw)))
-    return tuple(out)
+    _count(root_seed, "root seed")
+    return tuple(Sample(i, tuple(_stream(root_seed, i, "prior-design").uniform(lo, hi).tolist()))
+                 for i in ids)
 
 
+# ------------------------------------------------------------- evaluations
 @dataclass(frozen=True)
 class Evaluation:
     sample_id: int
@@ -97,8 +123,8 @@ class Evaluation:
     error: str
 
     def __post_init__(self):
-        _nonnegative_int(self.sample_id, "sample ID")
-        _nonnegative_int(self.iterations, "iteration count")
+        _count(self.sample_id, "sample ID")
+        _count(self.iterations, "iteration count")
         if self.status not in STATUS_NAMES:
             raise ValueError("unrecognized evaluation status")
         if not self.backend:
@@ -106,7 +132,8 @@ class Evaluation:
 
     @property
     def admitted(self) -> bool:
-        return self.status == "Converged" and not self.error
+        """Converged and without error text: the only rows a posterior uses."""
+        return not self.error and self.status == "Converged"
 
 
 @dataclass(frozen=True)
@@ -114,105 +141,120 @@ class EvaluationSet:
     records: tuple
 
     def __post_init__(self):
-        _unique_ids([r.sample_id for r in self.records])
+        _distinct_ids([r.sample_id for r in self.records])
 
     @property
     def failure_count(self) -> int:
-        return sum(0 if r.admitted else 1 for r in self.records)
+        return len(self.records) - sum(r.admitted for r in self.records)
+
+    def _canonical(self) -> tuple:
+        """The records in sample-ID order (the order posterior quantities
+        are formed in, so they do not depend on how rows arrived)."""
+        return tuple(sorted(self.records, key=lambda r: r.sample_id))
 
 
-def _record(sample: Sample, result, backend: str, error: str = "") -> Evaluation:
-    """One engine row as an Evaluation (statistics.py:134-158): failed rows
-    keep no predictions, and a Converged row without finite predictions and
-    log-likelihood becomes Unknown."""
-    status = str(result.status.name)
-    preds = () if status in FAILED_WITHOUT_PREDICTIONS else tuple(float(v) for v in
-                                                                  result.predictions)
-    loglik = float(result.log_lik) if status == "Converged" else None
-    if status == "Converged":
-        finite = (len(preds) > 0 and all(math.isfinite(v) for v in preds) and
-                  loglik is not None and math.isfinite(loglik))
-        if not finite:
-            status, loglik, error = "Unknown", None, "nonfinite or missing successful output"
-    return Evaluation(sample.sample_id, sample.theta, status, preds, loglik,
-                      int(result.simple_iters), backend, error)
+def _from_engine_row(sample: Sample, row, backend: str, error: str = "") -> Evaluation:
+    """One engine row as an Evaluation. Rows of the failed statuses keep
+    their status and error but no predictions; a Converged row whose
+    predictions or log-likelihood are missing or nonfinite is recorded as
+    Unknown, so it can never be admitted (statistics.py:134-158)."""
+    status = str(row.status.name)
+    iterations = int(row.simple_iters)
+    if status in FAILED_WITHOUT_PREDICTIONS:
+        return Evaluation(sample.sample_id, sample.theta, status, (), None, iterations,
+                          backend, error)
+    predictions = tuple(float(v) for v in row.predictions)
+    if status != "Converged":
+        return Evaluation(sample.sample_id, sample.theta, status, predictions, None,
+                          iterations, backend, error)
+    loglik = float(row.log_lik)
+    if predictions and math.isfinite(loglik) and all(map(math.isfinite, predictions)):
+        return Evaluation(sample.sample_id, sample.theta, status, predictions, loglik,
+                          iterations, backend, error)
+    return Evaluation(sample.sample_id, sample.theta, "Unknown", predictions, None, iterations,
+                      backend, _NONFINITE_SUCCESS)
 
 
-def _checked_samples(samples: Sequence[Sample]) -> tuple:
+def _design_rows(samples: Sequence[Sample]) -> tuple:
     rows = tuple(samples)
-    _unique_ids([s.sample_id for s in rows])
-    if rows:
-        n = len(rows[0].theta)
-        if n == 0 or any(len(s.theta) != n for s in rows):
-            raise ValueError("sample parameter vectors must have equal nonzero lengths")
+    _distinct_ids([s.sample_id for s in rows])
+    widths = {len(s.theta) for s in rows}
+    if rows and (len(widths) != 1 or 0 in widths):
+        raise ValueError("sample parameter vectors must have equal nonzero lengths")
     return rows
 
 
 def evaluate_scalar(samples: Sequence[Sample], model_factory: Callable[[], Any]) -> EvaluationSet:
-    """A fresh ForwardModel with its warm-start cache off for every sample
+    """Each sample on a fresh ForwardModel whose warm-start cache is off
     (statistics.py:171-184)."""
     out = []
-    for s in _checked_samples(samples):
-        m = model_factory()
-        if not isinstance(m, ForwardModel):
+    for s in _design_rows(samples):
+        model = model_factory()
+        if not isinstance(model, ForwardModel):
             raise TypeError("model_factory must construct cfdr_amd.model.ForwardModel")
-        m.cache().set_byte_budget(0)
-        ev = m.evaluate_with_diagnostics(s.theta)
-        out.append(_record(s, ev.result, BACKEND_MODEL, ev.error))
+        model.cache().set_byte_budget(0)
+        diag = model.evaluate_with_diagnostics(s.theta)
+        out.append(_from_engine_row(s, diag.result, BACKEND_MODEL, diag.error))
     return EvaluationSet(tuple(out))
 
 
 def evaluate_batch(samples: Sequence[Sample], batch: Any, *, workers: int = 1) -> EvaluationSet:
-    """All samples in one ForwardBatch evaluation; failed rows stay aligned
-    with their IDs (statistics.py:187-205)."""
-    rows = _checked_samples(samples)
+    """Every sample in one ForwardBatch evaluation (one GPU launch); failed
+    rows stay aligned with their IDs (statistics.py:187-205)."""
+    rows = _design_rows(samples)
     if not isinstance(batch, engine.ForwardBatch):
         raise TypeError("batch must be cfdr_amd.engine.ForwardBatch")
-    _nonnegative_int(workers, "worker count")
-    if workers < 1:
+    if _count(workers, "worker count") == 0:
         raise ValueError("worker count must be positive")
     if not rows:
         return EvaluationSet(())
-    res = batch.evaluate(np.asarray([s.theta for s in rows], dtype=float), workers)
-    if len(res.rows) != len(rows):
+    native = batch.evaluate(np.asarray([s.theta for s in rows], dtype=float), workers).rows
+    if len(native) != len(rows):
         raise RuntimeError("native batch returned a different row count")
-    return EvaluationSet(tuple(_record(s, r.result, BACKEND_BATCH, r.error)
-                               for s, r in zip(rows, res.rows)))
+    return EvaluationSet(tuple(_from_engine_row(s, n.result, BACKEND_BATCH, n.error)
+                               for s, n in zip(rows, native)))
 
 
-def _require_resolved(evaluations: EvaluationSet) -> None:
-    if not evaluations.records:
+# ------------------------------------------------------ importance posterior
+def _resolved(evaluations: EvaluationSet) -> tuple:
+    """The records, once every one of them is admitted."""
+    records = evaluations.records
+    if not records:
         raise ValueError("at least one evaluation is required")
-    n = evaluations.failure_count
-    if n:
-        raise ValueError(f"{n} unresolved evaluations; posterior is undefined")
+    unresolved = evaluations.failure_count
+    if unresolved:
+        raise ValueError(f"{unresolved} unresolved evaluations; posterior is undefined")
+    return records
 
 
 def posterior_weights(evaluations: EvaluationSet) -> np.ndarray:
-    """Self-normalised likelihood weights of a prior design (statistics.py:217-232)."""
-    _require_resolved(evaluations)
-    ll = [r.log_likelihood for r in evaluations.records]
-    if any(v is None or not math.isfinite(v) for v in ll):
+    """Self-normalised likelihood weights of a design drawn from the prior:
+    w_i = exp(l_i - max l) / sum_j exp(l_j - max l), in record order."""
+    loglik = [r.log_likelihood for r in _resolved(evaluations)]
+    if not all(v is not None and math.isfinite(v) for v in loglik):
         raise ValueError("successful log likelihoods must be finite")
-    top = max(ll)
-    w = [math.exp(v - top) for v in ll]
-    total = math.fsum(w)
-    return np.asarray(w, dtype=float) / total
+    peak = max(loglik)
+    unnormalised = [math.exp(v - peak) for v in loglik]
+    return np.asarray(unnormalised, dtype=float) / math.fsum(unnormalised)
 
 
 def resample_posterior(evaluations: EvaluationSet, sample_ids: Sequence[int], *,
                        root_seed: int) -> tuple:
-    """Multinomial draws from the importance posterior, one stream per
-    output ID over the records in ID order (statistics.py:235-260)."""
-    ids = _unique_ids(sample_ids)
-    by_id = EvaluationSet(tuple(sorted(evaluations.records, key=lambda r: r.sample_id)))
-    w = posterior_weights(by_id)
-    _nonnegative_int(root_seed, "root seed")
-    n = len(by_id.records)
-    return tuple(Sample(i, by_id.records[int(sample_rng(root_seed, i, "posterior-resample")
-                                             .choice(n, p=w))].theta) for i in ids)
+    """Independent multinomial draws from the importance posterior: output
+    ID k picks a record of the ID-ordered set with its own
+    "posterior-resample" stream, so the draws do not depend on input order,
+    output order or chunking."""
+    out_ids = _distinct_ids(sample_ids)
+    canonical = EvaluationSet(evaluations._canonical())
+    weights = posterior_weights(canonical)
+    _count(root_seed, "root seed")
+    pool = canonical.records
+    picks = [int(_stream(root_seed, k, "posterior-resample").choice(len(pool), p=weights))
+             for k in out_ids]
+    return tuple(Sample(k, pool[j].theta) for k, j in zip(out_ids, picks))
 
 
+# --------------------------------------------------- predictive distribution
 @dataclass(frozen=True)
 class PredictiveSummary:
     sample_ids: tuple
@@ -228,115 +270,131 @@ class PredictiveSummary:
     includes_observation_noise: bool
 
 
+class _Noise(NamedTuple):
+    purpose: str          # stream purpose of this component's draws
+    variance: np.ndarray  # per output
+
+
+def _noise_variance(value, n_out: int) -> np.ndarray:
+    if value is None:
+        return np.zeros(n_out)
+    v = np.asarray(value, dtype=float)
+    if v.shape == (n_out,) and np.isfinite(v).all() and (v >= 0).all():
+        return v
+    raise ValueError("noise components must be finite nonnegative output-length vectors")
+
+
 def predictive_summary(evaluations: EvaluationSet, *, root_seed: int, observation_sigma=None,
                        surrogate_variance=None, discrepancy_variance=None) -> PredictiveSummary:
-    """Equal-weight empirical mixture of the predictions plus independent
+    """The equal-weight mixture of the admitted predictions plus independent
     Gaussian surrogate, discrepancy and observation components
-    (statistics.py:277-339). The variances are the mixture's exact ones
-    (population variance over records in ID order plus the components)."""
-    _require_resolved(evaluations)
-    _nonnegative_int(root_seed, "root seed")
-    latent = np.asarray([r.predictions for r in evaluations.records], dtype=float)
-    if latent.ndim != 2 or latent.shape[1] == 0 or not np.all(np.isfinite(latent)):
+    (statistics.py:277-339). samples holds one noisy draw per record (record
+    order; each component from that record's own stream); the variances are
+    the mixture's exact ones: the population variance of the predictions
+    over records in ID order plus each component's variance."""
+    records = _resolved(evaluations)
+    _count(root_seed, "root seed")
+    latent = np.asarray([r.predictions for r in records], dtype=float)
+    if not (latent.ndim == 2 and latent.shape[1] and np.isfinite(latent).all()):
         raise ValueError("successful predictions must be finite equal-length nonempty vectors")
-    dim = latent.shape[1]
-
-    def comp(v):
-        a = np.zeros(dim) if v is None else np.asarray(v, dtype=float)
-        if a.shape != (dim,) or not np.all(np.isfinite(a)) or np.any(a < 0):
-            raise ValueError("noise components must be finite nonnegative output-length vectors")
-        return a
-
-    obs_var = comp(observation_sigma) ** 2
-    sur_var = comp(surrogate_variance)
-    dis_var = comp(discrepancy_variance)
-    if not np.all(np.isfinite(obs_var)):
+    n_out = latent.shape[1]
+    sigma = _noise_variance(observation_sigma, n_out)
+    with np.errstate(over="ignore"):  # an infinite variance is refused below
+        sigma_sq = sigma * sigma
+    components = (_Noise("surrogate-noise", _noise_variance(surrogate_variance, n_out)),
+                  _Noise("discrepancy", _noise_variance(discrepancy_variance, n_out)),
+                  _Noise("observation-noise", sigma_sq))
+    if not np.isfinite(components[2].variance).all():
         raise ValueError("observation variance must be finite")
-    noisy = latent.copy()
-    for row, rec in enumerate(evaluations.records):
-        for purpose, var in (("surrogate-noise", sur_var), ("discrepancy", dis_var),
-                             ("observation-noise", obs_var)):
-            if np.any(var):
-                noisy[row] += sample_rng(root_seed, rec.sample_id, purpose).normal(
-                    0, np.sqrt(var), dim)
-    order = np.argsort([r.sample_id for r in evaluations.records])
-    ordered = latent[order]
-    mean = ordered.mean(axis=0)
-    var = ordered.var(axis=0, ddof=0)
-    total = var + sur_var + dis_var + obs_var
-    if not np.all(np.isfinite(noisy)) or not np.all(np.isfinite(total)):
+    active = [c for c in components if c.variance.any()]
+    draws = latent.copy()
+    for row, rec in zip(draws, records):
+        for c in active:
+            row += _stream(root_seed, rec.sample_id, c.purpose).normal(
+                0.0, np.sqrt(c.variance), n_out)
+    by_id = latent[np.argsort([r.sample_id for r in records], kind="stable")]
+    with np.errstate(over="ignore", invalid="ignore"):  # overflow is reported below
+        mean, spread = by_id.mean(axis=0), by_id.var(axis=0)
+    # summed in the component order (surrogate, discrepancy, observation)
+    total = spread + components[0].variance + components[1].variance + components[2].variance
+    if not (np.isfinite(draws).all() and np.isfinite(total).all()):
         raise ValueError("predictive samples and total variance must be finite")
-    return PredictiveSummary(tuple(r.sample_id for r in evaluations.records),
-                             tuple(tuple(float(v) for v in row) for row in noisy), tuple(mean),
-                             tuple(var), tuple(sur_var), tuple(dis_var), tuple(obs_var),
-                             tuple(total), bool(np.any(sur_var)), bool(np.any(dis_var)),
-                             bool(np.any(obs_var)))
+    sur, dis, obs = (tuple(c.variance.tolist()) for c in components)
+    return PredictiveSummary(
+        sample_ids=tuple(r.sample_id for r in records),
+        samples=tuple(tuple(row.tolist()) for row in draws),
+        latent_mean=tuple(mean.tolist()), latent_variance=tuple(spread.tolist()),
+        surrogate_variance=sur, discrepancy_variance=dis, observation_variance=obs,
+        total_variance=tuple(total.tolist()),
+        includes_surrogate_noise=bool(components[0].variance.any()),
+        includes_discrepancy=bool(components[1].variance.any()),
+        includes_observation_noise=bool(components[2].variance.any()))
 
 
-def _finite_nonempty(values, what: str) -> np.ndarray:
-    a = np.asarray(values, dtype=float)
-    if a.ndim != 1 or a.size == 0 or not np.all(np.isfinite(a)):
-        raise ValueError(f"{what} must be a finite nonempty vector")
-    return a
-
-
+# ------------------------------------------------ conformal, scores, ranks
 def conformal_quantile(scores, alpha: float) -> float:
-    """The ceil((n+1)(1-alpha))-th smallest score, +inf past n (statistics.py:349-355)."""
-    s = _finite_nonempty(scores, "conformal scores")
-    if not math.isfinite(alpha) or not 0 < alpha < 1 or np.any(s < 0):
+    """The k-th smallest score with k = ceil((n + 1)(1 - alpha)), or +inf
+    when k exceeds n (statistics.py:349-355)."""
+    s = _finite_vector(scores, "conformal scores")
+    if not (math.isfinite(alpha) and 0.0 < alpha < 1.0) or (s < 0).any():
         raise ValueError("alpha must lie in (0,1) and scores must be nonnegative")
-    k = math.ceil((s.size + 1) * (1 - alpha))
-    if k > s.size:
-        return math.inf
-    return float(np.sort(s)[k - 1])
+    k = math.ceil((s.size + 1) * (1.0 - alpha))
+    return math.inf if k > s.size else float(np.partition(s, k - 1)[k - 1])
 
 
 def split_conformal_intervals(calibration_y, calibration_prediction, test_prediction, *,
                               alpha: float):
-    """Absolute-residual split conformal intervals (statistics.py:358-378)."""
-    y = _finite_nonempty(calibration_y, "calibration targets")
-    f = _finite_nonempty(calibration_prediction, "calibration predictions")
-    t = _finite_nonempty(test_prediction, "test predictions")
-    if y.shape != f.shape:
+    """Absolute-residual split conformal intervals: test predictions plus or
+    minus the conformal quantile of the calibration residuals
+    (statistics.py:358-378)."""
+    y = _finite_vector(calibration_y, "calibration targets")
+    fit = _finite_vector(calibration_prediction, "calibration predictions")
+    test = _finite_vector(test_prediction, "test predictions")
+    if y.shape != fit.shape:
         raise ValueError("calibration target and prediction shapes must match")
-    r = conformal_quantile(np.abs(y - f), alpha)
-    return t - r, t + r
+    radius = conformal_quantile(np.abs(y - fit), alpha)
+    return test - radius, test + radius
 
 
 def empirical_crps(samples, truth: float) -> float:
-    """CRPS of the equal-weight empirical forecast, M^2 denominator
-    (statistics.py:381-396): mean |x - y| minus the sorted-sample form of
-    half the mean pairwise distance."""
-    x = np.sort(_finite_nonempty(samples, "forecast samples"))
+    """CRPS of the equal-weight M-member forecast with the M^2 denominator,
+    E|X - y| - E|X - X'| / 2 (statistics.py:381-396). The pair term uses
+    the sorted members: sum over pairs i < j of (x_j - x_i) is
+    sum_j ((j - 1) x_j - (x_1 + ... + x_{j-1})), a prefix-sum pass instead
+    of the M^2 pairwise array; members are shifted by the truth first, which
+    changes no difference and keeps large offsets from cancelling."""
+    x = np.sort(_finite_vector(samples, "forecast samples"))
     if not math.isfinite(truth):
         raise ValueError("truth must be finite")
     m = x.size
-    c = x - truth
-    weights = 2.0 * np.arange(1, m + 1) - m - 1
-    value = float(np.mean(np.abs(c)) - np.dot(weights, c) / (m * m))
+    with np.errstate(over="ignore", invalid="ignore"):  # overflow is reported below
+        c = x - truth
+        before = np.concatenate(([0.0], np.cumsum(c)[:-1]))
+        pair_sum = float(np.sum(np.arange(m) * c - before))
+        value = float(np.mean(np.abs(c))) - pair_sum / (m * m)
     if not math.isfinite(value):
         raise ValueError("CRPS arithmetic must remain finite")
     return value
 
 
 def randomized_ranks(truth, draws, trial_ids: Sequence[int], *, root_seed: int) -> np.ndarray:
-    """Per trial and parameter: draws below the truth plus a uniform integer
-    in [0, ties] from the (trial ID, parameter) stream (statistics.py:399-426)."""
+    """Rank of each trial's truth among its draws, per parameter: the draws
+    strictly below it plus a uniform integer in [0, ties] from the stream of
+    (trial ID, "rank-tie:<parameter>") (statistics.py:399-426)."""
     t = np.asarray(truth, dtype=float)
     d = np.asarray(draws, dtype=float)
-    ids = _unique_ids(trial_ids)
-    _nonnegative_int(root_seed, "root seed")
-    aligned = (t.ndim == 2 and d.ndim == 3 and t.shape[0] == len(ids) and
-               d.shape[0] == len(ids) and d.shape[2] == t.shape[1] and t.size > 0 and
-               d.shape[1] > 0 and np.all(np.isfinite(t)) and np.all(np.isfinite(d)))
-    if not aligned:
+    ids = _distinct_ids(trial_ids)
+    _count(root_seed, "root seed")
+    n = len(ids)
+    if not (t.ndim == 2 and d.ndim == 3 and t.shape[0] == n == d.shape[0] and
+            d.shape[2] == t.shape[1] and t.size and d.shape[1] and np.isfinite(t).all() and
+            np.isfinite(d).all()):
         raise ValueError("truth [trial,parameter] and draws [trial,draw,parameter] must align")
-    below = np.sum(d < t[:, None, :], axis=1, dtype=np.int64)
-    ties = np.sum(d == t[:, None, :], axis=1)
-    for row, tid in enumerate(ids):
-        for p in range(t.shape[1]):
-            below[row, p] += sample_rng(root_seed, tid, f"rank-tie:{p}").integers(
-                int(ties[row, p]) + 1)
+    below = (d < t[:, None, :]).sum(axis=1, dtype=np.int64)
+    ties = (d == t[:, None, :]).sum(axis=1)
+    for (trial, param), k in np.ndenumerate(ties):
+        below[trial, param] += _stream(root_seed, ids[trial], f"rank-tie:{param}").integers(
+            int(k) + 1)
     return below
 
 
@@ -348,20 +406,22 @@ class RankHistograms:
 
 
 def rank_histograms(ranks, *, n_draws: int, bins: int = 20) -> RankHistograms:
-    """Per-parameter histograms of ranks in [0, n_draws] with the exact
-    discrete-uniform expectation per bin (statistics.py:435-462)."""
-    _nonnegative_int(n_draws, "draw count")
-    _nonnegative_int(bins, "bin count")
+    """Per-parameter histograms of ranks in [0, n_draws]: rank r falls in bin
+    floor(r * B / (n_draws + 1)) with B = min(bins, n_draws + 1), and each
+    bin's expected count under uniform ranks is its number of ranks times
+    trials / (n_draws + 1) (statistics.py:435-462). No p-value is given."""
+    _count(n_draws, "draw count")
+    _count(bins, "bin count")
     r = np.asarray(ranks)
-    valid = (n_draws >= 1 and bins >= 1 and r.ndim == 2 and r.size > 0 and
-             np.issubdtype(r.dtype, np.integer) and not np.any(r < 0) and
-             not np.any(r > n_draws))
-    if not valid:
+    if not (n_draws >= 1 and bins >= 1 and r.ndim == 2 and r.size and
+            np.issubdtype(r.dtype, np.integer) and r.min() >= 0 and r.max() <= n_draws):
         raise ValueError("ranks must be a nonempty integer [trial,parameter] array in [0,n_draws]")
-    nb = min(bins, n_draws + 1)
-    to_bin = np.arange(n_draws + 1, dtype=np.int64) * nb // (n_draws + 1)
-    per_bin = np.bincount(to_bin, minlength=nb)
-    counts = np.stack([np.bincount(to_bin[r[:, j]], minlength=nb)
-                       for j in range(r.shape[1])]).astype(np.int64)
-    expected = np.tile(per_bin * (r.shape[0] / (n_draws + 1)), (r.shape[1], 1))
+    n_bins = min(bins, n_draws + 1)
+    to_bin = (np.arange(n_draws + 1, dtype=np.int64) * n_bins) // (n_draws + 1)
+    counts = np.zeros((r.shape[1], n_bins), dtype=np.int64)
+    for param in range(r.shape[1]):
+        np.add.at(counts[param], to_bin[r[:, param]], 1)
+    per_bin = np.bincount(to_bin, minlength=n_bins)
+    expected = np.broadcast_to(per_bin * (r.shape[0] / (n_draws + 1)),
+                               (r.shape[1], n_bins)).copy()
     return RankHistograms(counts, expected, to_bin)
