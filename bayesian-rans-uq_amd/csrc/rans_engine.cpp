@@ -1663,9 +1663,22 @@ int guarded(char *err, size_t errcap, F &&f) {
 // ----------------------------------------------------------------------------
 namespace {
 
-// a later candidate replaces the best so far only when it is more than 1 %
-// faster, so shapes within timing noise of each other keep the earlier one
-constexpr double kTuneMargin = 0.99;
+// The kept shape is the first candidate, in the order tried (fewer threads
+// and fewer workgroups first), whose time is within 3 % of the fastest: the
+// box-to-box spread of these timings is about 1-2 %, so shapes within noise
+// of each other resolve the same way on every device.
+constexpr double kTuneSlack = 1.03;
+
+// index of the first finite time within kTuneSlack of the smallest, or -1
+int first_within_slack(const double *ms, int n) {
+    double fastest = std::numeric_limits<double>::infinity();
+    for (int i = 0; i < n; ++i)
+        if (std::isfinite(ms[i])) fastest = std::min(fastest, ms[i]);
+    if (!std::isfinite(fastest)) return -1;
+    for (int i = 0; i < n; ++i)
+        if (std::isfinite(ms[i]) && ms[i] <= kTuneSlack * fastest) return i;
+    return -1;
+}
 
 int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_params,
                   int32_t iterations, rq_tune_out *out) {
@@ -1694,8 +1707,6 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
                                  rate.data());
     const int nmax = (int)(sizeof(out->block_size) / sizeof(out->block_size[0]));
     out->n_candidates = 0;
-    double best = std::numeric_limits<double>::infinity();
-    int best_block = saved_block, best_grid = saved_grid;
     for (const Cand &c : cands) {
         if (out->n_candidates >= nmax) break;
         E->block = c.block;
@@ -1714,13 +1725,12 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
         out->grid_blocks[k] = c.grid;
         out->amg_block_rows[k] = E->settings.amg_block_rows;
         out->kernel_ms[k] = ms;
-        if (std::isfinite(ms) && ms < kTuneMargin * best) {
-            best = ms;
-            best_block = c.block;
-            best_grid = c.grid;
-        }
     }
-    if (std::isfinite(best)) {
+    const int pick = first_within_slack(out->kernel_ms, out->n_candidates);
+    double best = pick >= 0 ? out->kernel_ms[pick] : std::numeric_limits<double>::infinity();
+    const int best_block = pick >= 0 ? out->block_size[pick] : saved_block;
+    const int best_grid = pick >= 0 ? out->grid_blocks[pick] : saved_grid;
+    if (pick >= 0) {
         E->block = best_block;
         E->block_user = true;
         E->settings.grid_blocks = best_grid;
@@ -1733,8 +1743,11 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
     // below which a level runs on one workgroup, at the chosen shape (it
     // changes only which workgroups compute a row, not the row's arithmetic)
     const int saved_rows = E->settings.amg_block_rows;
-    if (std::isfinite(best) && best_grid > 1 && E->mesh_host.amg.levels > 1) {
-        int best_rows = saved_rows;
+    if (pick >= 0 && best_grid > 1 && E->mesh_host.amg.levels > 1) {
+        // the configured threshold first, then the alternatives
+        double rows_ms[4] = {best, 0, 0, 0};
+        int rows_of[4] = {saved_rows, 0, 0, 0};
+        int nr = 1;
         for (int rows : {256, 512, 2048}) {
             if (out->n_candidates >= nmax || rows == saved_rows) continue;
             E->settings.amg_block_rows = rows;
@@ -1749,12 +1762,11 @@ int autotune_impl(rq_engine *E, const double *theta, int64_t n_rows, int32_t n_p
             out->grid_blocks[k] = best_grid;
             out->amg_block_rows[k] = rows;
             out->kernel_ms[k] = ms;
-            if (std::isfinite(ms) && ms < kTuneMargin * best) {
-                best = ms;
-                best_rows = rows;
-            }
+            rows_ms[nr] = ms;
+            rows_of[nr++] = rows;
         }
-        E->settings.amg_block_rows = best_rows;
+        const int r = first_within_slack(rows_ms, nr);
+        E->settings.amg_block_rows = r >= 0 ? rows_of[r] : saved_rows;
     }
     out->chosen_block_size = E->block;
     out->chosen_grid_blocks = E->settings.grid_blocks;

@@ -165,12 +165,24 @@ def solve_bytes(N, F, B, iters, krylov, turb_start, settings, levels=None):
 
 
 # --------------------------------------------------------------- helpers
+def _host_cpus():
+    """CPUs this process may run on (its affinity mask)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def _cores():
+    """Workers for the CPU baselines: every CPU of the process's affinity
+    mask, within the CPU share the host grants the job (OMP_NUM_THREADS when
+    it is set: on a shared GPU host more workers than the share would only
+    queue behind each other)."""
+    n = _host_cpus()
     env = os.environ.get("OMP_NUM_THREADS")
-    n = os.cpu_count() or 1
-    if env and env.isdigit():
+    if env and env.isdigit() and int(env) > 0:
         n = min(n, int(env))
-    return max(1, min(n, 16))
+    return max(1, n)
 
 
 def _traffic_from_profile():

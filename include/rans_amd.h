@@ -184,13 +184,14 @@ void rq_engine_destroy(rq_engine *engine);
  * candidate workgroup size (64, 128, 256, 512)
  * and, for arenas in HBM of at least 1024 cells, every workgroups-per-draw
  * count 2..64 that the device can hold runs the given rows with each solve
- * capped at `iterations` outer iterations, in the order listed; a candidate
- * replaces the best so far when its kernel time is more than 1 % shorter, and
- * the result becomes the engine's launch shape for later evaluations. When the fastest shape
- * spreads draws with an AMG pressure solve over several workgroups, the
- * one-workgroup multigrid threshold (amg_block_rows 256, 512, 2048 against
- * the configured one) is timed at that shape next. kernel_ms is NaN for a
- * shape that could not run. */
+ * capped at `iterations` outer iterations, in the order listed. The kept
+ * shape is the first one in that order whose kernel time is within 3 % of
+ * the fastest (shapes within timing noise of each other resolve the same way
+ * on every device), and it becomes the engine's launch shape for later
+ * evaluations. When the kept shape spreads draws with an AMG pressure solve
+ * over several workgroups, the one-workgroup multigrid threshold
+ * (amg_block_rows: the configured one, then 256, 512, 2048) is chosen the
+ * same way at that shape. kernel_ms is NaN for a shape that could not run. */
 typedef struct rq_tune_out {
     int32_t n_candidates;
     int32_t block_size[16], grid_blocks[16], amg_block_rows[16];
