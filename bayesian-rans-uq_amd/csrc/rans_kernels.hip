@@ -201,7 +201,8 @@ struct GRefF {
 template <int BS>
 struct Reducer {
     static constexpr int NW = BS / 64;
-    double *buf;  // LDS scratch [2][NW][4]
+    static constexpr int kStride = 8;  // values per wave in one reduction
+    double *buf;  // LDS scratch [2][NW][kStride]
     int parity;
     // multi-workgroup draws: block totals meet in the draw's GridState and
     // every workgroup adds them in rank order (the same bits everywhere)
@@ -211,19 +212,21 @@ struct Reducer {
 
     template <int NV, bool MAX>
     DEV void run(double (&v)[NV]) {
+        static_assert(NV >= 1 && NV <= kStride, "a reduction takes at most 8 values");
 #pragma unroll
         for (int i = 0; i < NV; ++i) v[i] = MAX ? wave_max(v[i]) : wave_sum(v[i]);
         if constexpr (NW > 1) {
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            double *b = buf + parity * (NW * 4);
+            double *b = buf + parity * (NW * kStride);
             if (lane == 0)
 #pragma unroll
-                for (int i = 0; i < NV; ++i) b[wave * 4 + i] = v[i];
+                for (int i = 0; i < NV; ++i) b[wave * kStride + i] = v[i];
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 double a = b[i];
-                for (int w = 1; w < NW; ++w) a = MAX ? smax(a, b[w * 4 + i]) : a + b[w * 4 + i];
+                for (int w = 1; w < NW; ++w)
+                    a = MAX ? smax(a, b[w * kStride + i]) : a + b[w * kStride + i];
                 v[i] = a;
             }
             parity ^= 1;
@@ -231,22 +234,27 @@ struct Reducer {
             __syncthreads();  // orders the phase's LDS writes before later reads
         }
         if (grid) {
-            double *pt = &grid->partials[gparity][0][0];
-            if (threadIdx.x == 0)
+            // the grid exchange carries 4 values per workgroup at a time
 #pragma unroll
-                for (int i = 0; i < NV; ++i) pt[rank * 4 + i] = v[i];
-            grid_barrier(grid, nblk, gl);
-            // device-scope loads: the partials come from other workgroups
-            auto ld = [&](int k) {
-                return __hip_atomic_load(pt + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            };
+            for (int i0 = 0; i0 < NV; i0 += 4) {
+                double *pt = &grid->partials[gparity][0][0];
+                if (threadIdx.x == 0)
 #pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                double a = ld(i);
-                for (int r = 1; r < nblk; ++r) a = MAX ? smax(a, ld(r * 4 + i)) : a + ld(r * 4 + i);
-                v[i] = a;
+                    for (int i = i0; i < NV && i < i0 + 4; ++i) pt[rank * 4 + i - i0] = v[i];
+                grid_barrier(grid, nblk, gl);
+                // device-scope loads: the partials come from other workgroups
+                auto ld = [&](int k) {
+                    return __hip_atomic_load(pt + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                };
+#pragma unroll
+                for (int i = i0; i < NV && i < i0 + 4; ++i) {
+                    double a = ld(i - i0);
+                    for (int r = 1; r < nblk; ++r)
+                        a = MAX ? smax(a, ld(r * 4 + i - i0)) : a + ld(r * 4 + i - i0);
+                    v[i] = a;
+                }
+                gparity ^= 1;
             }
-            gparity ^= 1;
         }
     }
     template <int NV>
@@ -313,6 +321,8 @@ struct Draw {
     // cells, faces and matrix rows are distributed over gs threads; a thread's
     // first index is gt (in one-workgroup mode: the workgroup's threads)
     int gt = threadIdx.x, gs = BS;
+    // cell variable the wave-resident Krylov solvers store their solution in
+    int xdst = V_kx;
     // multi-workgroup draws: this workgroup's rank among nblk
     int rank = 0, nblk = 1;
     GridState *grid = nullptr;
@@ -530,43 +540,73 @@ struct Draw {
         nb_[0] = b.x; nb_[1] = b.y; nb_[2] = b.z; nb_[3] = b.w;
     }
 
+    // A cell's slots as a branch-free set: every index is clamped to a record
+    // that exists (face 0, boundary face 0, the cell itself), so a row loads
+    // the data of all its slots at once and the slots' arithmetic interleaves;
+    // the contribution of a slot that does not apply is dropped with a select.
+    // Every sum keeps its slot order and its operations, so the results are
+    // bitwise those of a loop that branches per slot. Internal faces need no
+    // owner/neighbour lookup: the face's other cell is the slot's neighbour,
+    // and the slot side says which of the two is the owner.
+    struct SlotSet {
+        int f[kSlots], b[kSlots], n[kSlots];
+        bool valid[kSlots], inner[kSlots], own[kSlots];
+    };
+    DEV void slot_set(int ci, SlotSet &q) const {
+        int code[kSlots], nbv[kSlots];
+        slots(ci, code, nbv);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            q.valid[s] = code[s] >= 0;
+            const int f = q.valid[s] ? slot_face(code[s]) : 0;
+            q.f[s] = f;
+            q.inner[s] = nbv[s] >= 0;
+            q.own[s] = slot_own(code[s]);
+            q.b[s] = (q.valid[s] && !q.inner[s]) ? f - nif : 0;
+            q.n[s] = q.inner[s] ? nbv[s] : ci;
+        }
+    }
+
     // ----------------------------------------------------------- gradients
     // greenGaussGrad (core/include/Field.hpp:178-212) of NF fields at one
     // cell, its faces gathered in face order.
     template <int NF>
     DEV void gg_cell(int ci, const int (&ph)[NF], const int (&phb)[NF], double (&gx)[NF],
                      double (&gy)[NF]) const {
-        int code[kSlots], nbv[kSlots];
-        slots(ci, code, nbv);
+        SlotSet q;
+        slot_set(ci, q);
+        double xc[NF];
 #pragma unroll
-        for (int i = 0; i < NF; ++i) gx[i] = gy[i] = 0.0;
+        for (int i = 0; i < NF; ++i) {
+            gx[i] = gy[i] = 0.0;
+            xc[i] = C(ci, ph[i]);
+        }
 #pragma unroll
         for (int s = 0; s < kSlots; ++s) {
-            if (code[s] < 0) continue;
-            const int f = slot_face(code[s]);
+            const int f = q.f[s];
             const double Sx = Fc(f, W_Sx), Sy = Fc(f, W_Sy);
-            if (f < nif) {
-                const v2i on = fon[f];
-                const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
-                const bool own = slot_own(code[s]);
+            const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+            const bool own = q.own[s], in = q.inner[s], on = q.valid[s];
+            // boundary values only where the slot is a boundary face (few
+            // cells have one): a masked load, with no arithmetic under it
+            double pb[NF];
 #pragma unroll
-                for (int i = 0; i < NF; ++i) {
-                    const double phiF = C(on.x, ph[i]) * wt + C(on.y, ph[i]) * wmv;
-                    if (own) {
-                        gx[i] = gx[i] + Sx * phiF;
-                        gy[i] = gy[i] + Sy * phiF;
-                    } else {
-                        gx[i] = gx[i] - Sx * phiF;
-                        gy[i] = gy[i] - Sy * phiF;
-                    }
-                }
-            } else {
+            for (int i = 0; i < NF; ++i) pb[i] = 0.0;
+            if (on && !in)
 #pragma unroll
-                for (int i = 0; i < NF; ++i) {
-                    const double pbv = Bd(f - nif, phb[i]);
-                    gx[i] = gx[i] + Sx * pbv;
-                    gy[i] = gy[i] + Sy * pbv;
-                }
+                for (int i = 0; i < NF; ++i) pb[i] = Bd(q.b[s], phb[i]);
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+                const double xn = C(q.n[s], ph[i]);
+                const double pbv = pb[i];
+                // internal: owner value * w + neighbour value * (1 - w)
+                const double phiF = (own ? xc[i] : xn) * wt + (own ? xn : xc[i]) * wmv;
+                const double v = in ? phiF : pbv;
+                const bool add = !in || own;
+                const double ax = add ? gx[i] + Sx * v : gx[i] - Sx * v;
+                const double ay = add ? gy[i] + Sy * v : gy[i] - Sy * v;
+                gx[i] = on ? ax : gx[i];
+                gy[i] = on ? ay : gy[i];
             }
         }
         const double V = C(ci, V_vol);
@@ -594,28 +634,31 @@ struct Draw {
     // F (phi_f^HO - phi_U) of the deferred convection correction at an
     // internal face (SIMPLESolver.cpp:495-532), 0 when the face flux is 0.
     // The host stores x_f - x_U in dO/dN for linear upwind and x_D - x_U
-    // for the limited scheme.
-    DEV double convection_corr(int f, v2i on, int ph, int gx, int gy, int scheme) const {
-        const double Fl = Fc(f, W_phi);
-        if (Fl == 0.0) return 0.0;
+    // for the limited scheme. Branch-free over the slot: the values of the
+    // cell (c) and of the slot's neighbour (n) are loaded, own says whether
+    // the cell is the face's owner.
+    DEV double convection_corr(int f, bool own, double Fl, double phc, double gxc, double gyc,
+                               int n, int ph, int gx, int gy, int scheme) const {
         const bool ownerUp = Fl > 0.0;
-        const int up = ownerUp ? on.x : on.y, dn = ownerUp ? on.y : on.x;
-        const double dx = ownerUp ? Fc(f, W_dOx) : Fc(f, W_dNx);
-        const double dy = ownerUp ? Fc(f, W_dOy) : Fc(f, W_dNy);
-        const double pu = C(up, ph);
+        const bool upIsCell = own == ownerUp;  // the upwind cell is this cell
+        const double phn = C(n, ph), gxn = C(n, gx), gyn = C(n, gy);
+        const double dOx = Fc(f, W_dOx), dOy = Fc(f, W_dOy), dNx = Fc(f, W_dNx), dNy = Fc(f, W_dNy);
+        const double dx = ownerUp ? dOx : dNx;
+        const double dy = ownerUp ? dOy : dNy;
+        const double pu = upIsCell ? phc : phn;
+        const double gux = upIsCell ? gxc : gxn, guy = upIsCell ? gyc : gyn;
         double phiF;
         if (scheme == CONV_LINEAR_UPWIND) {
-            phiF = pu + (C(up, gx) * dx + C(up, gy) * dy);
+            phiF = pu + (gux * dx + guy * dy);
         } else {
-            const double dphi = C(dn, ph) - pu;
-            double psi = 0.0;
-            if (fabs(dphi) > 1e-300) {
-                const double r = 2.0 * (C(up, gx) * dx + C(up, gy) * dy) / dphi - 1.0;
-                psi = r > 0.0 ? (r * r + r) / (r * r + 1.0) : 0.0;
-            }
+            const double pd = upIsCell ? phn : phc;
+            const double dphi = pd - pu;
+            const double r = 2.0 * (gux * dx + guy * dy) / dphi - 1.0;
+            const double psiR = r > 0.0 ? (r * r + r) / (r * r + 1.0) : 0.0;
+            const double psi = fabs(dphi) > 1e-300 ? psiR : 0.0;
             phiF = pu + 0.5 * psi * dphi;
         }
-        return Fl * (phiF - pu);
+        return Fl == 0.0 ? 0.0 : Fl * (phiF - pu);
     }
 
     // y = A x for row ci (LinearSystem::matvec order: diagonal, then faces)
@@ -624,8 +667,11 @@ struct Draw {
         const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
         double y = C(ci, V_diag) * C(ci, x);
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) y += off(ci, s) * C(nbv[s], x);
+        for (int s = 0; s < kSlots; ++s) {
+            // every slot's loads issue together; unused slots are dropped
+            const double a = off(ci, s), xn = C(nbv[s] >= 0 ? nbv[s] : ci, x);
+            y = nbv[s] >= 0 ? y + a * xn : y;
+        }
         return y;
     }
 
@@ -634,8 +680,10 @@ struct Draw {
         const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
         bool ok = isfinite(dg) && isfinite(sr);
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) ok = ok && isfinite(off(ci, s));
+        for (int s = 0; s < kSlots; ++s) {
+            const double a = off(ci, s);
+            ok = ok && (nbv[s] < 0 || isfinite(a));
+        }
         return ok;
     }
 
@@ -647,43 +695,41 @@ struct Draw {
         const double nu = S.nu;
         const int Ubv = comp == 0 ? B_Ux : B_Uy;
         const int Uc = comp == 0 ? V_Ux : V_Uy;
-        int code[kSlots], nbv[kSlots];
-        slots(ci, code, nbv);
+        SlotSet q;
+        slot_set(ci, q);
         double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
         const double nuTc = C(ci, V_nuT);
+        double Sf[kSlots], mFlux[kSlots], wt[kSlots], wmv[kSlots], nuTn[kSlots];
+        bool wall[kSlots];
 #pragma unroll
         for (int s = 0; s < kSlots; ++s) {
-            if (code[s] < 0) continue;
-            const int f = slot_face(code[s]);
-            const double Sf = Fc(f, W_A);
+            const int f = q.f[s];
+            Sf[s] = Fc(f, W_A);
             const double delta = Fc(f, W_dlt);
-            const double mFlux = Fc(f, W_phi);
-            if (f < nif) {
-                const double nuEff_o = nu + (slot_own(code[s]) ? nuTc : C(nbv[s], V_nuT));
-                const double nuEff_n = nu + (slot_own(code[s]) ? C(nbv[s], V_nuT) : nuTc);
-                const double nuEff_f = Fc(f, W_w) * nuEff_o + Fc(f, W_wm) * nuEff_n;
-                const double Df = nuEff_f * Sf / delta;
-                const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
-                if (slot_own(code[s])) {
-                    dg += Df + cPos;
-                    offv[s] = -(Df + cNeg);
-                } else {
-                    dg += Df + cNeg;
-                    offv[s] = -(Df + cPos);
-                }
-            } else {
-                const int b = f - nif;
-                const bool wall = bflag[b] & 1;
-                const double nuEff = wall ? nu : nu + nuTc;
-                const double Db = nuEff * Sf / delta;
-                if (mFlux >= 0) {
-                    dg += mFlux + Db;
-                    sr += Db * Bd(b, Ubv);
-                } else {
-                    dg += Db;
-                    sr += (Db - mFlux) * Bd(b, Ubv);
-                }
-            }
+            mFlux[s] = Fc(f, W_phi);
+            wt[s] = Fc(f, W_w);
+            wmv[s] = Fc(f, W_wm);
+            nuTn[s] = C(q.n[s], V_nuT);
+            wall[s] = bflag[q.b[s]] & kFlagWall;
+            const double bU = Bd(q.b[s], Ubv);
+            const bool own = q.own[s], in = q.inner[s];
+            // internal face: nuEff interpolated from owner and neighbour;
+            // boundary face: nu at a wall, nu + nuT of the cell elsewhere
+            const double nuEff_o = nu + (own ? nuTc : nuTn[s]);
+            const double nuEff_n = nu + (own ? nuTn[s] : nuTc);
+            const double nuEff_f = wt[s] * nuEff_o + wmv[s] * nuEff_n;
+            const double nuEffB = wall[s] ? nu : nu + nuTc;
+            const double D = (in ? nuEff_f : nuEffB) * Sf[s] / delta;
+            const double m = mFlux[s];
+            const double cPos = smax(m, 0.0), cNeg = smax(-m, 0.0);
+            const double dIn = own ? D + cPos : D + cNeg;
+            const double oIn = -(own ? D + cNeg : D + cPos);
+            const bool pos = m >= 0;
+            const double dB = pos ? m + D : D;
+            const double sB = (pos ? D : D - m) * bU;
+            dg = q.valid[s] ? dg + (in ? dIn : dB) : dg;
+            sr = (q.valid[s] && !in) ? sr + sB : sr;
+            offv[s] = in ? oIn : 0.0;
         }
         const double V = C(ci, V_vol);
         const double dpdx = comp == 0 ? C(ci, V_gPx) : C(ci, V_gPy);
@@ -699,46 +745,38 @@ struct Draw {
             double tsrc = 0.0;
 #pragma unroll
             for (int s = 0; s < kSlots; ++s) {
-                if (code[s] < 0) continue;
-                const int f = slot_face(code[s]);
-                if (f < nif) {
-                    const bool own = slot_own(code[s]);
-                    const int n = nbv[s];
-                    const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
-                    const double nuo = own ? nuTc : C(n, V_nuT), nun = own ? C(n, V_nuT) : nuTc;
-                    const double gan_ = C(n, ga), gbn_ = C(n, gb);
-                    const double gao = own ? gac : gan_, gan = own ? gan_ : gac;
-                    const double gbo = own ? gbc : gbn_, gbn = own ? gbn_ : gbc;
-                    const double cf = wt * nuo + wmv * nun;
-                    const double gxf = gao * wt + gan * wmv;
-                    const double gyf = gbo * wt + gbn * wmv;
-                    const double flux = cf * (gxf * Fc(f, W_nx) + gyf * Fc(f, W_ny)) * Fc(f, W_A);
-                    if (own) tsrc += flux;
-                    else tsrc -= flux;
-                } else {
-                    const bool wall = bflag[f - nif] & 1;
-                    const double cf = wall ? 0.0 : nuTc;
-                    const double flux = cf * (gac * Fc(f, W_nx) + gbc * Fc(f, W_ny)) * Fc(f, W_A);
-                    tsrc += flux;
-                }
+                const int f = q.f[s];
+                const bool own = q.own[s], in = q.inner[s];
+                const double gan_ = C(q.n[s], ga), gbn_ = C(q.n[s], gb);
+                const double nx = Fc(f, W_nx), ny = Fc(f, W_ny);
+                const double nuo = own ? nuTc : nuTn[s], nun = own ? nuTn[s] : nuTc;
+                const double gao = own ? gac : gan_, gan = own ? gan_ : gac;
+                const double gbo = own ? gbc : gbn_, gbn = own ? gbn_ : gbc;
+                const double cfI = wt[s] * nuo + wmv[s] * nun;
+                const double gxf = gao * wt[s] + gan * wmv[s];
+                const double gyf = gbo * wt[s] + gbn * wmv[s];
+                const double cf = in ? cfI : (wall[s] ? 0.0 : nuTc);
+                const double gX = in ? gxf : gac, gY = in ? gyf : gbc;
+                const double flux = cf * (gX * nx + gY * ny) * Sf[s];
+                const double t = (!in || own) ? tsrc + flux : tsrc - flux;
+                tsrc = q.valid[s] ? t : tsrc;
             }
             sr += tsrc;
         }
+        const double uc = C(ci, Uc);
         if (S.momentum_convection != CONV_UPWIND) {
             const int gx = comp == 0 ? V_gUxx : V_gUyx;
             const int gy = comp == 0 ? V_gUxy : V_gUyy;
+            const double gxc = C(ci, gx), gyc = C(ci, gy);
 #pragma unroll
             for (int s = 0; s < kSlots; ++s) {
-                if (code[s] < 0) continue;
-                const int f = slot_face(code[s]);
-                if (f >= nif) continue;
-                const double corr = convection_corr(f, fon[f], Uc, gx, gy, S.momentum_convection);
-                if (slot_own(code[s])) sr -= corr;
-                else sr += corr;
+                const double corr = convection_corr(q.f[s], q.own[s], mFlux[s], uc, gxc, gyc,
+                                                    q.n[s], Uc, gx, gy, S.momentum_convection);
+                const double t = q.own[s] ? sr - corr : sr + corr;
+                sr = q.inner[s] ? t : sr;
             }
         }
         const double alphaU = S.alpha_u;
-        const double uc = C(ci, Uc);
         aPu = dg;
         sr += (1.0 - alphaU) / alphaU * dg * uc;
         dg /= alphaU;
@@ -748,79 +786,94 @@ struct Draw {
         ok = isfinite(dg) && isfinite(sr) && isfinite(uc);
         double y = dg * uc;
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) {
-                off(ci, s) = offv[s];
-                ok = ok && isfinite(offv[s]);
-                y += offv[s] * C(nbv[s], Uc);
-            }
+        for (int s = 0; s < kSlots; ++s) {
+            const double xn = C(q.n[s], Uc);
+            if (q.inner[s]) off(ci, s) = offv[s];
+            ok = ok && (!q.inner[s] || isfinite(offv[s]));
+            y = q.inner[s] ? y + offv[s] * xn : y;
+        }
         return sr - y;
     }
 
     // ---------------------------------------------------------- k / omega rows
-    // SIMPLESolver::assembleKEquation (SIMPLESolver.cpp:947-1023)
-    DEV void k_row(int ci, bool &ok) {
+    // Diffusion and upwind convection of one k or omega row over its slots
+    // (the face loops of assembleKEquation / assembleOmegaEquation): sig0 and
+    // sig1 are the inner and outer diffusion coefficients blended by F1,
+    // Bv the boundary variable. Fills dg, sr and offv; Sf, delta, nuEff of
+    // internal faces and the face fluxes are returned for the callers' other
+    // slot terms.
+    struct TurbSlots {
+        double Sf[kSlots], delta[kSlots], mFlux[kSlots], nuEff[kSlots];
+    };
+    DEV void turb_faces(int ci, const SlotSet &q, double sig0, double sig1, int Bv, double F1c,
+                        double nuTc, double &dg, double &sr, double (&offv)[kSlots],
+                        TurbSlots &t) const {
         const double nu = S.nu;
-        int code[kSlots], nbv[kSlots];
-        slots(ci, code, nbv);
-        double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
-        const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
+        // boundary faces: the cell's blended coefficient
+        const double sgB = blend(sig0, sig1, F1c);
+        const double nuEffB = nu + sgB * nuTc;
 #pragma unroll
         for (int s = 0; s < kSlots; ++s) {
-            if (code[s] < 0) continue;
-            const int f = slot_face(code[s]);
+            const int f = q.f[s];
             const double Sf = Fc(f, W_A);
             const double delta = Fc(f, W_dlt);
-            const double mFlux = Fc(f, W_phi);
-            if (f < nif) {
-                const bool own = slot_own(code[s]);
-                const int n = nbv[s];
-                const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
-                const double F1o = own ? F1c : C(n, V_F1), F1n = own ? C(n, V_F1) : F1c;
-                const double nto = own ? nuTc : C(n, V_nuT), ntn = own ? C(n, V_nuT) : nuTc;
-                const double F1f = wt * F1o + wmv * F1n;
-                const double sk = blend(c[0], c[4], F1f);
-                const double nuEff_o = nu + sk * nto;
-                const double nuEff_n = nu + sk * ntn;
-                const double nuEff_f = wt * nuEff_o + wmv * nuEff_n;
-                const double Df = nuEff_f * Sf / delta;
-                const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
-                if (own) {
-                    dg += Df + cPos;
-                    offv[s] = -(Df + cNeg);
-                } else {
-                    dg += Df + cNeg;
-                    offv[s] = -(Df + cPos);
-                }
-            } else {
-                const int b = f - nif;
-                const double sk = blend(c[0], c[4], F1c);
-                const double nuEff = nu + sk * nuTc;
-                const double Db = nuEff * Sf / delta;
-                if (mFlux >= 0) {
-                    dg += mFlux + Db;
-                    sr += Db * Bd(b, B_k);
-                } else {
-                    dg += Db;
-                    sr += (Db - mFlux) * Bd(b, B_k);
-                }
-            }
+            const double m = Fc(f, W_phi);
+            const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+            const double F1n = C(q.n[s], V_F1), ntn_ = C(q.n[s], V_nuT);
+            const double bv = Bd(q.b[s], Bv);
+            const bool own = q.own[s], in = q.inner[s];
+            const double F1o = own ? F1c : F1n, F1nb = own ? F1n : F1c;
+            const double nto = own ? nuTc : ntn_, ntn = own ? ntn_ : nuTc;
+            const double F1f = wt * F1o + wmv * F1nb;
+            const double sg = blend(sig0, sig1, F1f);
+            const double nuEff_o = nu + sg * nto;
+            const double nuEff_n = nu + sg * ntn;
+            const double nuEff_f = wt * nuEff_o + wmv * nuEff_n;
+            const double D = (in ? nuEff_f : nuEffB) * Sf / delta;
+            const double cPos = smax(m, 0.0), cNeg = smax(-m, 0.0);
+            const double dIn = own ? D + cPos : D + cNeg;
+            const double oIn = -(own ? D + cNeg : D + cPos);
+            const bool pos = m >= 0;
+            const double dB = pos ? m + D : D;
+            const double sB = (pos ? D : D - m) * bv;
+            dg = q.valid[s] ? dg + (in ? dIn : dB) : dg;
+            sr = (q.valid[s] && !in) ? sr + sB : sr;
+            offv[s] = in ? oIn : 0.0;
+            t.Sf[s] = Sf;
+            t.delta[s] = delta;
+            t.mFlux[s] = m;
+            t.nuEff[s] = nuEff_f;
         }
+    }
+
+    // deferred convection correction of a k or omega row (internal slots)
+    DEV double turb_convection(int ci, const SlotSet &q, const TurbSlots &t, double sr, int ph,
+                               int gx, int gy) const {
+        const double phc = C(ci, ph), gxc = C(ci, gx), gyc = C(ci, gy);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const double corr = convection_corr(q.f[s], q.own[s], t.mFlux[s], phc, gxc, gyc,
+                                                q.n[s], ph, gx, gy, S.turbulence_convection);
+            const double v = q.own[s] ? sr - corr : sr + corr;
+            sr = q.inner[s] ? v : sr;
+        }
+        return sr;
+    }
+
+    // SIMPLESolver::assembleKEquation (SIMPLESolver.cpp:947-1023); on return
+    // kr holds the row's residual b - A k.
+    DEV void k_row(int ci, bool &ok, double &resid) {
+        SlotSet q;
+        slot_set(ci, q);
+        double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
+        const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
+        TurbSlots t;
+        turb_faces(ci, q, c[0], c[4], B_k, F1c, nuTc, dg, sr, offv, t);
         const double V = C(ci, V_vol);
         sr += C(ci, V_Pk) * V;
         const double destruction = c[8] * smax(C(ci, V_w), 1e-20);
         dg += destruction * V;
-        if (S.turbulence_convection != CONV_UPWIND) {
-#pragma unroll
-            for (int s = 0; s < kSlots; ++s) {
-                if (code[s] < 0) continue;
-                const int f = slot_face(code[s]);
-                if (f >= nif) continue;
-                const double corr = convection_corr(f, fon[f], V_k, V_gKx, V_gKy, S.turbulence_convection);
-                if (slot_own(code[s])) sr -= corr;
-                else sr += corr;
-            }
-        }
+        if (S.turbulence_convection != CONV_UPWIND) sr = turb_convection(ci, q, t, sr, V_k, V_gKx, V_gKy);
         const double aK = S.alpha_k;
         const double kc = C(ci, V_k);
         sr += (1.0 - aK) / aK * dg * kc;
@@ -828,73 +881,46 @@ struct Draw {
         C(ci, V_diag) = dg;
         C(ci, V_src) = sr;
         ok = isfinite(dg) && isfinite(sr) && isfinite(kc);
+        // b - A x at x = k (LinearSystem::matvec order: diagonal, then faces)
+        double y = dg * kc;
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) {
-                off(ci, s) = offv[s];
-                ok = ok && isfinite(offv[s]);
-            }
+        for (int s = 0; s < kSlots; ++s) {
+            const double xn = C(q.n[s], V_k);
+            if (q.inner[s]) off(ci, s) = offv[s];
+            ok = ok && (!q.inner[s] || isfinite(offv[s]));
+            y = q.inner[s] ? y + offv[s] * xn : y;
+        }
+        resid = sr - y;
     }
 
     // SIMPLESolver::assembleOmegaEquation (SIMPLESolver.cpp:1030-1169) and
     // constrainOmega (SIMPLESolver.cpp:686-704); returns the constrained
-    // initial value of the omega solve.
-    DEV double omega_row(int ci, bool &ok) {
+    // initial value x0 of the omega solve and the row's residual b - A x0
+    // (every cell's x0 is its wall value where it has one, else omega, so
+    // the neighbours' x0 are known without a barrier).
+    DEV double omega_row(int ci, bool &ok, double &resid) {
         const double nu = S.nu;
-        int code[kSlots], nbv[kSlots];
-        slots(ci, code, nbv);
+        SlotSet q;
+        slot_set(ci, q);
         double dg = 0.0, sr = 0.0, balance = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
         const double amplitude = 6 * nu / c[2];
         const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
-        const double omsc = planar ? C(ci, V_oms) : 0.0;
+        TurbSlots t;
+        turb_faces(ci, q, c[1], c[5], B_w, F1c, nuTc, dg, sr, offv, t);
+        if (planar) {
+            const double omsc = C(ci, V_oms);
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-            if (code[s] < 0) continue;
-            const int f = slot_face(code[s]);
-            const double Sf = Fc(f, W_A);
-            const double delta = Fc(f, W_dlt);
-            const double mFlux = Fc(f, W_phi);
-            if (f < nif) {
-                const bool own = slot_own(code[s]);
-                const int n = nbv[s];
-                const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
-                const double F1o = own ? F1c : C(n, V_F1), F1n = own ? C(n, V_F1) : F1c;
-                const double nto = own ? nuTc : C(n, V_nuT), ntn = own ? C(n, V_nuT) : nuTc;
-                const double F1f = wt * F1o + wmv * F1n;
-                const double sw = blend(c[1], c[5], F1f);
-                const double nuEff_o = nu + sw * nto;
-                const double nuEff_n = nu + sw * ntn;
-                const double nuEff_f = wt * nuEff_o + wmv * nuEff_n;
-                const double Df = nuEff_f * Sf / delta;
-                const double cPos = smax(mFlux, 0.0), cNeg = smax(-mFlux, 0.0);
-                if (own) {
-                    dg += Df + cPos;
-                    offv[s] = -(Df + cNeg);
-                } else {
-                    dg += Df + cNeg;
-                    offv[s] = -(Df + cPos);
-                }
-                if (planar) {
-                    const double omsn = C(n, V_oms);
-                    const double so = own ? omsc : omsn, sn = own ? omsn : omsc;
-                    const double difference = (sn - so) / delta;
-                    const double correction =
-                        amplitude * Sf * ((nuEff_f - nu) * Fc(f, W_omg) - nuEff_f * difference);
-                    if (own) balance += correction;
-                    else balance -= correction;
-                }
-            } else {
-                const int b = f - nif;
-                const double sw = blend(c[1], c[5], F1c);
-                const double nuEff = nu + sw * nuTc;
-                const double Db = nuEff * Sf / delta;
-                if (mFlux >= 0) {
-                    dg += mFlux + Db;
-                    sr += Db * Bd(b, B_w);
-                } else {
-                    dg += Db;
-                    sr += (Db - mFlux) * Bd(b, B_w);
-                }
+            for (int s = 0; s < kSlots; ++s) {
+                const bool own = q.own[s];
+                const double omsn = C(q.n[s], V_oms);
+                const double omg = Fc(q.f[s], W_omg);
+                const double so = own ? omsc : omsn, sn = own ? omsn : omsc;
+                const double difference = (sn - so) / t.delta[s];
+                const double nuEff_f = t.nuEff[s];
+                const double correction =
+                    amplitude * t.Sf[s] * ((nuEff_f - nu) * omg - nuEff_f * difference);
+                const double v = own ? balance + correction : balance - correction;
+                balance = q.inner[s] ? v : balance;
             }
         }
         const double V = C(ci, V_vol);
@@ -911,46 +937,40 @@ struct Draw {
             if (correction < 0) dg -= correction / omC;
             else sr += correction;
         }
-        if (S.turbulence_convection != CONV_UPWIND) {
-#pragma unroll
-            for (int s = 0; s < kSlots; ++s) {
-                if (code[s] < 0) continue;
-                const int f = slot_face(code[s]);
-                if (f >= nif) continue;
-                const double corr = convection_corr(f, fon[f], V_w, V_gWx, V_gWy, S.turbulence_convection);
-                if (slot_own(code[s])) sr -= corr;
-                else sr += corr;
-            }
-        }
+        if (S.turbulence_convection != CONV_UPWIND) sr = turb_convection(ci, q, t, sr, V_w, V_gWx, V_gWy);
         const double aW = S.alpha_omega;
         sr += (1.0 - aW) / aW * dg * wc;
         dg /= aW;
         const double wvc = C(ci, V_wallv);
+        double x0n[kSlots];
 #pragma unroll
         for (int s = 0; s < kSlots; ++s) {
-            if (nbv[s] < 0) continue;
-            const bool own = slot_own(code[s]);
-            const double vn_ = C(nbv[s], V_wallv);
+            const bool own = q.own[s], in = q.inner[s];
+            const double vn_ = C(q.n[s], V_wallv);
+            const double wn_ = C(q.n[s], V_w);
+            x0n[s] = isfinite(vn_) ? vn_ : wn_;
             const double vo = own ? wvc : vn_, vn = own ? vn_ : wvc;
-            if (isfinite(vo)) {
-                if (!own) sr -= offv[s] * vo;
-                offv[s] = 0.0;
-            } else if (isfinite(vn)) {
-                if (own) sr -= offv[s] * vn;
-                offv[s] = 0.0;
-            }
+            const bool fo = isfinite(vo), fn = isfinite(vn);
+            // a fixed value on the owner side (else the neighbour side) moves
+            // the coefficient into the source of the other side's row
+            const bool sub = in && (fo ? !own : (fn && own));
+            const double v = fo ? vo : vn;
+            sr = sub ? sr - offv[s] * v : sr;
+            offv[s] = (in && (fo || fn)) ? 0.0 : offv[s];
         }
         if (isfinite(wvc)) sr = dg * wvc;
         C(ci, V_diag) = dg;
         C(ci, V_src) = sr;
         const double x0 = isfinite(wvc) ? wvc : wc;
         ok = isfinite(dg) && isfinite(sr) && isfinite(x0);
+        double y = dg * x0;
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) {
-                off(ci, s) = offv[s];
-                ok = ok && isfinite(offv[s]);
-            }
+        for (int s = 0; s < kSlots; ++s) {
+            if (q.inner[s]) off(ci, s) = offv[s];
+            ok = ok && (!q.inner[s] || isfinite(offv[s]));
+            y = q.inner[s] ? y + offv[s] * x0n[s] : y;
+        }
+        resid = sr - y;
         return x0;
     }
 
@@ -1120,14 +1140,20 @@ struct Draw {
         for (int j = 0; j < CPL; ++j) {
             const int ci = tid + 64 * j;
             R.act[j] = ci < nc;
-            const v4i nb4 = R.act[j] ? slot4[2 * ci + 1] : v4i{-1, -1, -1, -1};
+            const v4i raw = slot4[2 * (R.act[j] ? ci : 0) + 1];
+            const v4i nb4 = R.act[j] ? raw : v4i{-1, -1, -1, -1};
             R.nbr[j][0] = nb4.x;
             R.nbr[j][1] = nb4.y;
             R.nbr[j][2] = nb4.z;
             R.nbr[j][3] = nb4.w;
-            R.dg[j] = R.act[j] ? C(ci, V_diag) : 0.0;
+            const int cc = R.act[j] ? ci : 0;
+            const double dgv = C(cc, V_diag);
+            R.dg[j] = R.act[j] ? dgv : 0.0;
 #pragma unroll
-            for (int s = 0; s < kSlots; ++s) R.of[j][s] = R.nbr[j][s] >= 0 ? off(ci, s) : 0.0;
+            for (int s = 0; s < kSlots; ++s) {
+                const double ov = off(cc, s);
+                R.of[j][s] = R.nbr[j][s] >= 0 ? ov : 0.0;
+            }
         }
     }
 
@@ -1139,8 +1165,10 @@ struct Draw {
         for (int j = 0; j < CPL; ++j) {
             double a = R.dg[j] * xs[j];
 #pragma unroll
-            for (int s = 0; s < kSlots; ++s)
-                if (R.nbr[j][s] >= 0) a += R.of[j][s] * C(R.nbr[j][s], xv);
+            for (int s = 0; s < kSlots; ++s) {
+                const double xn = C(R.nbr[j][s] >= 0 ? R.nbr[j][s] : 0, xv);
+                a = R.nbr[j][s] >= 0 ? a + R.of[j][s] * xn : a;
+            }
             y[j] = a;
         }
     }
@@ -1157,7 +1185,7 @@ struct Draw {
     DEV void store_x(const Rows<CPL> &R, const double (&x)[CPL]) {
 #pragma unroll
         for (int j = 0; j < CPL; ++j)
-            if (R.act[j]) C(tid + 64 * j, V_kx) = x[j];
+            if (R.act[j]) C(tid + 64 * j, xdst) = x[j];
     }
 
     // BiCGSTABSolver::solve (LinearSolver.hpp:259-343), wave 0 only
@@ -1331,15 +1359,20 @@ struct Draw {
         for (int j = 0; j < CPL; ++j) {
             const int ci = tid + 64 * j;
             R.act[j] = ci < nc;
-            const v4i nb4 = R.act[j] ? slot4[2 * ci + 1] : v4i{-1, -1, -1, -1};
+            const v4i raw = slot4[2 * (R.act[j] ? ci : 0) + 1];
+            const v4i nb4 = R.act[j] ? raw : v4i{-1, -1, -1, -1};
             R.nbr[j][0] = nb4.x;
             R.nbr[j][1] = nb4.y;
             R.nbr[j][2] = nb4.z;
             R.nbr[j][3] = nb4.w;
-            R.dg[j] = R.act[j] ? (float)C(ci, V_diag) : 0.0f;
+            const int cc = R.act[j] ? ci : 0;
+            const float dgv = (float)C(cc, V_diag);
+            R.dg[j] = R.act[j] ? dgv : 0.0f;
 #pragma unroll
-            for (int s = 0; s < kSlots; ++s)
-                R.of[j][s] = R.nbr[j][s] >= 0 ? (float)off(ci, s) : 0.0f;
+            for (int s = 0; s < kSlots; ++s) {
+                const float ov = (float)off(cc, s);
+                R.of[j][s] = R.nbr[j][s] >= 0 ? ov : 0.0f;
+            }
         }
     }
 
@@ -1349,8 +1382,10 @@ struct Draw {
         for (int j = 0; j < CPL; ++j) {
             float a = R.dg[j] * xs[j];
 #pragma unroll
-            for (int s = 0; s < kSlots; ++s)
-                if (R.nbr[j][s] >= 0) a += R.of[j][s] * (float)C(R.nbr[j][s], xv);
+            for (int s = 0; s < kSlots; ++s) {
+                const float xn = (float)C(R.nbr[j][s] >= 0 ? R.nbr[j][s] : 0, xv);
+                a = R.nbr[j][s] >= 0 ? a + R.of[j][s] * xn : a;
+            }
             y[j] = a;
         }
     }
@@ -1371,7 +1406,7 @@ struct Draw {
         for (int j = 0; j < CPL; ++j)
             if (R.act[j]) {
                 const int ci = tid + 64 * j;
-                C(ci, V_kx) = C(ci, V_kx) + scale * (double)d[j];
+                C(ci, xdst) = C(ci, V_kx) + scale * (double)d[j];
             }
     }
 
@@ -1688,8 +1723,10 @@ struct Draw {
         const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
         double y = C(ci, V_diag) * x(ci);
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
-            if (nbv[s] >= 0) y += off(ci, s) * x(nbv[s]);
+        for (int s = 0; s < kSlots; ++s) {
+            const double a = off(ci, s), xn = x(nbv[s] >= 0 ? nbv[s] : ci);
+            y = nbv[s] >= 0 ? y + a * xn : y;
+        }
         return y;
     }
 
@@ -2275,23 +2312,39 @@ struct Draw {
 
     // The wave-resident path is compiled into the launch shapes that small
     // meshes use (up to 256 threads).
-    DEV Krylov solve(int kind, double rr, bool zeroStart) {
-        if (kind == SOLVER_AMG && zeroStart && M.amg.levels > 0) {
-            if constexpr (kF32) return pcg32<true>(rr, S.inner_iterations, S.inner_tolerance);
-            else return amg_pcg(rr, S.inner_iterations, S.inner_tolerance);
-        }
+    // The solution is left in kx and, when dst names another cell variable,
+    // also in dst (the wave-resident solvers store it there directly).
+    DEV Krylov solve(int kind, double rr, bool zeroStart, int dst = V_kx) {
         if constexpr (BS <= 256 && !kGrid) {
-            if (nc <= 256) {
+            if (nc <= 256 && !(kind == SOLVER_AMG && zeroStart && M.amg.levels > 0)) {
+                xdst = dst;
                 if (tid < 64) {
                     Krylov r;
                     if (nc <= 64) r = solve_wave<1>(kind, rr, zeroStart);
                     else if (nc <= 128) r = solve_wave<2>(kind, rr, zeroStart);
                     else r = solve_wave<4>(kind, rr, zeroStart);
+                    // a solve that stops at a zero residual stores nothing: x
+                    // is its start value
+                    if (dst != V_kx && r.initial < 1e-30)
+                        for (int ci = tid; ci < nc; ci += 64) C(ci, dst) = C(ci, V_kx);
                     if (tid == 0) *kres = r;
                 }
+                xdst = V_kx;
                 __syncthreads();
                 return *kres;
             }
+        }
+        const Krylov r = solve_block(kind, rr, zeroStart);
+        if (dst != V_kx) {
+            for (int ci = gt; ci < nc; ci += gs) C(ci, dst) = C(ci, V_kx);
+            sync();
+        }
+        return r;
+    }
+    DEV Krylov solve_block(int kind, double rr, bool zeroStart) {
+        if (kind == SOLVER_AMG && zeroStart && M.amg.levels > 0) {
+            if constexpr (kF32) return pcg32<true>(rr, S.inner_iterations, S.inner_tolerance);
+            else return amg_pcg(rr, S.inner_iterations, S.inner_tolerance);
         }
         if constexpr (kF32) {
             if (kind == SOLVER_PCG && zeroStart)
@@ -2635,9 +2688,7 @@ struct Draw {
                     fail(ERR_NONFINITE_SYSTEM);
                     break;
                 }
-                rUx = solve(S.momentum_solver, a[3], false);
-                for (int ci = gt; ci < nc; ci += gs) C(ci, V_Ux) = C(ci, V_kx);
-                sync();
+                rUx = solve(S.momentum_solver, a[3], false, V_Ux);
                 mark(2);
             }
             // ---- y momentum: its transpose stress sees the new Ux
@@ -2669,9 +2720,7 @@ struct Draw {
                     fail(ERR_NONFINITE_SYSTEM);
                     break;
                 }
-                rUy = solve(S.momentum_solver, a[2], false);
-                for (int ci = gt; ci < nc; ci += gs) C(ci, V_Uy) = C(ci, V_kx);
-                sync();
+                rUy = solve(S.momentum_solver, a[2], false, V_Uy);
                 mark(4);
             }
             bc_velocity();
@@ -2712,23 +2761,23 @@ struct Draw {
             sync();
             double a8[3] = {0.0, 0.0, 0.0};
             for (int ci = gt; ci < nc; ci += gs) {
-                int code[kSlots], nbv[kSlots];
-                slots(ci, code, nbv);
-                double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
+                SlotSet q;
+                slot_set(ci, q);
+                double dg = 0.0, sr = 0.0, offv[kSlots];
 #pragma unroll
                 for (int s = 0; s < kSlots; ++s) {
-                    if (code[s] < 0) continue;
-                    const int f = slot_face(code[s]);
-                    if (f < nif) {
-                        const double coeff = Fc(f, W_pCoeff);
-                        dg += coeff;
-                        offv[s] = -coeff;
-                        if (slot_own(code[s])) sr -= Fc(f, W_phiPred);
-                        else sr += Fc(f, W_phiPred);
-                    } else {
-                        sr -= Fc(f, W_phiPred);
-                        if (bflag[f - nif] & 2) dg += Fc(f, W_pCoeff);
-                    }
+                    const int f = q.f[s];
+                    const double coeff = Fc(f, W_pCoeff), pred = Fc(f, W_phiPred);
+                    const bool outlet = bflag[q.b[s]] & kFlagOutlet;
+                    const bool in = q.inner[s];
+                    // internal faces and outlet faces add their coefficient;
+                    // the predicted flux leaves the owner and enters the
+                    // neighbour, and leaves through every boundary face
+                    dg = (q.valid[s] && (in || outlet)) ? dg + coeff : dg;
+                    const double t = (!in || q.own[s]) ? sr - pred : sr + pred;
+                    sr = q.valid[s] ? t : sr;
+                    offv[s] = -coeff;
+                    if (in) off(ci, s) = offv[s];
                 }
                 if (!has_outlet && ci == 0) dg += fabs(dg) > 1e-30 ? dg : 1.0;
                 C(ci, V_diag) = dg;
@@ -2738,16 +2787,18 @@ struct Draw {
                 bool okp = isfinite(dg) && isfinite(sr);
 #pragma unroll
                 for (int s = 0; s < kSlots; ++s)
-                    if (nbv[s] >= 0) {
-                        off(ci, s) = offv[s];
-                        okp = okp && isfinite(offv[s]);
-                    }
+                    okp = okp && (!q.inner[s] || isfinite(offv[s]));
                 a8[0] += fabs(sr);
                 a8[1] += sr * sr;
                 a8[2] += okp ? 0.0 : 1.0;
             }
-            red.sum(ms);
-            red.sum(a8);
+            // one reduction for the flux scale and the p' row sums
+            double a4[4] = {ms[0], a8[0], a8[1], a8[2]};
+            red.sum(a4);
+            ms[0] = a4[0];
+            a8[0] = a4[1];
+            a8[1] = a4[2];
+            a8[2] = a4[3];
             const double massFluxScale = ms[0];
             mark(5);
             const double contScaled = a8[0] / smax(massFluxScale, 1e-300);
@@ -2755,10 +2806,8 @@ struct Draw {
                 fail(ERR_NONFINITE_SYSTEM);
                 break;
             }
-            const Krylov rP = solve(S.pressure_solver, a8[1], true);
+            const Krylov rP = solve(S.pressure_solver, a8[1], true, V_pP);
             mark(6);
-            for (int ci = gt; ci < nc; ci += gs) C(ci, V_pP) = C(ci, V_kx);
-            sync();
             for (int b = gt; b < nb; b += gs) Bd(b, B_pP) = (bflag[b] & 2) ? 0.0 : C(fon[nif + b].x, V_pP);
             sync();
             // ---- correctVelocity / correctPressure (SIMPLESolver.cpp:916-940)
@@ -2785,19 +2834,26 @@ struct Draw {
             sync();
             mark(7);
 
+            // velocity maxima of the convergence norms (U is final for this
+            // iteration: the turbulence transport below does not change it)
+            double mu[2] = {1e-300, 0.0};
+            for (int ci = gt; ci < nc; ci += gs) {
+                const double ux = C(ci, V_Ux), uy = C(ci, V_Uy);
+                mu[0] = smax(mu[0], sqrt(ux * ux + uy * uy));
+                const double dx = ux - C(ci, V_UxP), dy = uy - C(ci, V_UyP);
+                mu[1] = smax(mu[1], sqrt(dx * dx + dy * dy));
+            }
+
             // ---- turbulence transport (SIMPLESolver.cpp:1476-1538)
             if (turbUpdate) {
                 double a9[2] = {0.0, 0.0};
                 for (int ci = gt; ci < nc; ci += gs) {
                     bool ok;
-                    k_row(ci, ok);
+                    double r;
+                    k_row(ci, ok, r);
                     C(ci, V_kx) = C(ci, V_k);
-                    a9[1] += ok ? 0.0 : 1.0;
-                }
-                sync();
-                for (int ci = gt; ci < nc; ci += gs) {
-                    const double r = C(ci, V_src) - spmv(ci, V_kx);
                     C(ci, V_kr) = r;
+                    a9[1] += ok ? 0.0 : 1.0;
                     a9[0] += r * r;
                 }
                 red.sum(a9);
@@ -2821,14 +2877,11 @@ struct Draw {
                 double a10[2] = {0.0, 0.0};
                 for (int ci = gt; ci < nc; ci += gs) {
                     bool ok;
-                    const double x0 = omega_row(ci, ok);
+                    double r;
+                    const double x0 = omega_row(ci, ok, r);
                     C(ci, V_kx) = x0;
-                    a10[1] += ok ? 0.0 : 1.0;
-                }
-                sync();
-                for (int ci = gt; ci < nc; ci += gs) {
-                    const double r = C(ci, V_src) - spmv(ci, V_kx);
                     C(ci, V_kr) = r;
+                    a10[1] += ok ? 0.0 : 1.0;
                     a10[0] += r * r;
                 }
                 red.sum(a10);
@@ -2850,8 +2903,11 @@ struct Draw {
                 }
                 sync();
                 bc_omega(c[2]);
-                double m4[4] = {mk[0], mk[1], mw[0], mw[1]};
+                // one reduction for the k/omega change maxima and the velocity norms
+                double m4[6] = {mk[0], mk[1], mw[0], mw[1], mu[0], mu[1]};
                 red.max(m4);
+                mu[0] = m4[4];
+                mu[1] = m4[5];
                 if (tid == 0) {
                     const double kChange = m4[1] / m4[0], omChange = m4[3] / m4[2];
                     bk->lastK = kChange;
@@ -2864,14 +2920,7 @@ struct Draw {
             }
 
             // ---- convergence bookkeeping (SIMPLESolver.cpp:1545-1705)
-            double mu[2] = {1e-300, 0.0};
-            for (int ci = gt; ci < nc; ci += gs) {
-                const double ux = C(ci, V_Ux), uy = C(ci, V_Uy);
-                mu[0] = smax(mu[0], sqrt(ux * ux + uy * uy));
-                const double dx = ux - C(ci, V_UxP), dy = uy - C(ci, V_UyP);
-                mu[1] = smax(mu[1], sqrt(dx * dx + dy * dy));
-            }
-            red.max(mu);
+            if (!turbUpdate) red.max(mu);
             mark(12);
             const bool monitor =
                 nobs > 0 && iter % (S.monitor_interval > 1 ? S.monitor_interval : 1) == 0;
@@ -3039,7 +3088,7 @@ struct Draw {
 template <int BS>
 struct BlockShared {
     Book book;
-    double redbuf[2 * (BS / 64) * 4];
+    double redbuf[2 * (BS / 64) * Reducer<BS>::kStride];
     double coeffs[11];
     DevSettings sset;
     Krylov kslot;
