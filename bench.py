@@ -14,7 +14,10 @@ run in one persistent-kernel launch. The line printed is
   cpu_baseline the reference's own FP64 solver (compiled from its sources into
                oracle/_ref/ref_harness where it was built; otherwise the CPU
                FP64 oracle) on the host cores, a bounded sample of the same
-               draws, one draw per process (thread)
+               draws, one draw per process (thread); workers = the CPUs of the
+               process's affinity mask within the job's CPU share
+               (OMP_NUM_THREADS), with the host's counts in cpu_baseline.host;
+               cpu_baseline_1worker times draw 0 alone on one process
   max_rel_err_vs_cpu_baseline  probe error of the GPU rows against the CPU rows
                of that sample (statuses must match)
 
@@ -27,8 +30,11 @@ off): single_solve (one FP64 solve at the default coefficients, one launch,
 with the oracle on one thread), mixed (the sweep with FP32 Krylov
 and FP64 defect correction), served (Poisson arrivals of single requests on
 64-512 cell channels against the plain one-launch-per-request path), hills
-(the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration;
---hills-full adds the 120x64 batch to convergence), warm_start (a
+(the 16-draw 120x64 and 8-draw 480x256 hill batches per outer iteration,
+and the 120x64 batch solved to convergence in FP64 and mixed precision,
+whose converged solves/s the line repeats as hill_120x64_converged_solves_per_s;
+--no-hills-full skips the latter), calibration (the staged calibration
+driver on the 1x128 channel: solves, launches, seconds), warm_start (a
 pattern-search poll of 4 neighbours of a converged point, cold in one batch
 launch against warm-started from the device-resident state in one launch),
 warm_fd_jacobian (the 22 perturbed solves of the eleven-coefficient warm-FD
@@ -183,6 +189,13 @@ def _cores():
     if env and env.isdigit() and int(env) > 0:
         n = min(n, int(env))
     return max(1, n)
+
+
+def _host():
+    """What the host offers the CPU baselines: its CPU count, this process's
+    affinity mask and the job's CPU share (OMP_NUM_THREADS)."""
+    return {"nproc": os.cpu_count(), "affinity_cpus": _host_cpus(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _traffic_from_profile():
@@ -436,26 +449,35 @@ def _mixed_vs_fp64(rm, r):
             "max_rel_probe_diff_vs_fp64": err}
 
 
-def hills_leg(full, cores, cpu):
+def hills_leg(full, cores, cpu, meshes=("hill_120x64", "hill_480x256"), block_size=0,
+              grid_blocks=0):
     """Hill meshes: the 16-draw 120x64 batch (AMG pressure, tol 1e-4) and the
     8-draw 480x256 batch, each timed per outer iteration on solves capped at
     a few hundred (resp. ten) iterations, on the launch shape the autotuner
     picks (timed on a shorter capped run first), in FP64 and with FP32 Krylov
     and FP64 defect correction. The CPU oracle runs the 120x64 draws one per
     thread with the same cap. With full=True the 16-draw 120x64 batch also
-    runs to convergence in both precisions (cap 25000; minutes)."""
+    runs to convergence in both precisions (cap 25000; minutes). meshes picks
+    the capped batches; block_size and grid_blocks (both nonzero) pin the
+    launch shape instead of tuning it, for profiler runs."""
     from cfdr_amd import cases, engine
     out = {}
     for key, nx, ny, nd, cap, tune_its in (("hill_120x64", 120, 64, 16, 200, 20),
                                            ("hill_480x256", 480, 256, 8, 10, 2)):
+        if key not in meshes:
+            continue
         prob = cases.hill_problem(nx=nx, ny=ny, max_iterations=cap)
         th = cases.hill_draws(nd)
         legs = {}
         for prec in ("fp64", "mixed"):
+            pinned = block_size > 0 and grid_blocks > 0
             b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu,
                                     prob.settings, prob.u_init, prob.p_init, prob.k_init,
-                                    prob.omega_init, mixed_precision=prec == "mixed")
-            tune = b.autotune(th, iterations=tune_its)
+                                    prob.omega_init, mixed_precision=prec == "mixed",
+                                    block_size=block_size if pinned else 0,
+                                    grid_blocks=grid_blocks if pinned else 0)
+            tune = ({"block_size": block_size, "grid_blocks": grid_blocks, "candidates": []}
+                    if pinned else b.autotune(th, iterations=tune_its))
             legs[prec] = (b.evaluate(th), b.last_timing(), tune)
             legs["levels"] = b.amg_levels()
             b.close()
@@ -501,9 +523,17 @@ def hills_leg(full, cores, cpu):
                 dt = max(per)
                 out[key]["cpu_baseline"] = {
                     "value": round(cores * ccap / dt, 1), "unit": "draw-iterations/s",
-                    "cores": cores, "kind": "reference",
+                    "cores": cores, "kind": "reference", "host": _host(),
                     "sample": f"{cores} draws capped at {ccap} outer iterations, one per "
                               f"process of the reference's solver, slowest {dt:.1f} s"}
+                one = reference_cpu_baseline(prob, rows[:1], 1)
+                if one is not None:
+                    t1 = one[0][0]["wall_seconds"]
+                    out[key]["cpu_baseline_1worker"] = {
+                        "value": round(ccap / t1, 1), "unit": "draw-iterations/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": f"1 draw capped at {ccap} outer iterations, one process of "
+                                  f"the reference's solver, {t1:.1f} s"}
             else:
                 sys.path.insert(0, os.path.join(ROOT, "oracle"))
                 import oracle_py
@@ -515,7 +545,7 @@ def hills_leg(full, cores, cpu):
                 dt = time.perf_counter() - t0
                 out[key]["cpu_baseline"] = {
                     "value": round(cores * ccap / dt, 1), "unit": "draw-iterations/s",
-                    "cores": cores, "kind": "port",
+                    "cores": cores, "kind": "port", "host": _host(),
                     "sample": f"{cores} draws capped at {ccap} outer iterations, one per "
                               f"thread, {dt:.1f} s wall"}
     if full:
@@ -640,6 +670,48 @@ def warm_fd_leg(cpu):
     return out
 
 
+def calibration_leg():
+    """The staged calibration (uniform design, pattern search, 5x5 Laplace
+    grid, two 64-draw importance posteriors, split conformal) on the 1x128
+    channel, each stage one batched launch. The observed profile is the
+    engine's own prediction at theta* = (0.322, 0.0952) at 30 wall distances
+    (y+ 1 to 182), noise 3e-4 of each value, every third point in the
+    likelihood; the stage-1 box is centred on theta* (a1 +/- 0.004, beta* +/-
+    0.0012). The reference's study runs 241 such solves one per thread."""
+    from cfdr_amd import cases, engine
+    from cfdr_amd.calibration import Calibration, CalibrationSettings, channel_batch
+    prob = cases.channel_problem()
+    y = np.array([ob.location[1] for ob in prob.obs.observables])
+    gen = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
+                              prob.u_init, prob.p_init, prob.k_init, prob.omega_init)
+    theta_star = np.array([0.322, 0.0952])
+    truth = gen.evaluate(theta_star[None, :]).predictions[0]
+    gen.close()
+    group = np.arange(truth.size) % 3
+    sigma = 3e-4 * truth
+    batch = channel_batch(prob, y, truth, np.where(group == 0, sigma, np.inf))
+    settings = CalibrationSettings(stage1_box=((0.318, 0.094), (0.326, 0.0964)))
+    t0 = time.perf_counter()
+    res = Calibration.for_batch(batch, settings).run(truth, sigma, group)
+    dt = time.perf_counter() - t0
+    batch.close()
+    print(f"# calibration: {res.solves} solves in {res.launches} launches, {dt:.1f} s",
+          file=sys.stderr, flush=True)
+    return {"solves": res.solves, "launches": res.launches, "seconds": round(dt, 2),
+            "solves_per_s": round(res.solves / dt, 3),
+            "optimum": [float(v) for v in res.optimum],
+            "laplace_center": [float(v) for v in res.center],
+            "posterior_mean": [float(v) for v in res.main.theta_mean],
+            "effective_sample_size": round(float(res.main.effective_sample_size), 2),
+            "held_out_covered": {k: res.coverage[k] for k in
+                                 ("held_out", "standard_covered", "recalibrated_covered",
+                                  "replicate_covered")},
+            "workload": "1x128 channel (tol 1e-8), synthetic profile at theta* (0.322, 0.0952), "
+                        "30 points, sigma 3e-4 relative, stage-1 box theta* +/- (0.004, 0.0012)",
+            "reference": "docs/figures/calibration_coverage.py runs its 241 solves in about 50 "
+                         "minutes on four threads (its 3x128 periodic channel)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -668,8 +740,16 @@ def main():
                     help="skip the single default-coefficient solve (GPU and one CPU thread)")
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-warm-fd", action="store_true")
-    ap.add_argument("--hills-full", action="store_true",
-                    help="also run the 16-draw 120x64 hill batch to convergence (minutes)")
+    ap.add_argument("--no-hills-full", action="store_true",
+                    help="skip the 16-draw 120x64 hill batch solved to convergence (FP64 and "
+                         "mixed, about 3.5 minutes together)")
+    ap.add_argument("--hill-meshes", default="hill_120x64,hill_480x256",
+                    help="capped hill batches to time (comma list)")
+    ap.add_argument("--hills-block-size", type=int, default=0,
+                    help="with --hills-grid-blocks: pin the hill launch shape (profiler runs)")
+    ap.add_argument("--hills-grid-blocks", type=int, default=0)
+    ap.add_argument("--no-calibration", action="store_true",
+                    help="skip the staged calibration on the 1x128 channel")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
@@ -783,6 +863,7 @@ def main():
                  "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 1)}
 
     cpu = None
+    cpu1 = None
     err = None
     if not args.no_cpu_baseline:
         # the reference's own solver where its harness was built, else the port
@@ -797,6 +878,17 @@ def main():
                 p, q = last.predictions[i], o["predictions"]
                 errs.append(float(np.max(np.abs(p - q) / np.abs(q))))
         err = max(errs) if errs else None
+        cpu["host"] = _host()
+        # one worker: the first draw alone
+        one = reference_cpu_baseline(prob, theta, 1)
+        if one is not None:
+            t1 = one[0][0]["wall_seconds"]
+            cpu1 = {"value": round(int(one[0][0]["status"] == 0) / t1, 4),
+                    "unit": "converged solves/s", "cores": 1, "kind": "reference",
+                    "sample": f"draw 0 of the sweep, one process of the reference's solver, "
+                              f"{t1:.2f} s"}
+        else:
+            rows1, cpu1 = cpu_baseline(prob, theta, 1)
 
     single = None
     if not args.no_single:
@@ -816,7 +908,13 @@ def main():
 
     hills = None
     if not args.no_hills:
-        hills = hills_leg(args.hills_full, _cores(), not args.no_cpu_baseline)
+        hills = hills_leg(not args.no_hills_full, _cores(), not args.no_cpu_baseline,
+                          tuple(args.hill_meshes.split(",")), args.hills_block_size,
+                          args.hills_grid_blocks)
+
+    calibration = None
+    if not args.no_calibration:
+        calibration = calibration_leg()
 
     timing = batch.last_timing()
     batch.close()  # every engine is released before the interpreter exits
@@ -848,6 +946,7 @@ def main():
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_1worker": cpu1,
         "max_rel_err_vs_cpu_baseline": err,
         "mixed": mixed,
         "single_solve": single,
@@ -855,6 +954,9 @@ def main():
         "warm_start": warm,
         "warm_fd_jacobian": warm_fd,
         "hills": hills,
+        "hill_120x64_converged_solves_per_s": (hills or {}).get("hill_120x64_full", {}).get(
+            "converged_per_s"),
+        "calibration": calibration,
         "block_size": timing["block_size"],
         "autotune": tune,
         "graph_replay": {"enabled": not args.no_graph, "graph_launches": graph_launches},

@@ -1,5 +1,6 @@
 """The launch-shape autotuner (rq_engine_autotune, no reference counterpart):
-every candidate is timed, the fastest shape is kept, and evaluations after
+every candidate is timed, the first shape within 3 % of the fastest is kept
+(so shapes within timing noise resolve the same way everywhere), and evaluations after
 tuning agree with an untuned engine to the parity bar (the shape only changes
 the order of parallel reductions)."""
 from __future__ import annotations
@@ -29,21 +30,22 @@ def _check_report(rep, blocks, grids, amg_rows=()):
     assert all(t > 0 for t in times if not math.isnan(t))
     assert any(not math.isnan(t) for t in times[:n]), "no candidate ran"
 
-    def scan(cands, best=math.inf, choice=None):
-        # the tuner's rule: in order, a candidate more than 1 % faster wins
-        for t, c in cands:
-            if not math.isnan(t) and t < 0.99 * best:
-                best, choice = t, c
-        return best, choice
+    def scan(cands):
+        # the tuner's rule: the first candidate, in the order tried, whose
+        # time is within 3 % of the fastest
+        cands = [(t, c) for t, c in cands if not math.isnan(t)]
+        fastest = min(t for t, _ in cands)
+        return next((t, c) for t, c in cands if t <= 1.03 * fastest)
 
     best, shape = scan(zip(times[:n], shapes[:n]))
     assert (rep["block_size"], rep["grid_blocks"]) == shape
-    # the multigrid threshold is timed at the chosen multi-workgroup shape
+    # the multigrid threshold is timed at the chosen multi-workgroup shape,
+    # the configured threshold (1024, at the shape's own time) first
     extra = rep["candidates"][n:]
     if shape[1] > 1 and amg_rows:
         assert [c["amg_block_rows"] for c in extra] == list(amg_rows)
         assert all((c["block_size"], c["grid_blocks"]) == shape for c in extra)
-        _, rows = scan([(c["kernel_ms"], c["amg_block_rows"]) for c in extra], best, 1024)
+        _, rows = scan([(best, 1024)] + [(c["kernel_ms"], c["amg_block_rows"]) for c in extra])
         assert rep["amg_block_rows"] == rows
     else:
         assert extra == [] and rep["amg_block_rows"] == 1024
