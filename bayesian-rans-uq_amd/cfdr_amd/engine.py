@@ -61,7 +61,7 @@ class RqSettings(C.Structure):
                 ("momentum_solver", C.c_int32), ("turbulence_solver", C.c_int32),
                 ("sst_variant", C.c_int32), ("mixed_precision", C.c_int32),
                 ("block_size", C.c_int32), ("profile", C.c_int32), ("grid_blocks", C.c_int32),
-                ("amg_block_rows", C.c_int32)]
+                ("amg_block_rows", C.c_int32), ("grid_plain_launch", C.c_int32)]
 
 
 class RqParams(C.Structure):
@@ -170,7 +170,8 @@ class _Keep:
 
 
 def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size: int = 0,
-                    profile: bool = False, grid_blocks: int = 0, amg_block_rows: int = 0):
+                    profile: bool = False, grid_blocks: int = 0, amg_block_rows: int = 0,
+                    grid_plain_launch: bool = False):
     r = RqSettings()
     r.max_iterations, r.convergence_tol, r.divergence_limit = s.max_iterations, \
         s.convergence_tol, s.divergence_limit
@@ -192,6 +193,7 @@ def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size
     r.profile = 1 if profile else 0
     r.grid_blocks = grid_blocks
     r.amg_block_rows = amg_block_rows
+    r.grid_plain_launch = 1 if grid_plain_launch else 0
     return r
 
 
@@ -311,7 +313,8 @@ class Problem:
                  settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
                  mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
-                 grid_blocks: int = 0, owner: str = "ForwardBatch", amg_block_rows: int = 0):
+                 grid_blocks: int = 0, owner: str = "ForwardBatch", amg_block_rows: int = 0,
+                 grid_plain_launch: bool = False):
         settings = settings if settings is not None else SolverSettings()
         settings.validate()
         param_set.validate()
@@ -339,7 +342,7 @@ class Problem:
         p.lower, p.upper = keep.d(param_set.lower_bounds()), keep.d(param_set.upper_bounds())
         self.o = obs_struct(prepared, keep)
         self.s = settings_struct(settings, mixed_precision, block_size, profile, grid_blocks,
-                                 amg_block_rows)
+                                 amg_block_rows, grid_plain_launch)
         self.init = np.asarray([u_init.x, u_init.y, p_init, k_init, omega_init], float)
         self.nu = float(nu)
 
@@ -417,11 +420,11 @@ class ForwardBatch:
                  settings: SolverSettings = None, u_init: Vec3 = Vec3(1.0, 0.0, 0.0),
                  p_init: float = 0.0, k_init: float = 1e-4, omega_init: float = 1.0,
                  mixed_precision: bool = False, block_size: int = 0, profile: bool = False,
-                 grid_blocks: int = 0, amg_block_rows: int = 0):
+                 grid_blocks: int = 0, amg_block_rows: int = 0, grid_plain_launch: bool = False):
         self.lib = load_library()
         pr = Problem(mesh, param_set, obs_op, bcs, nu, settings, u_init, p_init, k_init,
                      omega_init, mixed_precision, block_size, profile, grid_blocks, "ForwardBatch",
-                     amg_block_rows)
+                     amg_block_rows, grid_plain_launch)
         self.param_set = param_set
         self.n_obs = obs_op.n_obs()
         self.n_cells = mesh.n_cells

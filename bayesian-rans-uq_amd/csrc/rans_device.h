@@ -151,6 +151,7 @@ struct DevSettings {
     int profile;           // 1: accumulate per-phase clock counts in DevResult::cycles
     int grid_blocks;       // workgroups per draw (1: one-workgroup draws)
     int amg_block_rows;    // multigrid levels of at most this many rows: one workgroup
+    int grid_plain_launch; // 1: multi-workgroup grids by an ordinary dispatch (host side only)
 };
 
 // One draw to solve: its mesh, its arena and its coefficients.

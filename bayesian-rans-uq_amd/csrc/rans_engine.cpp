@@ -947,6 +947,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         S.profile = s.profile;
         S.grid_blocks = s.grid_blocks;
         S.amg_block_rows = s.amg_block_rows > 0 ? s.amg_block_rows : 1024;
+        S.grid_plain_launch = s.grid_plain_launch != 0;
         HIPCHECK(hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device));
         E->block_user = s.block_size != 0;
         if (s.block_size) E->block = s.block_size;

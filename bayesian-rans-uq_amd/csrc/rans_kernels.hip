@@ -3431,6 +3431,14 @@ static hipError_t launch_grid_t(const DevSlot *slots, int nslots, int nblk, cons
                     (void *)&pred_stride, (void *)&fields, (void *)&field_stride,
                     (void *)&states, (void *)&nblk, (void *)&nslots};
     const int per_xcd = (nslots + kXcds - 1) / kXcds;  // draws on the fullest XCD
+    if (s.grid_plain_launch) {
+        // the same grid as an ordinary dispatch (the caller has checked that
+        // it fits the device at once)
+        hipLaunchKernelGGL(kern, dim3(kXcds * per_xcd * nblk), dim3(kGridBlock), lds_bytes,
+                           stream, slots, sc, results, pred, pred_stride, fields, field_stride,
+                           states, nblk, nslots);
+        return hipGetLastError();
+    }
     return hipLaunchCooperativeKernel((const void *)kern, dim3(kXcds * per_xcd * nblk),
                                       dim3(kGridBlock), args, (unsigned)lds_bytes, stream);
 }

@@ -34,7 +34,7 @@ and FP64 defect correction), served (Poisson arrivals of single requests on
 and the 120x64 batch solved to convergence in FP64 and mixed precision,
 whose converged solves/s the line repeats as hill_120x64_converged_solves_per_s;
 --no-hills-full skips the latter), calibration (the staged calibration
-driver on the 1x128 channel: solves, launches, seconds), warm_start (a
+driver on the 1x32 channel: solves, launches, seconds), warm_start (a
 pattern-search poll of 4 neighbours of a converged point, cold in one batch
 launch against warm-started from the device-resident state in one launch),
 warm_fd_jacobian (the 22 perturbed solves of the eleven-coefficient warm-FD
@@ -198,8 +198,10 @@ def _host():
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
-def _traffic_from_profile():
-    path = os.path.join(ROOT, "profiles", "simple_solve_kernel_channel_sweep64_traffic.json")
+def _traffic_from_profile(name="simple_solve_kernel_channel_sweep64_traffic.json"):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -450,7 +452,7 @@ def _mixed_vs_fp64(rm, r):
 
 
 def hills_leg(full, cores, cpu, meshes=("hill_120x64", "hill_480x256"), block_size=0,
-              grid_blocks=0):
+              grid_blocks=0, plain_launch=False):
     """Hill meshes: the 16-draw 120x64 batch (AMG pressure, tol 1e-4) and the
     8-draw 480x256 batch, each timed per outer iteration on solves capped at
     a few hundred (resp. ten) iterations, on the launch shape the autotuner
@@ -459,7 +461,8 @@ def hills_leg(full, cores, cpu, meshes=("hill_120x64", "hill_480x256"), block_si
     thread with the same cap. With full=True the 16-draw 120x64 batch also
     runs to convergence in both precisions (cap 25000; minutes). meshes picks
     the capped batches; block_size and grid_blocks (both nonzero) pin the
-    launch shape instead of tuning it, for profiler runs."""
+    launch shape instead of tuning it, and plain_launch dispatches the
+    multi-workgroup grid as an ordinary launch, for profiler runs."""
     from cfdr_amd import cases, engine
     out = {}
     for key, nx, ny, nd, cap, tune_its in (("hill_120x64", 120, 64, 16, 200, 20),
@@ -475,7 +478,8 @@ def hills_leg(full, cores, cpu, meshes=("hill_120x64", "hill_480x256"), block_si
                                     prob.settings, prob.u_init, prob.p_init, prob.k_init,
                                     prob.omega_init, mixed_precision=prec == "mixed",
                                     block_size=block_size if pinned else 0,
-                                    grid_blocks=grid_blocks if pinned else 0)
+                                    grid_blocks=grid_blocks if pinned else 0,
+                                    grid_plain_launch=plain_launch)
             tune = ({"block_size": block_size, "grid_blocks": grid_blocks, "candidates": []}
                     if pinned else b.autotune(th, iterations=tune_its))
             legs[prec] = (b.evaluate(th), b.last_timing(), tune)
@@ -506,6 +510,12 @@ def hills_leg(full, cores, cpu, meshes=("hill_120x64", "hill_480x256"), block_si
                                  "kernel": "grid_solve_kernel" if t["grid_blocks"] > 1
                                  else "simple_solve_kernel",
                                  "algorithmic_bytes_per_launch": int(launch_bytes),
+                                 "traffic": (_traffic_from_profile(
+                                     "grid_solve_kernel_hill120x64_traffic.json")
+                                     if key == "hill_120x64" else None),
+                                 "traffic_note": ("HBM bytes of one such launch measured at 16 "
+                                                  "workgroups per draw (profiles/)"
+                                                  if key == "hill_120x64" else None),
                                  "amg_levels": legs["levels"]},
                     "mixed": dict({"ms_per_outer_iteration": round(rm.kernel_ms / cap, 3),
                                    "grid_blocks": tm["grid_blocks"],
@@ -672,15 +682,20 @@ def warm_fd_leg(cpu):
 
 def calibration_leg():
     """The staged calibration (uniform design, pattern search, 5x5 Laplace
-    grid, two 64-draw importance posteriors, split conformal) on the 1x128
-    channel, each stage one batched launch. The observed profile is the
-    engine's own prediction at theta* = (0.322, 0.0952) at 30 wall distances
-    (y+ 1 to 182), noise 3e-4 of each value, every third point in the
-    likelihood; the stage-1 box is centred on theta* (a1 +/- 0.004, beta* +/-
-    0.0012). The reference's study runs 241 such solves one per thread."""
+    grid, two 64-draw importance posteriors, split conformal) on the 1x32
+    geometric channel (ratio 1.15, tol 1e-6), each stage one batched launch.
+    The observed profile is the engine's own prediction at theta* = (0.322,
+    0.0952) at 30 wall distances (y+ 1 to 182), noise 3e-4 of each value,
+    every third point in the likelihood; the stage-1 box is centred on
+    theta* (a1 +/- 0.004, beta* +/- 0.0012). On the 1x128 channel the same
+    synthetic likelihood is a ridge narrower than the final pattern-search
+    step, so its 5x5 grid is not quadratic and the Laplace stage refuses it
+    (the peak it fits lies outside the grid), as the reference's driver does
+    (calibration_coverage.py:253-254). The reference's study runs 241 solves
+    one per thread."""
     from cfdr_amd import cases, engine
     from cfdr_amd.calibration import Calibration, CalibrationSettings, channel_batch
-    prob = cases.channel_problem()
+    prob = cases.channel_problem(ny=32, ratio=1.15, n_probes=30, tol=1e-6, max_iterations=40000)
     y = np.array([ob.location[1] for ob in prob.obs.observables])
     gen = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
                               prob.u_init, prob.p_init, prob.k_init, prob.omega_init)
@@ -692,9 +707,11 @@ def calibration_leg():
     batch = channel_batch(prob, y, truth, np.where(group == 0, sigma, np.inf))
     settings = CalibrationSettings(stage1_box=((0.318, 0.094), (0.326, 0.0964)))
     t0 = time.perf_counter()
-    res = Calibration.for_batch(batch, settings).run(truth, sigma, group)
+    try:
+        res = Calibration.for_batch(batch, settings).run(truth, sigma, group)
+    finally:
+        batch.close()
     dt = time.perf_counter() - t0
-    batch.close()
     print(f"# calibration: {res.solves} solves in {res.launches} launches, {dt:.1f} s",
           file=sys.stderr, flush=True)
     return {"solves": res.solves, "launches": res.launches, "seconds": round(dt, 2),
@@ -706,10 +723,23 @@ def calibration_leg():
             "held_out_covered": {k: res.coverage[k] for k in
                                  ("held_out", "standard_covered", "recalibrated_covered",
                                   "replicate_covered")},
-            "workload": "1x128 channel (tol 1e-8), synthetic profile at theta* (0.322, 0.0952), "
-                        "30 points, sigma 3e-4 relative, stage-1 box theta* +/- (0.004, 0.0012)",
+            "workload": "1x32 channel (ratio 1.15, tol 1e-6), synthetic profile at theta* "
+                        "(0.322, 0.0952), 30 points, sigma 3e-4 relative, stage-1 box theta* "
+                        "+/- (0.004, 0.0012)",
             "reference": "docs/figures/calibration_coverage.py runs its 241 solves in about 50 "
                          "minutes on four threads (its 3x128 periodic channel)"}
+
+
+def _leg(name, fn, *args):
+    """Run one auxiliary leg. An exception raised by the leg's own logic (a
+    calibration stage that refuses its grid, a CPU baseline that cannot
+    start) is reported in that leg's entry instead of ending the run, so the
+    headline line is always printed."""
+    try:
+        return fn(*args)
+    except Exception as exc:  # noqa: BLE001 - reported in the line
+        print(f"# {name} leg failed: {type(exc).__name__}: {exc}", file=sys.stderr, flush=True)
+        return {"error": f"{type(exc).__name__}: {exc}"}
 
 
 def main():
@@ -748,6 +778,9 @@ def main():
     ap.add_argument("--hills-block-size", type=int, default=0,
                     help="with --hills-grid-blocks: pin the hill launch shape (profiler runs)")
     ap.add_argument("--hills-grid-blocks", type=int, default=0)
+    ap.add_argument("--hills-plain-launch", action="store_true",
+                    help="launch the multi-workgroup hill grids as ordinary dispatches (the "
+                         "profiler writes no trace of a process that made cooperative launches)")
     ap.add_argument("--no-calibration", action="store_true",
                     help="skip the staged calibration on the 1x128 channel")
     args = ap.parse_args()
@@ -892,29 +925,30 @@ def main():
 
     single = None
     if not args.no_single:
-        single = single_solve_leg(not args.no_cpu_baseline)
+        single = _leg("single_solve", single_solve_leg, not args.no_cpu_baseline)
 
     served = None
     if not args.no_serve:
-        served = served_leg(args.serve_requests, args.serve_rate, args.serve_hills)
+        served = _leg("served", served_leg, args.serve_requests, args.serve_rate,
+                      args.serve_hills)
 
     warm = None
     if not args.no_warm:
-        warm = warm_leg(2)
+        warm = _leg("warm_start", warm_leg, 2)
 
     warm_fd = None
     if not args.no_warm_fd:
-        warm_fd = warm_fd_leg(not args.no_cpu_baseline)
+        warm_fd = _leg("warm_fd_jacobian", warm_fd_leg, not args.no_cpu_baseline)
 
     hills = None
     if not args.no_hills:
-        hills = hills_leg(not args.no_hills_full, _cores(), not args.no_cpu_baseline,
-                          tuple(args.hill_meshes.split(",")), args.hills_block_size,
-                          args.hills_grid_blocks)
+        hills = _leg("hills", hills_leg, not args.no_hills_full, _cores(),
+                     not args.no_cpu_baseline, tuple(args.hill_meshes.split(",")),
+                     args.hills_block_size, args.hills_grid_blocks, args.hills_plain_launch)
 
     calibration = None
     if not args.no_calibration:
-        calibration = calibration_leg()
+        calibration = _leg("calibration", calibration_leg)
 
     timing = batch.last_timing()
     batch.close()  # every engine is released before the interpreter exits

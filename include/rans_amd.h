@@ -81,6 +81,11 @@ typedef struct rq_settings {
     int32_t amg_block_rows;  /* multi-workgroup draws: multigrid levels of at most this many
                                 rows run on one workgroup of the draw, larger ones on all of
                                 them (0 = 1024); the results do not depend on it */
+    int32_t grid_plain_launch; /* multi-workgroup draws: 0 = cooperative launch (co-residency
+                                  guaranteed by the runtime); 1 = ordinary dispatch of the same
+                                  grid, sized by the same occupancy check, for tools that cannot
+                                  trace cooperative launches. Every grid-barrier wait is bounded,
+                                  so a grid that is not co-resident ends as Unknown rows. */
 } rq_settings;
 
 /* InferenceParameterSet (core/include/InferenceParameters.hpp:18-188). */
