@@ -19,9 +19,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .config import (PATCH_TYPES_ORDER, SCHEMES, SOLVERS, STATUS_NAMES, FlowBoundaryConditions,
-                     InferenceParameterSet, ObservationOperator, SolverSettings, Vec3,
-                     validate_coefficients)
+from .config import (BC_NEUMANN, PATCH_TYPES_ORDER, SCHEMES, SOLVERS, STATUS_NAMES,
+                     FlowBoundaryConditions, InferenceParameterSet, ObservationOperator,
+                     SolverSettings, Vec3, validate_coefficients)
 from .mesh import Mesh
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librans_amd.so")
@@ -39,7 +39,7 @@ class RqMesh(C.Structure):
                 ("face_delta", _dp), ("face_d", _dp), ("face_weight", _dp), ("face_shift", _dp),
                 ("cell_center", _dp), ("cell_volume", _dp), ("wall_distance", _dp),
                 ("cell_face_ptr", _ip), ("cell_face_idx", _ip), ("patch_type", _ip),
-                ("patch_face_ptr", _ip), ("patch_face_idx", _ip)]
+                ("patch_face_ptr", _ip), ("patch_face_idx", _ip), ("cell_folded", _dp)]
 
 
 class RqBc(C.Structure):
@@ -197,7 +197,30 @@ def settings_struct(s: SolverSettings, mixed_precision: bool = False, block_size
     return r
 
 
-def mesh_struct(m: Mesh, keep: _Keep):
+def planar_problem(mesh: Mesh, bcs: FlowBoundaryConditions):
+    """The mesh and boundary conditions the engine solves on: a planar mesh
+    as it is; a one-layer extruded mesh (an OpenFOAM 2D case) with its
+    front and back faces folded out (Mesh.fold_empty_faces) and its "empty"
+    patches' conditions dropped. Those must stay zero-gradient in every
+    field, as channelDefaults leaves a patch of another type
+    (BoundaryCondition.hpp:160-166): only then is a folded face the
+    zero-gradient term the engine adds. Returns (mesh, bcs, folded or None)."""
+    planar, kept, folded = mesh.fold_empty_faces()
+    if folded is None:
+        return mesh, bcs, None
+    for i in range(mesh.n_patches()):
+        if i in kept:
+            continue
+        for lst in (bcs.velocity, bcs.pressure, bcs.k, bcs.omega):
+            if lst[i].type != BC_NEUMANN:
+                raise ValueError(f"patch {mesh.patches[i].name!r} of type 'empty' needs "
+                                 "zero-gradient conditions in every field")
+    pick = lambda lst: [lst[i] for i in kept]  # noqa: E731
+    return planar, FlowBoundaryConditions(pick(bcs.velocity), pick(bcs.pressure),
+                                          pick(bcs.k), pick(bcs.omega)), folded
+
+
+def mesh_struct(m: Mesh, keep: _Keep, folded: np.ndarray = None):
     r = RqMesh()
     r.n_cells, r.n_internal, r.n_faces, r.n_patches = m.n_cells, m.n_internal, m.n_faces, \
         m.n_patches()
@@ -215,6 +238,7 @@ def mesh_struct(m: Mesh, keep: _Keep):
     r.patch_face_ptr = keep.i(ptr)
     r.patch_face_idx = keep.i(np.concatenate([p.faces for p in m.patches])
                               if m.patches else np.zeros(1, np.int32))
+    r.cell_folded = keep.d(folded) if folded is not None else None
     return r
 
 
@@ -319,9 +343,9 @@ class Problem:
         settings.validate()
         param_set.validate()
         bcs.validate(mesh)
-        mesh.validate_2d()
         if not mesh.wall_distance_valid():
             raise ValueError(f"{owner} needs compute_wall_distance before construction")
+        mesh, bcs, folded = planar_problem(mesh, bcs)
         if not math.isfinite(nu) or nu <= 0:
             raise ValueError("SIMPLE kinematic viscosity must be finite and positive")
         if isinstance(u_init, (tuple, list)):
@@ -330,7 +354,7 @@ class Problem:
             raise ValueError("planar solver: initial velocity must have zero z component")
         prepared = obs_op.prepare(mesh)
         self.keep = keep = _Keep()
-        self.m = mesh_struct(mesh, keep)
+        self.m = mesh_struct(mesh, keep, folded)
         b = self.b = RqBc()
         ut, uv, pt, pv, kt, kv, wt, wv = bcs.arrays()
         b.u_type, b.p_type, b.k_type, b.w_type = keep.i(ut), keep.i(pt), keep.i(kt), keep.i(wt)
@@ -428,7 +452,7 @@ class ForwardBatch:
         self.param_set = param_set
         self.n_obs = obs_op.n_obs()
         self.n_cells = mesh.n_cells
-        self.n_faces = mesh.n_faces
+        self.n_faces = int(pr.m.n_faces)  # in-plane faces (front/back faces are folded)
         err = C.create_string_buffer(512)
         h = self.lib.rq_engine_create(*pr.args(), err, 512)
         if not h:

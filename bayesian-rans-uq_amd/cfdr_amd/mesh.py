@@ -13,6 +13,8 @@ component and every cell/face centre sits at z = dz/2.
 from __future__ import annotations
 
 import math
+import os
+import re
 import struct
 from dataclasses import dataclass, field
 
@@ -102,6 +104,68 @@ class Mesh:
         """The solver path is planar: every face normal must lie in x-y."""
         if np.any(self.face_normal[:, 2] != 0.0):
             raise ValueError("mesh must be 2D: every face normal needs a zero z component")
+
+    def fold_empty_faces(self):
+        """The planar form of a one-layer extruded mesh (an OpenFOAM 2D case).
+
+        Its front and back faces (unit normal along z) must make up the
+        patches of type "empty"; every other face normal must lie in x-y
+        exactly. In the reference's solve such a face adds, in the momentum,
+        k and omega rows of its owner, the zero-gradient boundary terms
+        diag += D and source += D phi_b with D = nuEff S / max(delta, 1e-20)
+        and phi_b the owner's current value, and nothing else: its flux is
+        zero and its in-plane normal components are zero, so it drops out of
+        the x-y gradients, the fluxes and the pressure correction. Returns
+        (planar mesh, indices of the kept patches, per-cell folded faces
+        [n_cells, 4] as (S, delta) twice in face order, zeros where a cell
+        has fewer) or (self, all patch indices, None) for a planar mesh."""
+        nz = self.face_normal[:, 2]
+        zf = np.abs(nz) > 0.5
+        if not zf.any():
+            self.validate_2d()
+            return self, list(range(self.n_patches())), None
+        if np.any(nz[~zf] != 0.0):
+            raise ValueError("mesh must be 2D: every face normal needs a zero z component "
+                             "(except the front and back faces of an 'empty' patch)")
+        if np.any(zf[:self.n_internal]):
+            raise ValueError("a z-normal internal face: only one-layer (2D) meshes are supported")
+        patch_of = self.face_patch
+        empty = [i for i, p in enumerate(self.patches) if p.type == "empty"]
+        if np.any(zf & ~np.isin(patch_of, empty)):
+            raise ValueError("front and back faces must belong to a patch of type 'empty'")
+        for i in empty:
+            if not np.all(zf[self.patches[i].faces]):
+                raise ValueError(f"patch {self.patches[i].name!r} of type 'empty' holds a face "
+                                 "whose normal is not along z")
+        keep = ~zf
+        new_id = np.cumsum(keep) - 1
+        folded = np.zeros((self.n_cells, 4))
+        ptr = np.zeros(self.n_cells + 1, np.int32)
+        idx = []
+        for c in range(self.n_cells):
+            k = 0
+            for f in self.cell_face_idx[self.cell_face_ptr[c]:self.cell_face_ptr[c + 1]]:
+                if zf[f]:
+                    if k == 2:
+                        raise ValueError("a cell with more than two front/back faces")
+                    folded[c, 2 * k] = self.face_area[f]
+                    folded[c, 2 * k + 1] = max(self.face_delta[f], 1e-20)
+                    k += 1
+                else:
+                    idx.append(int(new_id[f]))
+            ptr[c + 1] = len(idx)
+        kept = [i for i in range(self.n_patches()) if i not in empty]
+        patches = [Patch(self.patches[i].name, self.patches[i].type,
+                         new_id[self.patches[i].faces].astype(np.int32)) for i in kept]
+        m = Mesh(self.n_internal, self.owner[keep].copy(), self.neighbor[keep].copy(),
+                 self.face_center[keep].copy(), self.face_normal[keep].copy(),
+                 self.face_area[keep].copy(), self.face_delta[keep].copy(),
+                 self.face_d[keep].copy(), self.face_weight[keep].copy(),
+                 self.face_shift[keep].copy(), self.cell_center.copy(), self.cell_volume.copy(),
+                 ptr, np.asarray(idx, np.int32), patches,
+                 None if self.nodes is None else self.nodes.copy(),
+                 None if self.wall_distance_ is None else self.wall_distance_.copy())
+        return m, kept, folded
 
     # ---- wall distance (core/src/Mesh.cpp:431-463) ----------------------
     def compute_wall_distance(self) -> None:
@@ -223,6 +287,75 @@ class Mesh:
             fh.write(out)
 
     @staticmethod
+    def load_from_file(path: str) -> "Mesh":
+        """Mesh::loadFromFile (core/src/Mesh.cpp:49-75): a directory is an
+        OpenFOAM case or polyMesh directory, a .foam or extensionless path an
+        OpenFOAM polyMesh, a .msh file the binary format."""
+        if os.path.isdir(path):
+            if os.path.exists(os.path.join(path, "points")) and \
+                    os.path.exists(os.path.join(path, "faces")):
+                return Mesh.load_openfoam(path)
+            poly = os.path.join(path, "constant", "polyMesh")
+            if os.path.exists(os.path.join(poly, "points")):
+                return Mesh.load_openfoam(poly)
+            raise RuntimeError("Cannot find polyMesh in: " + path)
+        ext = os.path.splitext(path)[1]
+        if ext in (".foam", ""):
+            return Mesh.load_openfoam(path)
+        if ext == ".msh":
+            return Mesh.load_binary(path)
+        raise RuntimeError("Unknown mesh format: " + ext)
+
+    @staticmethod
+    def load_openfoam(poly_mesh_dir: str) -> "Mesh":
+        """Mesh::loadOpenFOAM (core/src/Mesh.cpp:119-320): points, faces,
+        owner, neighbour and boundary of an ASCII polyMesh. Faces keep the
+        file's order (internal faces first, then each patch from its
+        startFace); a cell's face list is its faces in ascending order;
+        geometry follows the reference's conventions, so every value is the
+        one its loader computes. A 2D case is one layer of cells whose front
+        and back faces form a patch of type "empty"."""
+        j = os.path.join
+        n_pts, blk = _foam_block(j(poly_mesh_dir, "points"), "points file")
+        nodes = np.asarray(_foam_numbers(blk, n_pts, _foam_vector), dtype=float).reshape(-1, 3)
+        n_f, blk = _foam_block(j(poly_mesh_dir, "faces"), "faces file")
+        face_nodes = _foam_numbers(blk, n_f, _foam_face)
+        n_own, blk = _foam_block(j(poly_mesh_dir, "owner"), f"owner file in {poly_mesh_dir}")
+        owner = np.asarray(_foam_numbers(blk, n_own, _foam_label), dtype=np.int32)
+        n_nbr, blk = _foam_block(j(poly_mesh_dir, "neighbour"),
+                                 f"neighbour file in {poly_mesh_dir}")
+        nbr_list = np.asarray(_foam_numbers(blk, n_nbr, _foam_label), dtype=np.int32)
+        if n_own != n_f or n_nbr > n_f:
+            raise RuntimeError("polyMesh owner/neighbour counts do not match the faces")
+        if any(v < 0 or v >= n_pts for fn in face_nodes for v in fn) or \
+                any(len(fn) < 3 for fn in face_nodes):
+            raise RuntimeError("polyMesh face refers to a missing point")
+        neighbor = np.full(n_f, -1, np.int32)
+        neighbor[:n_nbr] = nbr_list
+        if np.any(owner < 0) or np.any(nbr_list < 0):
+            raise RuntimeError("polyMesh cell labels must be nonnegative")
+        nc = int(max(owner.max(initial=-1), nbr_list.max(initial=-1))) + 1
+        lists = [[] for _ in range(nc)]
+        for fi in range(n_f):
+            lists[owner[fi]].append(fi)
+            if neighbor[fi] >= 0:
+                lists[neighbor[fi]].append(fi)
+        ptr = np.zeros(nc + 1, np.int32)
+        ptr[1:] = np.cumsum([len(v) for v in lists])
+        idx = np.asarray([f for v in lists for f in v], np.int32)
+        patches = []
+        for name, ptype, nfaces, start in _foam_boundary(j(poly_mesh_dir, "boundary")):
+            if start < 0 or nfaces < 0 or start + nfaces > n_f:
+                raise RuntimeError(f"patch {name} lies outside the face list")
+            patches.append(Patch(name, ptype, np.arange(start, start + nfaces, dtype=np.int32)))
+        fcen, fnorm, farea = _polygon_faces(nodes, face_nodes)
+        shift = np.zeros((n_f, 3))
+        cc, vol = _cells_from_faces(nc, owner, neighbor, fcen, fnorm, farea, shift, ptr, idx)
+        d, delta, w = _face_metrics(n_nbr, owner, neighbor, fcen, fnorm, shift, cc)
+        return Mesh(n_nbr, owner, neighbor, fcen, fnorm, farea, delta, d, w, shift, cc, vol,
+                    ptr, idx, patches, nodes)
+
+    @staticmethod
     def load_binary(path: str) -> "Mesh":
         """Mesh::loadBinary (core/src/Mesh.cpp:817-982), MSH2 and MSH1."""
         with open(path, "rb") as fh:
@@ -284,6 +417,235 @@ class Mesh:
                  np.asarray(ptr, np.int32), np.asarray(idx, np.int32), patches, nodes)
         _admit_loaded(m)
         return m
+
+
+# ---------------------------------------------------------------------------
+# OpenFOAM polyMesh input (core/src/Mesh.cpp:49-320 in the reference)
+# ---------------------------------------------------------------------------
+# Lines a count search passes over: banner and header keywords start with one
+# of these characters (readFoamCount, core/src/Mesh.cpp:97-115).
+_FOAM_SKIP_FIRST = set("/FOovCa{}")
+_FOAM_TOKEN = re.compile(r"[()]|[^\s()]+")
+
+
+def _foam_strip(line: str) -> str:
+    cut = line.find("//")
+    return (line[:cut] if cut >= 0 else line).lstrip(" \t\r")
+
+
+def _foam_leading_int(text: str):
+    """The integer a C++ std::stoi reads from the start of text, or None."""
+    m = re.match(r"\s*[+-]?\d+", text)
+    return int(m.group(0)) if m else None
+
+
+def _foam_block(path: str, what: str):
+    """(count, tokens of the data block) of one polyMesh file: the first
+    line holding an integer (after the header) is the count, the data start
+    after the next line that is exactly "(" (core/src/Mesh.cpp:85-115)."""
+    try:
+        with open(path) as fh:
+            lines = fh.read().splitlines()
+    except OSError:
+        raise RuntimeError(f"Cannot open {what}") from None
+    count, at = None, 0
+    while at < len(lines):
+        line = _foam_strip(lines[at]).rstrip(" \t\r")
+        at += 1
+        if not line or line[0] in _FOAM_SKIP_FIRST:
+            continue
+        count = _foam_leading_int(line)
+        if count is not None:
+            break
+    if count is None:
+        raise RuntimeError("Could not read count")
+    while at < len(lines):
+        line = _foam_strip(lines[at])
+        at += 1
+        if line == "(":
+            return count, lines[at:]
+    raise RuntimeError("Could not find data block")
+
+
+def _foam_numbers(lines, count: int, per_item):
+    """Parse count items of a data block; per_item(tokens, pos) -> (item, pos)."""
+    toks = _FOAM_TOKEN.findall("\n".join(lines))
+    out, pos = [], 0
+    try:
+        for _ in range(count):
+            item, pos = per_item(toks, pos)
+            out.append(item)
+    except (IndexError, ValueError):
+        raise RuntimeError("Malformed OpenFOAM data block") from None
+    return out
+
+
+def _foam_vector(toks, pos):
+    if toks[pos] != "(" or toks[pos + 4] != ")":
+        raise ValueError
+    return (float(toks[pos + 1]), float(toks[pos + 2]), float(toks[pos + 3])), pos + 5
+
+
+def _foam_face(toks, pos):
+    nv = int(toks[pos])
+    if toks[pos + 1] != "(" or toks[pos + 2 + nv] != ")":
+        raise ValueError
+    return [int(t) for t in toks[pos + 2:pos + 2 + nv]], pos + 3 + nv
+
+
+def _foam_label(toks, pos):
+    return int(toks[pos]), pos + 1
+
+
+def _foam_boundary(path: str):
+    """Patches (name, type, nFaces, startFace) of a polyMesh boundary file
+    (core/src/Mesh.cpp:205-262); an absent file means no patches."""
+    if not os.path.exists(path):
+        return []
+    n, lines = _foam_block(path, "boundary file")
+    patches, at = [], 0
+    for _ in range(n):
+        name = None
+        while at < len(lines):
+            line = _foam_strip(lines[at]).rstrip(" \t\r")
+            at += 1
+            if line and line not in ("{", "}"):
+                name = line
+                break
+        ptype, nfaces, start = "", 0, 0
+        while at < len(lines):
+            line = _foam_strip(lines[at])
+            at += 1
+            if "}" in line:
+                break
+            parts = line.split()
+            if not parts:
+                continue
+            if parts[0] == "type" and len(parts) > 1:
+                ptype = parts[1][:-1] if parts[1].endswith(";") else parts[1]
+            elif parts[0] == "nFaces" and len(parts) > 1:
+                nfaces = _foam_leading_int(parts[1]) or 0
+            elif parts[0] == "startFace" and len(parts) > 1:
+                start = _foam_leading_int(parts[1]) or 0
+        patches.append((name or "", ptype, nfaces, start))
+    return patches
+
+
+def _vnorm(v):
+    return np.sqrt(v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1] + v[..., 2] * v[..., 2])
+
+
+def _vcross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _vdot(a, b):
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _polygon_faces(nodes: np.ndarray, face_nodes):
+    """Centre (area centroid of the triangle fan about the node average),
+    area and unit normal of every face (core/src/Mesh.cpp:268-303): each sum
+    runs over the face's nodes in order, faces of equal node count at once."""
+    nf = len(face_nodes)
+    center = np.zeros((nf, 3))
+    normal = np.zeros((nf, 3))
+    area = np.zeros(nf)
+    by_count = {}
+    for fi, fn in enumerate(face_nodes):
+        by_count.setdefault(len(fn), []).append(fi)
+    for nv, ids in by_count.items():
+        ids = np.asarray(ids)
+        conn = np.asarray([face_nodes[i] for i in ids], dtype=np.int64)
+        pts = nodes[conn]  # [m, nv, 3]
+        ctr = np.zeros((len(ids), 3))
+        for j in range(nv):
+            ctr = ctr + pts[:, j]
+        ctr = ctr / float(nv)
+        an = np.zeros((len(ids), 3))
+        for j in range(nv):
+            an = an + _vcross(pts[:, j] - ctr, pts[:, (j + 1) % nv] - ctr) * 0.5
+        a = _vnorm(an)
+        safe = np.where(a > 1e-300, a, 1.0)
+        n = np.where((a > 1e-300)[:, None], an / safe[:, None], 0.0)
+        csum = np.zeros((len(ids), 3))
+        asum = np.zeros(len(ids))
+        for j in range(nv):
+            pa, pb = pts[:, j], pts[:, (j + 1) % nv]
+            at = 0.5 * _vdot(_vcross(pa - ctr, pb - ctr), n)
+            csum = csum + ((ctr + pa) + pb) * (at / 3.0)[:, None]
+            asum = asum + at
+        ok = np.abs(asum) > 1e-300
+        c = np.where(ok[:, None], csum / np.where(ok, asum, 1.0)[:, None], ctr)
+        center[ids], normal[ids], area[ids] = c, n, a
+    return center, normal, area
+
+
+def _cells_from_faces(nc, owner, neighbor, fcen, fnorm, farea, shift, ptr, idx):
+    """Exact cell volumes and centroids (core/src/Mesh.cpp:340-373): pyramids
+    on the faces about the face-centre average for closed cells, extruded
+    triangles for cells without z-normal faces; cells of equal face count at
+    once, each sum over the cell's faces in its face-list order."""
+    center = np.zeros((nc, 3))
+    volume = np.zeros(nc)
+    counts = np.diff(ptr)
+    for k in np.unique(counts):
+        cells = np.nonzero(counts == k)[0]
+        faces = np.stack([idx[ptr[cells] + j] for j in range(k)], axis=1)  # [m, k]
+        own = owner[faces] == cells[:, None]
+        local = np.where(own[..., None], fcen[faces], fcen[faces] - shift[faces])
+        c0 = np.zeros((len(cells), 3))
+        for j in range(k):
+            c0 = c0 + local[:, j]
+        c0 = c0 / float(k)
+        zopen = ~np.any(np.abs(fnorm[faces][..., 2]) > 1e-9, axis=1)
+        hsum = np.zeros(len(cells))
+        csum = np.zeros((len(cells), 3))
+        for j in range(k):
+            sign = np.where(own[:, j], 1.0, -1.0)
+            cf = local[:, j]
+            h = sign * _vdot(cf - c0, fnorm[faces[:, j]]) * farea[faces[:, j]]
+            piece = np.where(zopen[:, None], (c0 + cf * 2.0) / 3.0, cf * 0.75 + c0 * 0.25)
+            hsum = hsum + h
+            csum = csum + piece * h[:, None]
+        vol = hsum / np.where(zopen, 2.0, 3.0)
+        ok = np.abs(hsum) > 1e-300
+        cen = np.where(ok[:, None], csum / np.where(ok, hsum, 1.0)[:, None], c0)
+        vol = np.where(vol < 1e-30, 1e-20, vol)
+        center[cells], volume[cells] = cen, vol
+    return center, volume
+
+
+def _face_metrics(nint, owner, neighbor, fcen, fnorm, shift, cc):
+    """Owner-to-neighbour vectors, distances and interpolation weights
+    (core/src/Mesh.cpp:377-415)."""
+    nf = owner.shape[0]
+    d = np.zeros((nf, 3))
+    delta = np.zeros(nf)
+    w = np.full(nf, 0.5)
+    fi = np.arange(nint)
+    if nint:
+        dd = (cc[neighbor[fi]] + shift[fi]) - cc[owner[fi]]
+        d[fi] = dd
+        delta[fi] = _vnorm(dd)
+        bad = np.nonzero(_vdot(dd, fnorm[fi]) <= 0.0)[0]
+        if bad.size:
+            raise RuntimeError(f"Mesh::computeFaceGeometry: internal face {int(bad[0])} has its "
+                               "neighbour behind it; periodic face without its shift?")
+    fb = np.arange(nint, nf)
+    if fb.size:
+        dd = fcen[fb] - cc[owner[fb]]
+        d[fb] = dd
+        delta[fb] = _vnorm(dd)
+    delta = np.where(delta < 1e-30, 1e-20, delta)
+    if nint:
+        dP = _vnorm(fcen[fi] - cc[owner[fi]])
+        dN = _vnorm(fcen[fi] - (cc[neighbor[fi]] + shift[fi]))
+        s = dP + dN
+        w[fi] = np.where(s > 1e-30, dN / np.where(s > 1e-30, s, 1.0), 0.5)
+    return d, delta, w
 
 
 def _require_positive(value: float, name: str) -> None:

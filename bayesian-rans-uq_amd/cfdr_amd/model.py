@@ -142,7 +142,7 @@ class ForwardModel:
                             omega_init, False, block_size, False, grid_blocks, "ForwardModel")
         self._param_set = param_set
         self.n_obs = obs_op.n_obs()
-        self.n_cells, self.n_faces = mesh.n_cells, mesh.n_faces
+        self.n_cells, self.n_faces = mesh.n_cells, int(pr.m.n_faces)
         opt = RqCacheOptions(cache_max_size, cache_threshold, cache_byte_budget)
         err = C.create_string_buffer(512)
         h = self.lib.rq_model_create(*pr.args(), C.byref(opt), err, 512)

@@ -124,11 +124,11 @@ class Server:
         settings.validate()
         param_set.validate()
         bcs.validate(mesh)
-        mesh.validate_2d()
         if not mesh.wall_distance_valid():
             raise ValueError("add_problem needs compute_wall_distance before registration")
+        mesh, bcs, folded = engine.planar_problem(mesh, bcs)
         keep = engine._Keep()
-        m = engine.mesh_struct(mesh, keep)
+        m = engine.mesh_struct(mesh, keep, folded)
         b = engine.RqBc()
         ut, uv, pt, pv, kt, kv, wt, wv = bcs.arrays()
         b.u_type, b.p_type, b.k_type, b.w_type = keep.i(ut), keep.i(pt), keep.i(kt), keep.i(wt)

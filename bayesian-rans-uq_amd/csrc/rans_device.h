@@ -127,6 +127,7 @@ struct DevMesh {
     int nc, nif, nf, nb, npatch;
     int has_outlet, planar;
     const double *wall_y;        // [nc] wall-face distance of the constrained cell, NaN otherwise
+    const double *folded;        // [4 nc] (area, delta) x 2 of folded front/back faces, or null
     int nobs;                    // prepared observations (their stencils are in the image)
     ArenaLayout lay;  // per-draw arena layout for this mesh
     AmgMesh amg;      // pressure-correction multigrid (levels == 0 when unused)

@@ -90,6 +90,7 @@ struct rq_engine {
     int n_active = 0, nobs = 0;
     bool wall_crossings = false;  // separation or reattachment observations present
     bool drag = false;            // drag observations present
+    bool folded = false;          // front/back faces of a one-layer mesh folded into the rows
     std::vector<int> active;
     std::vector<double> base, lower, upper, observed, variance;
     DevSettings settings{};
@@ -583,6 +584,12 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
     for (int f = 0; f < nf; ++f)
         if (seen[f] != (f < nif ? 2 : 1)) throw Fail{RQ_EINVAL, "incomplete cell face lists"};
     if (K > kSlots) throw Fail{RQ_EINVAL, "cells with more than 4 faces are not supported"};
+    if (m.cell_folded)
+        for (int64_t i = 0; i < 2 * (int64_t)nc; ++i) {
+            const double a = m.cell_folded[2 * i], d = m.cell_folded[2 * i + 1];
+            if (!std::isfinite(a) || a < 0 || (a > 0 && !(std::isfinite(d) && d > 0)))
+                throw Fail{RQ_EINVAL, "invalid folded front/back face geometry"};
+        }
     // boundary faces: patches must list them in face order
     std::vector<int> bpatch(nb, -1);
     {
@@ -840,6 +847,9 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
                 soa[L.bnd + (int64_t)v * nbs + b] = image[L.bnd + (int64_t)b * kNB + v];
         const size_t o_soa = B.add(soa);
         size_t o_wy = B.add(wall_y);
+        size_t o_fold = 0;
+        if (m.cell_folded)
+            o_fold = B.add(std::vector<double>(m.cell_folded, m.cell_folded + 4 * (int64_t)nc));
         struct LevOff {
             size_t rowptr, col, diagpos, agg, mptr, midx, gptr, gidx;
         };
@@ -864,6 +874,8 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         D.nc = nc; D.nif = nif; D.nf = nf; D.nb = nb; D.npatch = np;
         D.has_outlet = has_outlet; D.planar = planar;
         D.wall_y = Dd(o_wy);
+        D.folded = m.cell_folded ? Dd(o_fold) : nullptr;
+        E->folded = m.cell_folded != nullptr;
         D.nobs = no;
         if (use_amg) {
             auto I = [&](size_t o) { return reinterpret_cast<const int *>(b0 + o); };
@@ -1096,10 +1108,11 @@ int choose_grid_blocks(const rq_engine *E, int ns, size_t grid_lds) {
         nblk = std::min(nblk, fit);
         if (nblk < 2) nblk = 1;
     }
-    // the multi-workgroup kernels do not evaluate drag, and the mixed-precision
-    // one neither separation nor reattachment: such problems run one
-    // workgroup per draw
-    if (nblk > 1 && (E->drag || (E->settings.mixed_precision != 0 && E->wall_crossings)))
+    // the multi-workgroup kernels do not evaluate drag or fold front/back
+    // faces, and the mixed-precision one evaluates neither separation nor
+    // reattachment: such problems run one workgroup per draw
+    if (nblk > 1 && (E->drag || E->folded ||
+                     (E->settings.mixed_precision != 0 && E->wall_crossings)))
         nblk = 1;
     return nblk;
 }

@@ -731,6 +731,7 @@ struct Draw {
             sr = (q.valid[s] && !in) ? sr + sB : sr;
             offv[s] = in ? oIn : 0.0;
         }
+        if constexpr (!kGrid) if (M.folded) folded_faces(ci, nu + nuTc, C(ci, Uc), dg, sr);
         const double V = C(ci, V_vol);
         const double dpdx = comp == 0 ? C(ci, V_gPx) : C(ci, V_gPy);
         sr -= dpdx * V;
@@ -795,6 +796,25 @@ struct Draw {
         return sr - y;
     }
 
+    // The front and back faces of a one-layer extruded mesh, folded out of
+    // the slots by the host: each is a zero-gradient boundary face with no
+    // flux, so it adds D = nuEff S / delta to the diagonal and D phi_b to the
+    // source of its owner's row, phi_b being the cell's current value (the
+    // boundary-face loops of SIMPLESolver.cpp:360-382, 983-1001, 1092-1113,
+    // after the cell's in-plane faces). nuEff is the row's boundary-face
+    // diffusivity. Not compiled into the multi-workgroup kernels; the host
+    // runs such meshes on one workgroup per draw.
+    DEV void folded_faces(int ci, double nuEff, double phi, double &dg, double &sr) const {
+        const double *fp = M.folded + 4 * (int64_t)ci;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const double A = fp[2 * e], dl = fp[2 * e + 1];
+            const double D = nuEff * A / dl;
+            dg = A > 0.0 ? dg + D : dg;
+            sr = A > 0.0 ? sr + D * phi : sr;
+        }
+    }
+
     // ---------------------------------------------------------- k / omega rows
     // Diffusion and upwind convection of one k or omega row over its slots
     // (the face loops of assembleKEquation / assembleOmegaEquation): sig0 and
@@ -805,8 +825,8 @@ struct Draw {
     struct TurbSlots {
         double Sf[kSlots], delta[kSlots], mFlux[kSlots], nuEff[kSlots];
     };
-    DEV void turb_faces(int ci, const SlotSet &q, double sig0, double sig1, int Bv, double F1c,
-                        double nuTc, double &dg, double &sr, double (&offv)[kSlots],
+    DEV void turb_faces(int ci, const SlotSet &q, double sig0, double sig1, int Bv, int Cv,
+                        double F1c, double nuTc, double &dg, double &sr, double (&offv)[kSlots],
                         TurbSlots &t) const {
         const double nu = S.nu;
         // boundary faces: the cell's blended coefficient
@@ -844,6 +864,7 @@ struct Draw {
             t.mFlux[s] = m;
             t.nuEff[s] = nuEff_f;
         }
+        if constexpr (!kGrid) if (M.folded) folded_faces(ci, nuEffB, C(ci, Cv), dg, sr);
     }
 
     // deferred convection correction of a k or omega row (internal slots)
@@ -868,7 +889,7 @@ struct Draw {
         double dg = 0.0, sr = 0.0, offv[kSlots] = {0.0, 0.0, 0.0, 0.0};
         const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
         TurbSlots t;
-        turb_faces(ci, q, c[0], c[4], B_k, F1c, nuTc, dg, sr, offv, t);
+        turb_faces(ci, q, c[0], c[4], B_k, V_k, F1c, nuTc, dg, sr, offv, t);
         const double V = C(ci, V_vol);
         sr += C(ci, V_Pk) * V;
         const double destruction = c[8] * smax(C(ci, V_w), 1e-20);
@@ -906,7 +927,7 @@ struct Draw {
         const double amplitude = 6 * nu / c[2];
         const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
         TurbSlots t;
-        turb_faces(ci, q, c[1], c[5], B_w, F1c, nuTc, dg, sr, offv, t);
+        turb_faces(ci, q, c[1], c[5], B_w, V_w, F1c, nuTc, dg, sr, offv, t);
         if (planar) {
             const double omsc = C(ci, V_oms);
 #pragma unroll

@@ -44,6 +44,13 @@ typedef struct rq_mesh_desc {
     const int32_t *cell_face_ptr, *cell_face_idx; /* each cell's faces, ascending */
     const int32_t *patch_type;               /* 0 wall 1 inlet 2 outlet 3 symmetry 4 other */
     const int32_t *patch_face_ptr, *patch_face_idx;
+    /* Front and back faces of a one-layer extruded mesh (an OpenFOAM 2D
+     * case, core/src/Mesh.cpp:119-320), folded out of the face arrays:
+     * per cell (area, delta) of up to two such faces in face order, area 0
+     * where a cell has fewer; NULL for a planar mesh. Each adds a
+     * zero-gradient boundary term (no flux) to the cell's momentum, k and
+     * omega rows (SIMPLESolver.cpp:360-382, 983-1001, 1092-1113). */
+    const double *cell_folded; /* [4*n_cells] or NULL */
 } rq_mesh_desc;
 
 /* FlowBoundaryConditions (core/include/BoundaryCondition.hpp:105-167):

@@ -26,7 +26,8 @@ class OrMesh(C.Structure):
                 ("fdelta", _dp), ("fdx", _dp), ("fdy", _dp), ("fweight", _dp),
                 ("fshx", _dp), ("fshy", _dp), ("ccx", _dp), ("ccy", _dp), ("vol", _dp),
                 ("wall_dist", _dp), ("cell_face_ptr", _ip), ("cell_face_idx", _ip),
-                ("patch_type", _ip), ("patch_face_ptr", _ip), ("patch_face_idx", _ip)]
+                ("patch_type", _ip), ("patch_face_ptr", _ip), ("patch_face_idx", _ip),
+                ("cell_folded", _dp)]
 
 
 class OrBc(C.Structure):
@@ -128,7 +129,25 @@ class _Keep:
         return a.ctypes.data_as(_ip)
 
 
-def marshal(mesh, bcs, settings, prepared, keep: _Keep):
+def planar(mesh, bcs):
+    """A one-layer extruded mesh (an OpenFOAM 2D case) with its front and
+    back faces folded out and its "empty" patches' conditions dropped, as
+    the reference's solve sees them (zero-gradient faces without flux);
+    (mesh, bcs, None) for a planar mesh."""
+    from cfdr_amd.config import BC_NEUMANN, FlowBoundaryConditions
+    flat, kept, folded = mesh.fold_empty_faces()
+    if folded is None:
+        return mesh, bcs, None
+    for i in range(mesh.n_patches()):
+        if i not in kept and any(lst[i].type != BC_NEUMANN
+                                 for lst in (bcs.velocity, bcs.pressure, bcs.k, bcs.omega)):
+            raise ValueError("an 'empty' patch needs zero-gradient conditions in every field")
+    pick = lambda lst: [lst[i] for i in kept]  # noqa: E731
+    return flat, FlowBoundaryConditions(pick(bcs.velocity), pick(bcs.pressure), pick(bcs.k),
+                                        pick(bcs.omega)), folded
+
+
+def marshal(mesh, bcs, settings, prepared, keep: _Keep, folded=None):
     from cfdr_amd.config import PATCH_TYPES_ORDER, SCHEMES, SOLVERS  # noqa: F401
     m = OrMesh()
     m.n_cells, m.n_internal, m.n_faces = mesh.n_cells, mesh.n_internal, mesh.n_faces
@@ -151,6 +170,7 @@ def marshal(mesh, bcs, settings, prepared, keep: _Keep):
     m.patch_face_ptr = keep.i(pptr)
     m.patch_face_idx = keep.i(np.concatenate([p.faces for p in mesh.patches])
                               if mesh.patches else np.zeros(1, np.int32))
+    m.cell_folded = keep.d(folded) if folded is not None else None
     b = OrBc()
     ut, uv, pt, pv, kt, kv, wt, wv = bcs.arrays()
     b.u_type, b.p_type, b.k_type, b.w_type = keep.i(ut), keep.i(pt), keep.i(kt), keep.i(wt)
@@ -193,9 +213,10 @@ class OracleProblem:
 
     def __init__(self, mesh, bcs, settings, obs_op, nu, init):
         self.keep = _Keep()
+        mesh, bcs, folded = planar(mesh, bcs)
         self.mesh = mesh
         self.prepared = obs_op.prepare(mesh)
-        self.structs = marshal(mesh, bcs, settings, self.prepared, self.keep)
+        self.structs = marshal(mesh, bcs, settings, self.prepared, self.keep, folded)
         self.nu = float(nu)
         self.init = _d(init)
 

@@ -1034,6 +1034,29 @@ static void convection_correction(ctx_t *X, const double *phi, const double *phi
 }
 
 /* SIMPLESolver::assembleMomentum (SIMPLESolver.cpp:303-482), components 0/1 */
+/* The front and back faces of a one-layer extruded mesh (an OpenFOAM 2D
+ * case), folded out of the face arrays: zero-gradient boundary faces with
+ * no flux, each adding D = nuEff S / delta to the diagonal and D phi to the
+ * source of its owner, phi the owner's value (the boundary-face loops of
+ * SIMPLESolver.cpp:360-382, 983-1001, 1092-1113; such faces come after the
+ * in-plane boundary faces in a case's face order). nuEff is nu + nuT for
+ * momentum (F1 NULL) and nu + sigma(F1) nuT for k and omega. */
+static void folded_faces(ctx_t *X, lsys_t *A, const double *phi, const double *F1, double s1,
+                         double s2) {
+    const double *fold = X->m->cell_folded;
+    if (!fold) return;
+    for (int ci = 0; ci < X->nc; ++ci) {
+        const double nuEff = F1 ? X->nu + blend(s1, s2, F1[ci]) * X->nuT[ci] : X->nu + X->nuT[ci];
+        for (int e = 0; e < 2; ++e) {
+            const double Sf = fold[4 * ci + 2 * e], delta = fold[4 * ci + 2 * e + 1];
+            if (!(Sf > 0.0)) continue;
+            const double Db = nuEff * Sf / delta;
+            A->diag[ci] += Db;
+            A->source[ci] += Db * phi[ci];
+        }
+    }
+}
+
 static void assemble_momentum(ctx_t *X, int comp) {
     const or_mesh_t *m = X->m;
     lsys_t *A = &X->sys;
@@ -1070,6 +1093,7 @@ static void assemble_momentum(ctx_t *X, int comp) {
             A->source[o] += (Db - mFlux) * Ub;
         }
     }
+    folded_faces(X, A, comp == 0 ? X->Ux : X->Uy, NULL, 0.0, 0.0);
     /* pressure gradient source */
     gg_grad(X, X->p, X->pb, X->gx, X->gy);
     for (int ci = 0; ci < X->nc; ++ci) {
@@ -1253,6 +1277,7 @@ static void assemble_k(ctx_t *X) {
             A->source[o] += (Db - mFlux) * kb;
         }
     }
+    folded_faces(X, A, X->k, X->F1, X->c[C_SK1], X->c[C_SK2]);
     for (int ci = 0; ci < X->nc; ++ci) {
         double vol = m->vol[ci];
         A->source[ci] += X->Pk[ci] * vol;
@@ -1318,6 +1343,7 @@ static void assemble_omega(ctx_t *X) {
             A->source[o] += (Db - mFlux) * wb;
         }
     }
+    folded_faces(X, A, X->w, X->F1, X->c[C_SW1], X->c[C_SW2]);
     for (int ci = 0; ci < X->nc; ++ci) {
         double vol = m->vol[ci];
         double F1 = X->F1[ci];

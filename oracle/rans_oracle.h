@@ -49,6 +49,10 @@ typedef struct {
     const int *cell_face_ptr, *cell_face_idx;      /* per-cell face lists, reference order */
     const int *patch_type;                         /* [n_patches] OR_PATCH_* */
     const int *patch_face_ptr, *patch_face_idx;    /* per-patch face lists */
+    /* front and back faces of a one-layer extruded mesh folded out of the
+     * face arrays: per cell (area, delta) twice in face order, area 0 where
+     * absent; NULL for a planar mesh (core/src/Mesh.cpp:119-320) */
+    const double *cell_folded;
 } or_mesh_t;
 
 typedef struct {

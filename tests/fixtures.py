@@ -58,6 +58,31 @@ def inout_channel(momentum_scheme, turbulence_scheme):
                          u_init=Vec3(1.0, 0.0, 0.0), p_init=0.0, k_init=0.001, omega_init=5.0)
 
 
+def foam_channel(max_iterations=6000):
+    """The developing channel of ref_channel_inout.json (limitedLinearUpwind,
+    PCG pressure, tol 1e-5) on the OpenFOAM 2D case foam_channel_24x12: a
+    graded 24x12 mesh of trapezoids, depth 0.5, its front and back faces an
+    "empty" patch (ref_foam_channel.json, the reference's own solve of the
+    case as its Mesh::loadFromFile reads it)."""
+    from cfdr_amd.config import (FlowBoundaryConditions, InferenceParameterSet,
+                                 ObservationOperator, SolverSettings, Vec3)
+    from cfdr_amd.mesh import Mesh
+    mesh = Mesh.load_from_file(os.path.join(GOLDEN, "foam_channel_24x12"))
+    mesh.compute_wall_distance()
+    bcs = FlowBoundaryConditions.channel_defaults(mesh, 1.0, 0.001, 5.0)
+    s = SolverSettings(max_iterations=max_iterations, convergence_tol=1e-5,
+                       pressure_solver="PCG", alpha_u=0.7, alpha_p=0.3)
+    obs = ObservationOperator()
+    for x, y, comp, observed in INOUT_PROBES:
+        obs.add_velocity_profile((x, y), comp, observed, 0.1)
+    for x, y in INOUT_TAPS:
+        obs.add_pressure_tap((x, y), 0.0, 0.1, 1.0, 0.0, 0.0, 1.0)
+    obs.add_skin_friction("bottom", (4.0, 0.0), 0.005, 0.001, 1.0)
+    return cases.Problem(mesh=mesh, bcs=bcs, settings=s, obs=obs,
+                         params=InferenceParameterSet.a1_betaStar(), nu=0.001,
+                         u_init=Vec3(1.0, 0.0, 0.0), p_init=0.0, k_init=0.001, omega_init=5.0)
+
+
 def couette_problem(pressure_solver="PCG"):
     """Moving-wall Couette flow of ref_couette.json: the reference's graded
     makeChannel2D(4, 96, 1.5, 2) mesh (MSH2 fixture), top wall at Uwall 2,

@@ -622,7 +622,39 @@ def served_channels_fixture(tmp):
     write("ref_served_channels.json", out)
 
 
-EXTRA = {"warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
+def foam_fixture(tmp):
+    """An OpenFOAM 2D case (one layer of hexahedra, the front and back faces
+    an "empty" patch), written by foam_case.py under foam_channel_24x12/ and
+    read by the reference's own Mesh::loadFromFile: the developing channel
+    (inout_lines) on a 24x12 mesh of depth 0.5 graded towards both walls
+    (ratio 1.2 per cell) with its interior rows displaced by a sine wave, so
+    the cells are trapezoids and the internal faces tilt. The reference's
+    geometry of the loaded mesh is kept as ref_foam_channel_24x12.msh; rows
+    to convergence and a 40-iteration capped state with fields."""
+    from foam_case import channel_nodes, write_extruded
+    case = os.path.join(HERE, "foam_channel_24x12")
+    x, y = channel_nodes(24, 12, 6.0, 1.0, 1.2, 0.03)
+    write_extruded(case, x, y, 0.5)
+    msh = os.path.join(HERE, "ref_foam_channel_24x12.msh")
+    thetas = [[0.31, 0.09], [0.33, 0.095]]
+    lines = [f"mesh_file {case}", f"save_mesh {msh}"] + inout_lines()
+    lines += [f"theta {a!r} {b!r}" for a, b in thetas]
+    res = run_case(lines)
+    capped = run_case([f"mesh_file {case}"] + inout_lines()
+                      + ["set max_iterations 40", "theta 0.31 0.09",
+                         f"dump {os.path.join(tmp, 'foam')}"])
+    capped["rows"][0]["fields"] = read_dump(os.path.join(tmp, "foam_0.bin"))
+    write("ref_foam_channel.json",
+          {"case": "tests/golden/foam_channel_24x12 (foam_case.channel_nodes(24, 12, 6.0, 1.0, "
+                   "1.2, 0.03), depth 0.5) with the developing-channel settings of "
+                   "ref_channel_inout.json (limitedLinearUpwind, PCG pressure, tol 1e-5, "
+                   "cap 6000)",
+           "mesh": "foam_channel_24x12", "reference_geometry": os.path.basename(msh),
+           "thetas": thetas, "rows": res["rows"],
+           "capped": {"max_iterations": 40, "theta": thetas[0], "rows": capped["rows"]}})
+
+
+EXTRA = {"foam": foam_fixture, "warm": warm_fixture, "body_force": body_force_fixture, "couette": couette_fixture,
          "plate": plate_fixture, "bfs": bfs_fixture, "crossings": crossings_fixture,
          "drag": drag_fixture, "hill_c3": hill_c3_fixture, "hill_c5": hill_c5_fixture,
          "served_channels": served_channels_fixture}

@@ -67,3 +67,20 @@ def test_capped_worst_measure_matches_one_workgroup(mixed):
         assert np.allclose(r.final_worst_measures, one.final_worst_measures, rtol=1e-6, atol=0), g
         assert np.allclose(r.contraction_rates, one.contraction_rates, rtol=1e-9, atol=0), g
         assert max_rel(r.predictions, one.predictions) <= 1e-9, g
+
+
+def test_plain_dispatch_of_the_grid_gives_the_same_bits():
+    """grid_plain_launch dispatches the same co-resident grid without the
+    cooperative launch API (for tools that cannot trace cooperative
+    launches): every output is bitwise the cooperative launch's."""
+    d = load("ref_channel_inout.json")["schemes"]["limitedLinearUpwind"]
+    prob = inout_channel(d["momentum_convection"], d["turbulence_convection"])
+    theta = np.asarray(d["thetas"])
+    coop = _batch(prob, grid_blocks=3).evaluate(theta)
+    b = _batch(prob, grid_blocks=3, grid_plain_launch=True)
+    plain = b.evaluate(theta)
+    assert b.last_timing()["grid_blocks"] == 3
+    assert [int(s) for s in plain.statuses] == [int(s) for s in coop.statuses]
+    assert list(plain.iteration_counts) == list(coop.iteration_counts)
+    assert np.array_equal(plain.predictions, coop.predictions)
+    assert np.array_equal(plain.final_worst_measures, coop.final_worst_measures)
