@@ -963,7 +963,7 @@ rq_engine *create_impl(const rq_mesh_desc *mp, const rq_param_desc *pp, const rq
         HIPCHECK(hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device));
         E->block_user = s.block_size != 0;
         if (s.block_size) E->block = s.block_size;
-        else E->block = nc <= 64 ? 64 : nc <= 128 ? 128 : nc <= 256 ? 256 : 512;
+        else E->block = nc <= 64 ? 64 : nc <= 256 ? 256 : 512;
         int maxLds = 0;
         HIPCHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                        E->device));

@@ -307,6 +307,10 @@ struct Krylov {
 
 template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false>
 struct Draw {
+    // lanes per cell of the momentum, k and omega row assembly: two in the
+    // 256-thread LDS kernels (an LDS arena holds at most about 130 cells, so
+    // one lane per cell would leave half of those threads idle), one elsewhere
+    static constexpr int QL = (kLDS && !kGrid && BS == 256) ? 2 : 1;
     const DevMesh &M;
     const DevSettings &S;
     const double *c;  // 11 coefficients (LDS)
@@ -318,6 +322,10 @@ struct Draw {
     int nc, nif, nf, nb, nbs;  // nbs = max(nb, 1): boundary arrays' length
     bool planar, has_outlet;
     const int tid = threadIdx.x;
+    // multi-lane rows: this lane's place in its cell's group, and the first
+    // cell and cell stride of the groups
+    const int qi = threadIdx.x % QL;
+    const int ct = threadIdx.x / QL, cs = BS / QL;
     // cells, faces and matrix rows are distributed over gs threads; a thread's
     // first index is gt (in one-workgroup mode: the workgroup's threads)
     int gt = threadIdx.x, gs = BS;
@@ -991,6 +999,399 @@ struct Draw {
             ok = ok && (!q.inner[s] || isfinite(offv[s]));
             y = q.inner[s] ? y + offv[s] * x0n[s] : y;
         }
+        resid = sr - y;
+        return x0;
+    }
+
+    // ------------------------------------------------ several lanes per cell
+    // The 256-thread LDS kernels assemble the momentum, k and omega rows with
+    // QL = 2 lanes per cell: lane qi of the pair evaluates the face terms of
+    // slots qi and qi + 2, the pair shares them with DPP broadcasts, and each
+    // lane adds them in slot order with the same operations as the one-lane
+    // rows above, so a row's values are bitwise the same. The row's own
+    // values are written by lane 0 of the pair; each lane writes the
+    // off-diagonals of its slots.
+    static constexpr int NJ = kSlots / QL;  // slots per lane
+    template <int L>
+    static DEV int lane_bc(int v) {  // lane L of the cell's group, in every lane of it
+        constexpr int ctrl = QL == 4 ? (L | (L << 2) | (L << 4) | (L << 6))
+                                     : (L | (L << 2) | ((2 + L) << 4) | ((2 + L) << 6));
+        return __builtin_amdgcn_mov_dpp(v, ctrl, 0xf, 0xf, false);
+    }
+    template <int L>
+    static DEV double lane_bcd(double v) {
+        const int lo = lane_bc<L>(__double2loint(v));
+        const int hi = lane_bc<L>(__double2hiint(v));
+        return __hiloint2double(hi, lo);
+    }
+    // the per-slot values a of every slot from each lane's own slots v
+    // (slot s lives in lane s % QL as its entry s / QL)
+    template <int S>
+    static DEV double slot_of(const double (&v)[NJ]) { return lane_bcd<S % QL>(v[S / QL]); }
+    template <int S>
+    static DEV int slot_ofi(const int (&v)[NJ]) { return lane_bc<S % QL>(v[S / QL]); }
+    static DEV void gatherq(const double (&v)[NJ], double (&a)[kSlots]) {
+        a[0] = slot_of<0>(v);
+        a[1] = slot_of<1>(v);
+        a[2] = slot_of<2>(v);
+        a[3] = slot_of<3>(v);
+    }
+    static DEV void gatherqi(const int (&v)[NJ], int (&a)[kSlots]) {
+        a[0] = slot_ofi<0>(v);
+        a[1] = slot_ofi<1>(v);
+        a[2] = slot_ofi<2>(v);
+        a[3] = slot_ofi<3>(v);
+    }
+    template <class T>
+    static DEV T pick(const T (&a)[kSlots], int s) {
+        return s == 0 ? a[0] : s == 1 ? a[1] : s == 2 ? a[2] : a[3];
+    }
+    // this lane's slots (qi + QL j) of a cell's slot set
+    struct OwnSlots {
+        int s[NJ], f[NJ], b[NJ], n[NJ];
+        bool valid[NJ], in[NJ], own[NJ];
+    };
+    DEV void own_slots(const SlotSet &q, OwnSlots &o) const {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int s = qi + QL * j;
+            o.s[j] = s;
+            o.f[j] = pick(q.f, s);
+            o.b[j] = pick(q.b, s);
+            o.n[j] = pick(q.n, s);
+            o.valid[j] = pick(q.valid, s);
+            o.in[j] = pick(q.inner, s);
+            o.own[j] = pick(q.own, s);
+        }
+    }
+    // y = diag x + sum over internal slots of off x_n and the row's
+    // finiteness (the residual loop of every row); writes this lane's
+    // off-diagonals
+    DEV double row_residual_q(int ci, const SlotSet &q, const OwnSlots &o,
+                              const double (&offv)[NJ], const double (&xn)[NJ], double y,
+                              bool &ok) {
+        double pr[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            if (o.in[j]) off(ci, o.s[j]) = offv[j];
+            pr[j] = offv[j] * xn[j];
+        }
+        double of4[kSlots], pr4[kSlots];
+        gatherq(offv, of4);
+        gatherq(pr, pr4);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            ok = ok && (!q.inner[s] || isfinite(of4[s]));
+            y = q.inner[s] ? y + pr4[s] : y;
+        }
+        return y;
+    }
+
+    // momentum_row with QL lanes per cell (the same operations, in the same order)
+    DEV double momentum_row_q(int ci, int comp, double &aPu, bool &ok) {
+        const double nu = S.nu;
+        const int Ubv = comp == 0 ? B_Ux : B_Uy;
+        const int Uc = comp == 0 ? V_Ux : V_Uy;
+        SlotSet q;
+        slot_set(ci, q);
+        OwnSlots o;
+        own_slots(q, o);
+        const double nuTc = C(ci, V_nuT);
+        double Sf[NJ], mFlux[NJ], wt[NJ], wmv[NJ], nuTn[NJ], offv[NJ], cd[NJ], cb[NJ];
+        bool wall[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int f = o.f[j];
+            Sf[j] = Fc(f, W_A);
+            const double delta = Fc(f, W_dlt);
+            mFlux[j] = Fc(f, W_phi);
+            wt[j] = Fc(f, W_w);
+            wmv[j] = Fc(f, W_wm);
+            nuTn[j] = C(o.n[j], V_nuT);
+            wall[j] = bflag[o.b[j]] & kFlagWall;
+            const double bU = Bd(o.b[j], Ubv);
+            const bool own = o.own[j], in = o.in[j];
+            const double nuEff_o = nu + (own ? nuTc : nuTn[j]);
+            const double nuEff_n = nu + (own ? nuTn[j] : nuTc);
+            const double nuEff_f = wt[j] * nuEff_o + wmv[j] * nuEff_n;
+            const double nuEffB = wall[j] ? nu : nu + nuTc;
+            const double D = (in ? nuEff_f : nuEffB) * Sf[j] / delta;
+            const double m = mFlux[j];
+            const double cPos = smax(m, 0.0), cNeg = smax(-m, 0.0);
+            const double dIn = own ? D + cPos : D + cNeg;
+            const double oIn = -(own ? D + cNeg : D + cPos);
+            const bool pos = m >= 0;
+            const double dB = pos ? m + D : D;
+            cb[j] = (pos ? D : D - m) * bU;
+            cd[j] = in ? dIn : dB;
+            offv[j] = in ? oIn : 0.0;
+        }
+        double cd4[kSlots], cb4[kSlots];
+        gatherq(cd, cd4);
+        gatherq(cb, cb4);
+        double dg = 0.0, sr = 0.0;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            dg = q.valid[s] ? dg + cd4[s] : dg;
+            sr = (q.valid[s] && !q.inner[s]) ? sr + cb4[s] : sr;
+        }
+        if (M.folded) folded_faces(ci, nu + nuTc, C(ci, Uc), dg, sr);
+        const double V = C(ci, V_vol);
+        const double dpdx = comp == 0 ? C(ci, V_gPx) : C(ci, V_gPy);
+        sr -= dpdx * V;
+        const double fb = comp == 0 ? S.body_force_x : S.body_force_y;
+        if (fb != 0.0) sr += fb * V;
+        {
+            const int ga = comp == 0 ? V_gUxx : V_gUxy;
+            const int gb = comp == 0 ? V_gUyx : V_gUyy;
+            const double gac = C(ci, ga), gbc = C(ci, gb);
+            double fl[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const int f = o.f[j];
+                const bool own = o.own[j], in = o.in[j];
+                const double gan_ = C(o.n[j], ga), gbn_ = C(o.n[j], gb);
+                const double nx = Fc(f, W_nx), ny = Fc(f, W_ny);
+                const double nuo = own ? nuTc : nuTn[j], nun = own ? nuTn[j] : nuTc;
+                const double gao = own ? gac : gan_, gan = own ? gan_ : gac;
+                const double gbo = own ? gbc : gbn_, gbn = own ? gbn_ : gbc;
+                const double cfI = wt[j] * nuo + wmv[j] * nun;
+                const double gxf = gao * wt[j] + gan * wmv[j];
+                const double gyf = gbo * wt[j] + gbn * wmv[j];
+                const double cf = in ? cfI : (wall[j] ? 0.0 : nuTc);
+                const double gX = in ? gxf : gac, gY = in ? gyf : gbc;
+                fl[j] = cf * (gX * nx + gY * ny) * Sf[j];
+            }
+            double fl4[kSlots];
+            gatherq(fl, fl4);
+            double tsrc = 0.0;
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const double t = (!q.inner[s] || q.own[s]) ? tsrc + fl4[s] : tsrc - fl4[s];
+                tsrc = q.valid[s] ? t : tsrc;
+            }
+            sr += tsrc;
+        }
+        const double uc = C(ci, Uc);
+        if (S.momentum_convection != CONV_UPWIND) {
+            const int gx = comp == 0 ? V_gUxx : V_gUyx;
+            const int gy = comp == 0 ? V_gUxy : V_gUyy;
+            const double gxc = C(ci, gx), gyc = C(ci, gy);
+            double co[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+                co[j] = convection_corr(o.f[j], o.own[j], mFlux[j], uc, gxc, gyc, o.n[j], Uc, gx,
+                                        gy, S.momentum_convection);
+            double co4[kSlots];
+            gatherq(co, co4);
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const double t = q.own[s] ? sr - co4[s] : sr + co4[s];
+                sr = q.inner[s] ? t : sr;
+            }
+        }
+        const double alphaU = S.alpha_u;
+        aPu = dg;
+        sr += (1.0 - alphaU) / alphaU * dg * uc;
+        dg /= alphaU;
+        if (qi == 0) {
+            C(ci, V_diag) = dg;
+            C(ci, V_src) = sr;
+            if (comp == 0) C(ci, V_aPs) = dg;
+        }
+        ok = isfinite(dg) && isfinite(sr) && isfinite(uc);
+        double xn[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) xn[j] = C(o.n[j], Uc);
+        const double y = row_residual_q(ci, q, o, offv, xn, dg * uc, ok);
+        return sr - y;
+    }
+
+    // turb_faces with QL lanes per cell; each lane keeps its slots' terms
+    struct TurbOwn {
+        double Sf[NJ], delta[NJ], mFlux[NJ], nuEff[NJ], offv[NJ];
+    };
+    DEV void turb_faces_q(int ci, const SlotSet &q, const OwnSlots &o, double sig0, double sig1,
+                          int Bv, int Cv, double F1c, double nuTc, double &dg, double &sr,
+                          TurbOwn &t) const {
+        const double nu = S.nu;
+        const double sgB = blend(sig0, sig1, F1c);
+        const double nuEffB = nu + sgB * nuTc;
+        double cd[NJ], cb[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int f = o.f[j];
+            const double Sf = Fc(f, W_A);
+            const double delta = Fc(f, W_dlt);
+            const double m = Fc(f, W_phi);
+            const double wt = Fc(f, W_w), wmv = Fc(f, W_wm);
+            const double F1n = C(o.n[j], V_F1), ntn_ = C(o.n[j], V_nuT);
+            const double bv = Bd(o.b[j], Bv);
+            const bool own = o.own[j], in = o.in[j];
+            const double F1o = own ? F1c : F1n, F1nb = own ? F1n : F1c;
+            const double nto = own ? nuTc : ntn_, ntn = own ? ntn_ : nuTc;
+            const double F1f = wt * F1o + wmv * F1nb;
+            const double sg = blend(sig0, sig1, F1f);
+            const double nuEff_o = nu + sg * nto;
+            const double nuEff_n = nu + sg * ntn;
+            const double nuEff_f = wt * nuEff_o + wmv * nuEff_n;
+            const double D = (in ? nuEff_f : nuEffB) * Sf / delta;
+            const double cPos = smax(m, 0.0), cNeg = smax(-m, 0.0);
+            const double dIn = own ? D + cPos : D + cNeg;
+            const double oIn = -(own ? D + cNeg : D + cPos);
+            const bool pos = m >= 0;
+            const double dB = pos ? m + D : D;
+            cb[j] = (pos ? D : D - m) * bv;
+            cd[j] = in ? dIn : dB;
+            t.offv[j] = in ? oIn : 0.0;
+            t.Sf[j] = Sf;
+            t.delta[j] = delta;
+            t.mFlux[j] = m;
+            t.nuEff[j] = nuEff_f;
+        }
+        double cd4[kSlots], cb4[kSlots];
+        gatherq(cd, cd4);
+        gatherq(cb, cb4);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            dg = q.valid[s] ? dg + cd4[s] : dg;
+            sr = (q.valid[s] && !q.inner[s]) ? sr + cb4[s] : sr;
+        }
+        if (M.folded) folded_faces(ci, nuEffB, C(ci, Cv), dg, sr);
+    }
+    DEV double turb_convection_q(int ci, const SlotSet &q, const OwnSlots &o, const TurbOwn &t,
+                                 double sr, int ph, int gx, int gy) const {
+        const double phc = C(ci, ph), gxc = C(ci, gx), gyc = C(ci, gy);
+        double co[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+            co[j] = convection_corr(o.f[j], o.own[j], t.mFlux[j], phc, gxc, gyc, o.n[j], ph, gx,
+                                    gy, S.turbulence_convection);
+        double co4[kSlots];
+        gatherq(co, co4);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const double v = q.own[s] ? sr - co4[s] : sr + co4[s];
+            sr = q.inner[s] ? v : sr;
+        }
+        return sr;
+    }
+
+    // k_row with QL lanes per cell
+    DEV void k_row_q(int ci, bool &ok, double &resid) {
+        SlotSet q;
+        slot_set(ci, q);
+        OwnSlots o;
+        own_slots(q, o);
+        double dg = 0.0, sr = 0.0;
+        const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
+        TurbOwn t;
+        turb_faces_q(ci, q, o, c[0], c[4], B_k, V_k, F1c, nuTc, dg, sr, t);
+        const double V = C(ci, V_vol);
+        sr += C(ci, V_Pk) * V;
+        const double destruction = c[8] * smax(C(ci, V_w), 1e-20);
+        dg += destruction * V;
+        if (S.turbulence_convection != CONV_UPWIND)
+            sr = turb_convection_q(ci, q, o, t, sr, V_k, V_gKx, V_gKy);
+        const double aK = S.alpha_k;
+        const double kc = C(ci, V_k);
+        sr += (1.0 - aK) / aK * dg * kc;
+        dg /= aK;
+        if (qi == 0) {
+            C(ci, V_diag) = dg;
+            C(ci, V_src) = sr;
+        }
+        ok = isfinite(dg) && isfinite(sr) && isfinite(kc);
+        double xn[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) xn[j] = C(o.n[j], V_k);
+        const double y = row_residual_q(ci, q, o, t.offv, xn, dg * kc, ok);
+        resid = sr - y;
+    }
+
+    // omega_row with QL lanes per cell
+    DEV double omega_row_q(int ci, bool &ok, double &resid) {
+        const double nu = S.nu;
+        SlotSet q;
+        slot_set(ci, q);
+        OwnSlots o;
+        own_slots(q, o);
+        double dg = 0.0, sr = 0.0, balance = 0.0;
+        const double amplitude = 6 * nu / c[2];
+        const double F1c = C(ci, V_F1), nuTc = C(ci, V_nuT);
+        TurbOwn t;
+        turb_faces_q(ci, q, o, c[1], c[5], B_w, V_w, F1c, nuTc, dg, sr, t);
+        if (planar) {
+            const double omsc = C(ci, V_oms);
+            double co[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const bool own = o.own[j];
+                const double omsn = C(o.n[j], V_oms);
+                const double omg = Fc(o.f[j], W_omg);
+                const double so = own ? omsc : omsn, sn = own ? omsn : omsc;
+                const double difference = (sn - so) / t.delta[j];
+                const double nuEff_f = t.nuEff[j];
+                co[j] = amplitude * t.Sf[j] * ((nuEff_f - nu) * omg - nuEff_f * difference);
+            }
+            double co4[kSlots];
+            gatherq(co, co4);
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const double v = q.own[s] ? balance + co4[s] : balance - co4[s];
+                balance = q.inner[s] ? v : balance;
+            }
+        }
+        const double V = C(ci, V_vol);
+        const double wc = C(ci, V_w);
+        const double omC = smax(wc, 1e-20);
+        const double alphaB = blend(c[3], c[7], F1c);
+        const double production = sst_pomega(nuTc, C(ci, V_S), C(ci, V_k), wc);
+        sr += alphaB * production * V;
+        const double betaB = blend(c[2], c[6], F1c);
+        dg += betaB * omC * V;
+        sr += (1.0 - F1c) * C(ci, V_CDkw) * V;
+        if (planar) {
+            const double correction = balance + nu * (6 * nu / c[2]) * C(ci, V_oml) * V;
+            if (correction < 0) dg -= correction / omC;
+            else sr += correction;
+        }
+        if (S.turbulence_convection != CONV_UPWIND)
+            sr = turb_convection_q(ci, q, o, t, sr, V_w, V_gWx, V_gWy);
+        const double aW = S.alpha_omega;
+        sr += (1.0 - aW) / aW * dg * wc;
+        dg /= aW;
+        const double wvc = C(ci, V_wallv);
+        double x0n[NJ], offv[NJ], pv[NJ];
+        int sb[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const bool own = o.own[j], in = o.in[j];
+            const double vn_ = C(o.n[j], V_wallv);
+            const double wn_ = C(o.n[j], V_w);
+            x0n[j] = isfinite(vn_) ? vn_ : wn_;
+            const double vo = own ? wvc : vn_, vn = own ? vn_ : wvc;
+            const bool fo = isfinite(vo), fn = isfinite(vn);
+            const bool sub = in && (fo ? !own : (fn && own));
+            const double v = fo ? vo : vn;
+            pv[j] = t.offv[j] * v;
+            sb[j] = sub ? 1 : 0;
+            offv[j] = (in && (fo || fn)) ? 0.0 : t.offv[j];
+        }
+        double pv4[kSlots];
+        int sb4[kSlots];
+        gatherq(pv, pv4);
+        gatherqi(sb, sb4);
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) sr = sb4[s] ? sr - pv4[s] : sr;
+        if (isfinite(wvc)) sr = dg * wvc;
+        if (qi == 0) {
+            C(ci, V_diag) = dg;
+            C(ci, V_src) = sr;
+        }
+        const double x0 = isfinite(wvc) ? wvc : wc;
+        ok = isfinite(dg) && isfinite(sr) && isfinite(x0);
+        const double y = row_residual_q(ci, q, o, offv, x0n, dg * x0, ok);
         resid = sr - y;
         return x0;
     }
@@ -2688,10 +3089,7 @@ struct Draw {
             Krylov rUx, rUy;
             {
                 double a[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int ci = gt; ci < nc; ci += gs) {
-                    double aPu;
-                    bool ok;
-                    const double r = momentum_row(ci, 0, aPu, ok);
+                auto xrow = [&](int ci, double r, double aPu, bool ok) {
                     const double ux = C(ci, V_Ux), uy = C(ci, V_Uy);
                     C(ci, V_kr) = r;
                     C(ci, V_kx) = ux;
@@ -2700,6 +3098,21 @@ struct Draw {
                     a[1] += contribution;
                     a[2] += ok ? 0.0 : 1.0;
                     a[3] += r * r;
+                };
+                if constexpr (QL > 1) {
+                    for (int ci = ct; ci < nc; ci += cs) {
+                        double aPu;
+                        bool ok;
+                        const double r = momentum_row_q(ci, 0, aPu, ok);
+                        if (qi == 0) xrow(ci, r, aPu, ok);
+                    }
+                } else {
+                    for (int ci = gt; ci < nc; ci += gs) {
+                        double aPu;
+                        bool ok;
+                        const double r = momentum_row(ci, 0, aPu, ok);
+                        xrow(ci, r, aPu, ok);
+                    }
                 }
                 red.sum(a);
                 momResid = a[0];
@@ -2724,15 +3137,27 @@ struct Draw {
                 }
                 sync();
                 double a[3] = {0.0, 0.0, 0.0};
-                for (int ci = gt; ci < nc; ci += gs) {
-                    double aPu;
-                    bool ok;
-                    const double r = momentum_row(ci, 1, aPu, ok);
+                auto yrow = [&](int ci, double r, bool ok) {
                     C(ci, V_kr) = r;
                     C(ci, V_kx) = C(ci, V_Uy);
                     a[0] += fabs(r);
                     a[1] += ok ? 0.0 : 1.0;
                     a[2] += r * r;
+                };
+                if constexpr (QL > 1) {
+                    for (int ci = ct; ci < nc; ci += cs) {
+                        double aPu;
+                        bool ok;
+                        const double r = momentum_row_q(ci, 1, aPu, ok);
+                        if (qi == 0) yrow(ci, r, ok);
+                    }
+                } else {
+                    for (int ci = gt; ci < nc; ci += gs) {
+                        double aPu;
+                        bool ok;
+                        const double r = momentum_row(ci, 1, aPu, ok);
+                        yrow(ci, r, ok);
+                    }
                 }
                 red.sum(a);
                 momResid = smax(momResid, a[0]);
@@ -2868,14 +3293,26 @@ struct Draw {
             // ---- turbulence transport (SIMPLESolver.cpp:1476-1538)
             if (turbUpdate) {
                 double a9[2] = {0.0, 0.0};
-                for (int ci = gt; ci < nc; ci += gs) {
-                    bool ok;
-                    double r;
-                    k_row(ci, ok, r);
+                auto krow = [&](int ci, double r, bool ok) {
                     C(ci, V_kx) = C(ci, V_k);
                     C(ci, V_kr) = r;
                     a9[1] += ok ? 0.0 : 1.0;
                     a9[0] += r * r;
+                };
+                if constexpr (QL > 1) {
+                    for (int ci = ct; ci < nc; ci += cs) {
+                        bool ok;
+                        double r;
+                        k_row_q(ci, ok, r);
+                        if (qi == 0) krow(ci, r, ok);
+                    }
+                } else {
+                    for (int ci = gt; ci < nc; ci += gs) {
+                        bool ok;
+                        double r;
+                        k_row(ci, ok, r);
+                        krow(ci, r, ok);
+                    }
                 }
                 red.sum(a9);
                 mark(8);
@@ -2896,14 +3333,26 @@ struct Draw {
                 bc_k();
                 sync();
                 double a10[2] = {0.0, 0.0};
-                for (int ci = gt; ci < nc; ci += gs) {
-                    bool ok;
-                    double r;
-                    const double x0 = omega_row(ci, ok, r);
+                auto wrow = [&](int ci, double x0, double r, bool ok) {
                     C(ci, V_kx) = x0;
                     C(ci, V_kr) = r;
                     a10[1] += ok ? 0.0 : 1.0;
                     a10[0] += r * r;
+                };
+                if constexpr (QL > 1) {
+                    for (int ci = ct; ci < nc; ci += cs) {
+                        bool ok;
+                        double r;
+                        const double x0 = omega_row_q(ci, ok, r);
+                        if (qi == 0) wrow(ci, x0, r, ok);
+                    }
+                } else {
+                    for (int ci = gt; ci < nc; ci += gs) {
+                        bool ok;
+                        double r;
+                        const double x0 = omega_row(ci, ok, r);
+                        wrow(ci, x0, r, ok);
+                    }
                 }
                 red.sum(a10);
                 mark(10);
