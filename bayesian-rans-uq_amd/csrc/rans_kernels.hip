@@ -1277,6 +1277,70 @@ struct Draw {
         return sr;
     }
 
+    // The gradients and SST closure of a turbulence update with the two lanes
+    // of a pair (SSTModel::computeFields, SSTModel.cpp:208-240; the startup
+    // nuT floor, SIMPLESolver.cpp:1333-1337): lane 0 takes the Green-Gauss
+    // gradients of Ux, Uy and p, lane 1 those of k and omega (each field's
+    // sum is the one-lane sum); the F1 and F2 arguments share their common
+    // terms and one tanh evaluates F1 in lane 0 and F2 in lane 1. Every value
+    // is computed with the operations of the one-lane loop.
+    DEV void turbulence_fields_pair(int ci, double nuTMin) {
+        const bool l0 = qi == 0;
+        const int ph[3] = {l0 ? V_Ux : V_k, l0 ? V_Uy : V_w, l0 ? V_p : V_k};
+        const int pb[3] = {l0 ? B_Ux : B_k, l0 ? B_Uy : B_w, l0 ? B_p : B_k};
+        double gx[3], gy[3];
+        gg_cell<3>(ci, ph, pb, gx, gy);
+        // lane 0 receives lane 1's k and omega gradients
+        const double gkx = lane_bcd<1>(gx[0]), gky = lane_bcd<1>(gy[0]);
+        const double gwx = lane_bcd<1>(gx[1]), gwy = lane_bcd<1>(gy[1]);
+        if (l0) {
+            C(ci, V_gUxx) = gx[0];
+            C(ci, V_gUxy) = gy[0];
+            C(ci, V_gUyx) = gx[1];
+            C(ci, V_gUyy) = gy[1];
+            C(ci, V_gPx) = gx[2];
+            C(ci, V_gPy) = gy[2];
+        } else {
+            C(ci, V_gKx) = gx[0];
+            C(ci, V_gKy) = gy[0];
+            C(ci, V_gWx) = gx[1];
+            C(ci, V_gWy) = gy[1];
+        }
+        const double nu = S.nu;
+        const double kc = smax(C(ci, V_k), 0.0);
+        const double wc = smax(C(ci, V_w), 1e-20);
+        const double y = smax(C(ci, V_wd), 1e-20);
+        const double omSafe = smax(wc, 1e-20);
+        const double cd = 2.0 * c[5] / omSafe * (gkx * gwx + gky * gwy);
+        // sst_f1 in lane 0, sst_f2 in lane 1, term by term
+        const double sqrtK = sqrt(smax(kc, 0.0));
+        const double om2 = smax(wc, 1e-20);
+        const double ySafe = smax(y, 1e-20);
+        const double y2s = ySafe * ySafe;
+        const double term1 = (l0 ? sqrtK : 2.0 * sqrtK) / (c[8] * om2 * ySafe);
+        const double term2 = 500.0 * nu / (y2s * om2);
+        const double CDpos = smax(cd, 1e-10);
+        const double term3 = 4.0 * c[5] * smax(kc, 0.0) / (CDpos * y2s);
+        const double arg1 = smin(smax(term1, term2), term3);
+        const double arg2 = smax(term1, term2);
+        const double a = l0 ? arg1 * arg1 * arg1 * arg1 : arg2 * arg2;
+        const double t = tanh(a);
+        const double f1 = t;                 // lane 0's
+        const double f2 = lane_bcd<1>(t);    // lane 1's
+        if (l0) {
+            const double S11 = gx[0], S22 = gy[1];
+            const double S12 = 0.5 * (gy[0] + gx[1]);
+            const double Sc = sqrt(2.0 * (S11 * S11 + S22 * S22 + 2.0 * (S12 * S12)));
+            C(ci, V_S) = Sc;
+            C(ci, V_CDkw) = cd;
+            C(ci, V_F1) = (S.sst_variant == 2) ? 1.0 : f1;
+            C(ci, V_F2) = f2;
+            const double nt = sst_nut(kc, wc, Sc, f2);
+            C(ci, V_Pk) = sst_pk(nt, Sc, kc, wc);
+            C(ci, V_nuT) = smax(nt, nuTMin);
+        }
+    }
+
     // k_row with QL lanes per cell
     DEV void k_row_q(int ci, bool &ok, double &resid) {
         SlotSet q;
@@ -3033,6 +3097,9 @@ struct Draw {
             if (turbUpdate) {
                 if (!turbEst && iter >= S.turb_start_iter + S.nut_floor_iters) turbEst = 1;
                 const double nuTMin = turbEst ? 0.0 : 0.1 * nu;
+                if constexpr (QL == 2) {
+                    for (int ci = ct; ci < nc; ci += cs) turbulence_fields_pair(ci, nuTMin);
+                } else
                 for (int ci = gt; ci < nc; ci += gs) {
                     const int ph5[5] = {V_Ux, V_Uy, V_k, V_w, V_p};
                     const int pb5[5] = {B_Ux, B_Uy, B_k, B_w, B_p};
