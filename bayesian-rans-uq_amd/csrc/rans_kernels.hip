@@ -305,7 +305,9 @@ struct Krylov {
     double initial;
 };
 
-template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false>
+// kServe: the serving kernel, whose problems choose their Krylov precision
+// at run time (the batch kernels are compiled for one precision each).
+template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false, bool kServe = false>
 struct Draw {
     // lanes per cell of the momentum, k and omega row assembly: two in the
     // 256-thread LDS kernels (an LDS arena holds at most about 130 cells, so
@@ -2055,7 +2057,9 @@ struct Draw {
 
     template <int CPL>
     DEV Krylov solve_wave(int kind, double rr, bool zeroStart) {
-        if (S.mixed_precision) {
+        bool f32 = kF32;
+        if constexpr (kServe) f32 = S.mixed_precision != 0;
+        if (f32) {
             if (kind == SOLVER_PCG && zeroStart)
                 return pcg_wave32<CPL>(rr, S.inner_iterations, S.inner_tolerance);
             return bicgstab_wave32<CPL>(rr, S.inner_iterations, S.inner_tolerance);
@@ -3636,7 +3640,7 @@ struct BlockShared {
 // rank/nblk/grid/aux describe a multi-workgroup draw (kGrid): its workgroups
 // share the arena and meet at grid barriers; aux is this workgroup's LDS for
 // its bookkeeping replica (contraction rings and monitor values).
-template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false>
+template <int BS, bool kLDS, bool kGrid = false, bool kF32 = false, bool kServe = false>
 DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &settings,
                     const double *coeffSrc, double *arena, double *amg, DevResult *res,
                     double *pred, double *fields, GridState *grid = nullptr, int rank = 0,
@@ -3677,7 +3681,7 @@ DEV void solve_draw(BlockShared<BS> &sh, const DevMesh &M, const DevSettings &se
     } else {
         __syncthreads();
     }
-    Draw<BS, kLDS, kGrid, kF32> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
+    Draw<BS, kLDS, kGrid, kF32, kServe> d(M, sh.sset, sh.coeffs, arena, &sh.book, sh.redbuf, &sh.kslot, amg,
                             aux);
     d.cbuf = sh.coarse_b;
     d.warm = warm;
@@ -3878,11 +3882,11 @@ __global__ void __launch_bounds__(BS) serve_kernel(const DevProblem *__restrict_
         double *arena = arenas + (int64_t)blockIdx.x * arena_stride;
         double *amg = amgs ? amgs + (int64_t)blockIdx.x * amg_stride : nullptr;
         if (P.lds)
-            solve_draw<BS, true>(sh, P.mesh, P.settings, coeffs, smem, amg, &o.res, o.predictions,
-                                 nullptr);
+            solve_draw<BS, true, false, false, true>(sh, P.mesh, P.settings, coeffs, smem, amg,
+                                                     &o.res, o.predictions, nullptr);
         else
-            solve_draw<BS, false>(sh, P.mesh, P.settings, coeffs, arena, amg, &o.res,
-                                  o.predictions, nullptr);
+            solve_draw<BS, false, false, false, true>(sh, P.mesh, P.settings, coeffs, arena, amg,
+                                                      &o.res, o.predictions, nullptr);
         // every thread's result stores reach host memory before the
         // completion number is published
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
