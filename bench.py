@@ -736,6 +736,36 @@ def calibration_leg():
                          "minutes on four threads (its 3x128 periodic channel)"}
 
 
+PHASE_NAMES = ("gradients + SST closure", "x-momentum rows", "x solve", "y-momentum rows",
+               "y solve", "Rhie-Chow and p' rows", "p' solve", "corrections", "k rows",
+               "k solve", "omega rows", "omega solve", "norms", "bookkeeping")
+
+
+def phase_clocks(block_size, max_iterations, mixed):
+    """Per-phase shader clocks of draw 0 of the sweep (rq_settings.profile):
+    one draw, capped at max_iterations outer iterations, one JSON line."""
+    from cfdr_amd import cases, engine
+    prob = cases.channel_problem(max_iterations=max_iterations)
+    theta = cases.channel_draws(64)[:1]
+    b = engine.ForwardBatch(prob.mesh, prob.params, prob.obs, prob.bcs, prob.nu, prob.settings,
+                            prob.u_init, prob.p_init, prob.k_init, prob.omega_init,
+                            block_size=block_size, profile=True, mixed_precision=mixed)
+    try:
+        r = b.evaluate(theta)
+    finally:
+        b.close()
+    it = int(r.iteration_counts[0])
+    cyc = np.asarray(r.phase_cycles[0][:len(PHASE_NAMES)], dtype=float)
+    print(json.dumps({"workload": "draw 0 of the 64-draw channel sweep, 1x128 cells",
+                      "block_size": block_size, "mixed_precision": bool(mixed),
+                      "iterations": it, "kernel_ms": round(float(r.kernel_ms), 2),
+                      "us_per_iteration": round(float(r.kernel_ms) * 1e3 / it, 2),
+                      "clocks_per_iteration": round(float(cyc.sum()) / it),
+                      "krylov_iterations": [int(v) for v in r.krylov_iterations[0]],
+                      "phases": {n: round(float(c) / it) for n, c in zip(PHASE_NAMES, cyc)}}))
+    return 0
+
+
 def _leg(name, fn, *args):
     """Run one auxiliary leg. An exception raised by the leg's own logic (a
     calibration stage that refuses its grid, a CPU baseline that cannot
@@ -789,9 +819,16 @@ def main():
                          "profiler writes no trace of a process that made cooperative launches)")
     ap.add_argument("--no-calibration", action="store_true",
                     help="skip the staged calibration on the 1x128 channel")
+    ap.add_argument("--phase-clocks", action="store_true",
+                    help="print the per-phase shader clocks of one draw (capped at "
+                         "--max-iterations, default 3000) at --block-size and exit")
+    ap.add_argument("--mixed", action="store_true",
+                    help="with --phase-clocks: FP32 Krylov with FP64 defect correction")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.phase_clocks:
+        return phase_clocks(args.block_size or 256, args.max_iterations or 3000, args.mixed)
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(draws are independent: several GPUs would each run their own batch)",
