@@ -1150,6 +1150,27 @@ static void assemble_momentum(ctx_t *X, int comp) {
     }
 }
 
+/* An experiment on the precision pattern, not part of the reference: with
+ * rounding on, every assembled system (momentum, p', k and omega rows after
+ * relaxation and wall constraints) is rounded to FP32 before it is used,
+ * which is what storing an FP32 assembly costs at the least. Residuals,
+ * norms, Krylov solves and observations stay FP64. Off by default; the flag
+ * is process-wide, for single-threaded experiments. */
+static int g_round_assembly = 0;
+void or_set_assembly_rounding(int on) { g_round_assembly = on != 0; }
+static void round_assembly(lsys_t *A, double *aP) {
+    if (!g_round_assembly) return;
+    for (int i = 0; i < A->n; ++i) {
+        A->diag[i] = (double)(float)A->diag[i];
+        A->source[i] = (double)(float)A->source[i];
+        if (aP) aP[i] = A->diag[i];
+    }
+    for (int f = 0; f < A->nif; ++f) {
+        A->upper[f] = (double)(float)A->upper[f];
+        A->lower[f] = (double)(float)A->lower[f];
+    }
+}
+
 /* interpolatedFluxes (SIMPLESolver.cpp:842-852) */
 static void interpolated_fluxes(const ctx_t *X, double *out) {
     const or_mesh_t *m = X->m;
@@ -1719,6 +1740,7 @@ int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t
         sres_t resUx = {0, 0, 0, 0}, resUy = {0, 0, 0, 0}, resUz = {0, 0, 0, 0};
         /* x momentum */
         assemble_momentum(X, 0);
+        round_assembly(&X->sys, X->aP);
         memcpy(X->solveValues, X->Ux, sizeof(double) * nc);
         ls_residual(&X->sys, X->solveValues, X->rbuf);
         momResid = l1(nc, X->rbuf);
@@ -1738,6 +1760,7 @@ int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t
         memcpy(X->aPstore, X->aP, sizeof(double) * nc);
         /* y momentum */
         assemble_momentum(X, 1);
+        round_assembly(&X->sys, X->aP);
         memcpy(X->solveValues, X->Uy, sizeof(double) * nc);
         ls_residual(&X->sys, X->solveValues, X->rbuf);
         momResid = smax(momResid, l1(nc, X->rbuf));
@@ -1760,6 +1783,7 @@ int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t
         for (int fi = 0; fi < nif; ++fi) X->phiPred[fi] += relaxCorr * (X->phi[fi] - X->phiBarOld[fi]);
         assemble_pressure(X, X->aPstore, X->phiPred);
         const double contScaled = l1(nc, X->sys.source) / smax(X->massFluxScale, 1e-300);
+        round_assembly(&X->sys, NULL);
         for (int i = 0; i < nc; ++i) X->solveValues[i] = 0.0;
         sres_t resP;
         if (linear_solve(st->pressure_solver, &X->amg, &X->sys, X->solveValues, st->inner_iterations,
@@ -1800,6 +1824,7 @@ int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t
             memcpy(X->kOld, X->k, sizeof(double) * nc);
             memcpy(X->wOld, X->w, sizeof(double) * nc);
             assemble_k(X);
+            round_assembly(&X->sys, NULL);
             memcpy(X->solveValues, X->k, sizeof(double) * nc);
             sres_t resK;
             if (linear_solve(st->turbulence_solver, &X->amg, &X->sys, X->solveValues,
@@ -1816,6 +1841,7 @@ int or_evaluate_ex(const or_mesh_t *mesh, const or_bc_t *bc, const or_settings_t
             assemble_omega(X);
             omega_wall_values(X);
             constrain_omega(X);
+            round_assembly(&X->sys, NULL);
             memcpy(X->solveValues, X->w, sizeof(double) * nc);
             for (int ci = 0; ci < nc; ++ci)
                 if (isfinite(X->wallOmega[ci])) X->solveValues[ci] = X->wallOmega[ci];

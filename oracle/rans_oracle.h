@@ -135,6 +135,10 @@ double or_sst_pomega(const double c[11], double nut, double s, double k, double 
 
 /* Linear-solver entry points on an LDU system, exported so the Krylov and
  * AMG restatements can be checked against exact solutions. */
+/* Experiment switch (off by default, process-wide): round every assembled
+ * system to FP32 before it is used; residuals, norms and solves stay FP64. */
+void or_set_assembly_rounding(int on);
+
 int or_linear_solve(int solver, int n, int nif, const int *own, const int *nbr, const double *diag,
                     const double *upper, const double *lower, const double *source, double *x,
                     int max_iter, double tol, int *iterations, double *initial_res,
