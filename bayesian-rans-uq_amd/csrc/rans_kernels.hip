@@ -2807,6 +2807,19 @@ struct Draw {
     DEV Krylov solve(int kind, double rr, bool zeroStart, int dst = V_kx) {
         if constexpr (BS <= 256 && !kGrid) {
             if (nc <= 256 && !(kind == SOLVER_AMG && zeroStart && M.amg.levels > 0)) {
+                // Every wave-resident solver stops before its first iteration
+                // when |r| < 1e-30 (x keeps its start value); rr is the block
+                // reduction's value, the same in every thread, so the whole
+                // workgroup takes that exit here without the detour through
+                // wave 0.
+                const double r0 = sqrt(rr);
+                if (r0 < 1e-30) {
+                    if (dst != V_kx) {
+                        for (int ci = tid; ci < nc; ci += BS) C(ci, dst) = C(ci, V_kx);
+                        __syncthreads();
+                    }
+                    return Krylov{0, 1, r0};
+                }
                 xdst = dst;
                 if (tid < 64) {
                     Krylov r;
