@@ -340,14 +340,18 @@ def copy_calibration(mib):
 HILL_MESHES = ((60, 32), (120, 64), (240, 128))
 
 
-def served_leg(n_requests, rate, hill_cap=0):
+def served_leg(n_requests, rate, hill_cap=0, plain_check=4):
     """The served stream: seeded Poisson arrivals of single-solve requests on
     1D channels of 64/128/256/512 cells (one first-cell height), submitted at
     their arrival times to the device request loop; latency = completion -
     arrival. The plain path serves the same arrivals one launch at a time, in
     order, with each mesh's solve time measured alone on the same GPU. With
     hill_cap > 0 the meshes also include the 60x32, 120x64 and 240x128 hills,
-    whose solves are capped at hill_cap outer iterations."""
+    whose solves are capped at hill_cap outer iterations. The plain path's
+    latencies come from that model (queueing in arrival order, each mesh's
+    solve time measured once); its first plain_check requests are also
+    served for real, one launch each at their arrival times, and reported
+    beside the model's latencies for the same requests."""
     from cfdr_amd import cases, engine
     from cfdr_amd.serve import Server
     keys = list(cases.SERVED_CHANNELS)
@@ -412,6 +416,34 @@ def served_leg(n_requests, rate, hill_cap=0):
         plat.append(end - ta)
     plat = np.array(plat)
     pspan = end - stream[0][0]
+    check = None
+    if plain_check > 0:
+        # the first requests served for real on the plain path: engines made
+        # before the clock starts, one launch per request in arrival order
+        sub = stream[:plain_check]
+        engines = {n: engine.ForwardBatch(probs[n].mesh, probs[n].params, probs[n].obs,
+                                          probs[n].bcs, probs[n].nu, probs[n].settings,
+                                          probs[n].u_init, probs[n].p_init, probs[n].k_init,
+                                          probs[n].omega_init) for n in {m for _, m, _ in sub}}
+        measured = []
+        try:
+            t0 = time.perf_counter()
+            for ta, n, th in sub:
+                while time.perf_counter() - t0 < ta:
+                    time.sleep(1e-3)
+                engines[n].evaluate(np.asarray([row(n, th)]))
+                measured.append(time.perf_counter() - t0 - ta)
+        finally:
+            for b in engines.values():
+                b.close()
+        model = plat[:plain_check]
+        check = {"requests": [str(n) for _, n, _ in sub],
+                 "measured_ms": [round(1e3 * v, 1) for v in measured],
+                 "model_ms": [round(1e3 * float(v), 1) for v in model],
+                 "max_rel_diff": round(float(np.max(np.abs(np.asarray(measured) - model) /
+                                                    model)), 4)}
+        print(f"# served: plain check on {plain_check} requests, max rel diff "
+              f"{check['max_rel_diff']}", file=sys.stderr, flush=True)
     by_status = {}
     for k in done:
         name = done[k][2].status.name
@@ -432,7 +464,8 @@ def served_leg(n_requests, rate, hill_cap=0):
                                      "p99": round(1e3 * float(np.percentile(plat, 99)), 1)},
                       "solve_s_by_cells": {str(n): round(v, 3) for n, v in svc.items()},
                       "note": "same arrivals served in order, one launch each; each mesh's "
-                              "solve time measured alone"},
+                              "solve time measured alone",
+                      "measured_check": check},
             "speedup_vs_plain": round((len(stream) / span) / (len(stream) / pspan), 2),
             "note": "tol 1e-8 and the 150000-iteration cap of the single-solve channel: the "
                     "256- and 512-cell channels end Unconverged at the cap on every path"}
@@ -799,6 +832,9 @@ def main():
     ap.add_argument("--serve-hills", type=int, default=0, metavar="CAP",
                     help="add the 60x32/120x64/240x128 hills to the served stream, their solves "
                          "capped at CAP outer iterations")
+    ap.add_argument("--serve-plain-check", type=int, default=4, metavar="K",
+                    help="also serve the stream's first K requests on the plain path for real "
+                         "and report them beside the plain path's model (0: skip)")
     ap.add_argument("--max-iterations", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-hills", action="store_true")
@@ -973,7 +1009,7 @@ def main():
     served = None
     if not args.no_serve:
         served = _leg("served", served_leg, args.serve_requests, args.serve_rate,
-                      args.serve_hills)
+                      args.serve_hills, args.serve_plain_check)
 
     warm = None
     if not args.no_warm:
