@@ -2820,16 +2820,13 @@ struct Draw {
                     }
                     return Krylov{0, 1, r0};
                 }
+                // past that exit every solver iterates and stores x in dst
                 xdst = dst;
                 if (tid < 64) {
                     Krylov r;
                     if (nc <= 64) r = solve_wave<1>(kind, rr, zeroStart);
                     else if (nc <= 128) r = solve_wave<2>(kind, rr, zeroStart);
                     else r = solve_wave<4>(kind, rr, zeroStart);
-                    // a solve that stops at a zero residual stores nothing: x
-                    // is its start value
-                    if (dst != V_kx && r.initial < 1e-30)
-                        for (int ci = tid; ci < nc; ci += 64) C(ci, dst) = C(ci, V_kx);
                     if (tid == 0) *kres = r;
                 }
                 xdst = V_kx;
