@@ -933,6 +933,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": _traffic_from_profile(),
+                "traffic_note": ("HBM bytes of one launch of this kernel at 256 threads, from "
+                                 "rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB copy "
+                                 "(profiles/); achieved counts algorithmic bytes, and the draws' "
+                                 "state stays in LDS"),
                 "kernel": "simple_solve_kernel", "kernel_ms": round(1e3 * kernel_avg_s, 3),
                 "algorithmic_bytes_per_launch": int(launch_bytes)}
 
